@@ -1,0 +1,106 @@
+// Host-side core: error strings, device allocation, FFT plan construction.
+#include "fxc_common.hpp"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <vector>
+
+namespace fxc {
+
+static thread_local char g_err[1024] = {0};
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+const char *last_error() { return g_err; }
+
+int dmalloc(void **ptr, size_t bytes) {
+  *ptr = nullptr;
+  if (bytes == 0) return FXC_OK;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    return FXC_ERR_ALLOC;
+  }
+  return FXC_OK;
+}
+
+void dfree(void *ptr) {
+  if (ptr) (void)hipFree(ptr);
+}
+
+// Radix preference: large butterflies first keeps the pass count low; the
+// odd primes 23 and 29 cover the correlation lengths 2001 = 3*23*29.
+bool factor_radices(int n, int *radix, int *npass) {
+  static const int prefs[] = {8, 7, 5, 4, 3, 2, 23, 29};
+  int m = n;
+  int cnt = 0;
+  if (n < 1) return false;
+  for (int r : prefs) {
+    while (m % r == 0 && m > 1) {
+      if (cnt >= kMaxPasses) return false;
+      radix[cnt++] = r;
+      m /= r;
+    }
+  }
+  if (m != 1) return false;
+  if (cnt == 0) {  // n == 1: a single trivial pass is not needed
+    *npass = 0;
+    return true;
+  }
+  *npass = cnt;
+  return true;
+}
+
+int fft_plan_create(FftPlan *plan, int n, int sign) {
+  FftPlanDev d{};
+  d.n = n;
+  if (!factor_radices(n, d.radix, &d.npass)) {
+    set_error("fft length %d has a prime factor without a butterfly", n);
+    return FXC_ERR_UNSUPPORTED;
+  }
+  std::vector<float2> tw;
+  int span = n;
+  for (int p = 0; p < d.npass; ++p) {
+    const int r = d.radix[p];
+    const int stride = span / r;
+    d.span[p] = span;
+    d.stride[p] = stride;
+    d.tw_off[p] = (int)tw.size();
+    const double base = sign * 2.0 * M_PI / (double)span;
+    for (int j = 0; j < stride; ++j) {
+      for (int q = 1; q < r; ++q) {
+        // reduce the integer phase before converting to an angle
+        const long long ph = ((long long)j * q) % (long long)span;
+        const double a = base * (double)ph;
+        tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+      }
+    }
+    span = stride;
+  }
+  plan->dev = d;
+  plan->sign = sign;
+  plan->tw_count = tw.size();
+  plan->d_tw = nullptr;
+  if (!tw.empty()) {
+    FXC_TRY(dmalloc((void **)&plan->d_tw, tw.size() * sizeof(float2)));
+    FXC_HIP(hipMemcpy(plan->d_tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  return FXC_OK;
+}
+
+void fft_plan_destroy(FftPlan *plan) {
+  if (!plan) return;
+  dfree(plan->d_tw);
+  plan->d_tw = nullptr;
+}
+
+}  // namespace fxc
+
+extern "C" const char *fxc_last_error(void) { return fxc::last_error(); }

@@ -1,0 +1,173 @@
+// Batched FFT entry points (complex and real) over the workgroup LDS FFT.
+//
+// These replace the cuFFT plans of the reference stages: R2C/C2R of the
+// window, prefilter and output lengths in sac2spec
+// (native/sac2spec/device_memory.cu:23-32), the C2R of xc_fast
+// (native/xc/memory/memory.cu:478-483) and the C2C plans of ncf_pws /
+// ncf_tfpws (native/pws/worker/gpu.cu:197-227,
+// native/tfpws/core/tfpws_workspace.cu:190-235).  Conventions follow the
+// DFT definition: forward exp(-i...), inverse exp(+i...), both unnormalised,
+// R2C returns bins 0..n/2.
+#include "fxc_fft_real.hpp"
+#include "fxc_plan_cache.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace fxc {
+
+int real_fft_plan_create(RealFftPlan *plan, int n, int sign) {
+  if (n < 2 || (n & 1)) {
+    set_error("real fft length %d must be even and >= 2", n);
+    return FXC_ERR_ARG;
+  }
+  plan->n = n;
+  FXC_TRY(fft_plan_create(&plan->half, n / 2, sign));
+  const int M = n / 2;
+  std::vector<float2> fold(M + 1);
+  for (int k = 0; k <= M; ++k) {
+    const double a = sign * 2.0 * M_PI * (double)k / (double)n;
+    fold[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  FXC_TRY(dmalloc((void **)&plan->d_fold, fold.size() * sizeof(float2)));
+  FXC_HIP(hipMemcpy(plan->d_fold, fold.data(), fold.size() * sizeof(float2), hipMemcpyHostToDevice));
+  return FXC_OK;
+}
+
+void real_fft_plan_destroy(RealFftPlan *plan) {
+  fft_plan_destroy(&plan->half);
+  dfree(plan->d_fold);
+  plan->d_fold = nullptr;
+}
+
+int max_transforms_per_wg(const FftPlanDev &plan, int nthreads, int lds_complex_capacity) {
+  (void)nthreads;  // threads loop over butterflies; only LDS capacity limits
+  return lds_complex_capacity / (plan.n > 0 ? plan.n : 1);
+}
+
+constexpr int kFftThreads = 1024;
+constexpr int kFftThreadsOddPrime = 256;
+
+template <unsigned kMask, int kThreads>
+__global__ void __launch_bounds__(kThreads)
+c2c_batch_kernel(const float2 *__restrict__ in, long long in_dist, float2 *__restrict__ out,
+                 long long out_dist, long long ntr, int per_wg, FftPlanDev plan,
+                 const float2 *__restrict__ tw, float sgn, float scale) {
+  extern __shared__ float2 lds[];
+  const long long t0 = (long long)blockIdx.x * per_wg;
+  const int nt = (int)((ntr - t0) < per_wg ? (ntr - t0) : per_wg);
+  const int n = plan.n;
+  for (int i = threadIdx.x; i < nt * n; i += blockDim.x) {
+    const int t = i / n, j = i - t * n;
+    lds[i] = in[(t0 + t) * in_dist + j];
+  }
+  __syncthreads();
+  lds_fft<kMask>(lds, nt, plan, tw, sgn);
+  for (int i = threadIdx.x; i < nt * n; i += blockDim.x) {
+    const int t = i / n, j = i - t * n;
+    out[(t0 + t) * out_dist + j] = cscale(lds[t * n + fft_pos(plan, j)], scale);
+  }
+}
+
+__global__ void __launch_bounds__(kFftThreads)
+r2c_batch_kernel(const float *__restrict__ in, long long in_dist, int in_len,
+                 float2 *__restrict__ out, long long out_dist, FftPlanDev plan,
+                 const float2 *__restrict__ tw, const float2 *__restrict__ fold, float scale) {
+  extern __shared__ float2 lds[];
+  const long long row = blockIdx.x;
+  const float *x = in + row * in_dist;
+  float2 *y = out + row * out_dist;
+  const int M = plan.n;
+  real_pack(lds, M, [&](int i) { return i < in_len ? x[i] : 0.f; });
+  lds_fft(lds, 1, plan, tw, -1.f);
+  r2c_unfold(lds, plan, fold, [&](int k, float2 v) { y[k] = cscale(v, scale); });
+}
+
+__global__ void __launch_bounds__(kFftThreads)
+c2r_batch_kernel(const float2 *__restrict__ in, long long in_dist, float *__restrict__ out,
+                 long long out_dist, int out_len, FftPlanDev plan,
+                 const float2 *__restrict__ tw, const float2 *__restrict__ fold, float scale) {
+  extern __shared__ float2 lds[];
+  const long long row = blockIdx.x;
+  const float2 *X = in + row * in_dist;
+  float *y = out + row * out_dist;
+  const int M = plan.n;
+  c2r_fold(lds, M, fold, [&](int k) { return X[k]; });
+  lds_fft(lds, 1, plan, tw, 1.f);
+  c2r_unpack(lds, plan, [&](int i, float v) {
+    if (i < out_len) y[i] = v * scale;
+  });
+}
+
+int set_max_lds(const void *kernel, size_t bytes) {
+  FXC_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return FXC_OK;
+}
+
+}  // namespace fxc
+
+using namespace fxc;
+
+extern "C" int fxc_fft_c2c(const float *d_in, long long in_dist, float *d_out, long long out_dist,
+                           int n, long long batch, int inverse, float scale, void *stream) {
+  FXC_CHECK_ARG(d_in && d_out && n > 0 && batch >= 0, "fxc_fft_c2c: invalid arguments");
+  if (batch == 0) return FXC_OK;
+  const FftPlan *p = nullptr;
+  FXC_TRY(plan_cache_complex(n, inverse ? 1 : -1, &p));
+  const bool smooth = plan_in_mask(p->dev, kRadixSmooth);
+  const int nthr = smooth ? kFftThreads : kFftThreadsOddPrime;
+  const int per = max_transforms_per_wg(p->dev, nthr, kLdsComplexCapacity);
+  FXC_CHECK_ARG(per >= 1, "fxc_fft_c2c: length %d does not fit one workgroup", n);
+  const size_t lds = (size_t)per * n * sizeof(float2);
+  const unsigned grid = (unsigned)ceil_div((size_t)batch, (size_t)per);
+  if (smooth) {
+    FXC_TRY(set_max_lds((const void *)c2c_batch_kernel<kRadixSmooth, kFftThreads>, lds));
+    hipLaunchKernelGGL((c2c_batch_kernel<kRadixSmooth, kFftThreads>), dim3(grid), dim3(nthr), lds,
+                       (hipStream_t)stream, (const float2 *)d_in, in_dist, (float2 *)d_out, out_dist,
+                       batch, per, p->dev, p->d_tw, inverse ? 1.f : -1.f, scale);
+  } else {
+    FXC_TRY(set_max_lds((const void *)c2c_batch_kernel<kRadixAll, kFftThreadsOddPrime>, lds));
+    hipLaunchKernelGGL((c2c_batch_kernel<kRadixAll, kFftThreadsOddPrime>), dim3(grid), dim3(nthr),
+                       lds, (hipStream_t)stream, (const float2 *)d_in, in_dist, (float2 *)d_out,
+                       out_dist, batch, per, p->dev, p->d_tw, inverse ? 1.f : -1.f, scale);
+  }
+  FXC_HIP(hipGetLastError());
+  return FXC_OK;
+}
+
+extern "C" int fxc_fft_r2c(const float *d_in, long long in_dist, int in_len, float *d_out,
+                           long long out_dist, int n, long long batch, float scale, void *stream) {
+  FXC_CHECK_ARG(d_in && d_out && n >= 2 && (n % 2) == 0 && batch >= 0 && in_len >= 0,
+                "fxc_fft_r2c: invalid arguments");
+  if (batch == 0) return FXC_OK;
+  const RealFftPlan *p = nullptr;
+  FXC_TRY(plan_cache_real(n, -1, &p));
+  FXC_CHECK_ARG(max_transforms_per_wg(p->half.dev, kFftThreads, kLdsComplexCapacity) >= 1,
+                "fxc_fft_r2c: length %d does not fit one workgroup", n);
+  const size_t lds = (size_t)(n / 2) * sizeof(float2);
+  FXC_TRY(set_max_lds((const void *)r2c_batch_kernel, lds));
+  hipLaunchKernelGGL(r2c_batch_kernel, dim3((unsigned)batch), dim3(kFftThreads), lds,
+                     (hipStream_t)stream, d_in, in_dist, in_len, (float2 *)d_out, out_dist,
+                     p->half.dev, p->half.d_tw, p->d_fold, scale);
+  FXC_HIP(hipGetLastError());
+  return FXC_OK;
+}
+
+extern "C" int fxc_fft_c2r(const float *d_in, long long in_dist, float *d_out, long long out_dist,
+                           int out_len, int n, long long batch, float scale, void *stream) {
+  FXC_CHECK_ARG(d_in && d_out && n >= 2 && (n % 2) == 0 && batch >= 0 && out_len >= 0 &&
+                    out_len <= n,
+                "fxc_fft_c2r: invalid arguments");
+  if (batch == 0) return FXC_OK;
+  const RealFftPlan *p = nullptr;
+  FXC_TRY(plan_cache_real(n, 1, &p));
+  FXC_CHECK_ARG(max_transforms_per_wg(p->half.dev, kFftThreads, kLdsComplexCapacity) >= 1,
+                "fxc_fft_c2r: length %d does not fit one workgroup", n);
+  const size_t lds = (size_t)(n / 2) * sizeof(float2);
+  FXC_TRY(set_max_lds((const void *)c2r_batch_kernel, lds));
+  hipLaunchKernelGGL(c2r_batch_kernel, dim3((unsigned)batch), dim3(kFftThreads), lds,
+                     (hipStream_t)stream, (const float2 *)d_in, in_dist, d_out, out_dist, out_len,
+                     p->half.dev, p->half.d_tw, p->d_fold, scale);
+  FXC_HIP(hipGetLastError());
+  return FXC_OK;
+}

@@ -1,0 +1,242 @@
+// Workgroup-level mixed-radix FFT staged in LDS (device code).
+//
+// One workgroup owns `ntrans` transforms of length n laid out back to back in
+// LDS.  Passes are in-place decimation-in-frequency butterflies, so a thread
+// holds one butterfly at a time and a single LDS buffer suffices: a
+// 37044-point real transform (an 18522-point complex one, 148 KB) lives in
+// the 160 KB LDS of one CU.  The output stays in digit-reversed order and
+// every consumer reads it through fft_pos(), which the fused kernels need
+// anyway because they only read the bins or lags they keep.
+//
+// Butterflies are written for radices 2, 3, 4, 5, 7, 8 and the odd primes the
+// correlation lengths need (23, 29); all constants are computed at compile
+// time in double precision and rounded once to float.
+#pragma once
+
+#include "fxc_common.hpp"
+
+namespace fxc {
+
+__host__ __device__ constexpr double cx_pi() { return 3.14159265358979323846264338327950288; }
+
+// Taylor series on [-pi, pi]; 40 terms give full double precision there.
+__host__ __device__ constexpr double cx_sin_reduced(double x) {
+  double term = x, sum = x;
+  for (int i = 1; i < 40; ++i) {
+    term *= -x * x / ((2.0 * i) * (2.0 * i + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+__host__ __device__ constexpr double cx_cos_reduced(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int i = 1; i < 40; ++i) {
+    term *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
+    sum += term;
+  }
+  return sum;
+}
+// cos/sin of 2*pi*j/r for integer j, r (argument reduced exactly).
+__host__ __device__ constexpr double cx_cos2pi(int j, int r) {
+  int jj = ((j % r) + r) % r;
+  if (2 * jj > r) jj -= r;
+  return cx_cos_reduced(2.0 * cx_pi() * jj / r);
+}
+__host__ __device__ constexpr double cx_sin2pi(int j, int r) {
+  int jj = ((j % r) + r) % r;
+  if (2 * jj > r) jj -= r;
+  return cx_sin_reduced(2.0 * cx_pi() * jj / r);
+}
+
+template <int R>
+struct RootTable {
+  float c[R];
+  float s[R];
+  constexpr RootTable() : c(), s() {
+    for (int j = 0; j < R; ++j) {
+      c[j] = (float)cx_cos2pi(j, R);
+      s[j] = (float)cx_sin2pi(j, R);
+    }
+  }
+};
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// i * sgn * a
+__device__ __forceinline__ float2 cmul_isgn(float2 a, float sgn) { return make_float2(-sgn * a.y, sgn * a.x); }
+
+// In-place DFT of an odd number R of values:
+//   b_q = sum_m a_m exp(sgn*2*pi*i*m*q/R).
+// Pairs m with R-m so each pair costs one real multiply per constant:
+//   b_q = a_0 + sum_m [t_m cos(2*pi*m*q/R) + i*sgn*u_m sin(2*pi*m*q/R)],
+// with t_m = a_m + a_{R-m} and u_m = a_m - a_{R-m}.
+template <int R>
+__device__ __forceinline__ void dft_generic(float2 *a, float sgn) {
+  static_assert((R & 1) == 1, "generic butterfly is for odd radices");
+  constexpr RootTable<R> tb{};
+  constexpr int H = (R - 1) / 2;
+  float2 t[H + 1], u[H + 1];
+#pragma unroll
+  for (int m = 1; m <= H; ++m) {
+    t[m] = cadd(a[m], a[R - m]);
+    u[m] = csub(a[m], a[R - m]);
+  }
+  float2 b[R];
+  float2 s0 = a[0];
+#pragma unroll
+  for (int m = 1; m <= H; ++m) s0 = cadd(s0, t[m]);
+  b[0] = s0;
+#pragma unroll
+  for (int q = 1; q <= H; ++q) {
+    float2 re = a[0];
+    float2 im = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int m = 1; m <= H; ++m) {
+      const int idx = (m * q) % R;
+      re.x += t[m].x * tb.c[idx];
+      re.y += t[m].y * tb.c[idx];
+      im.x += u[m].x * tb.s[idx];
+      im.y += u[m].y * tb.s[idx];
+    }
+    const float2 iim = cmul_isgn(im, sgn);
+    b[q] = cadd(re, iim);
+    b[R - q] = csub(re, iim);
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) a[q] = b[q];
+}
+
+template <int R>
+__device__ __forceinline__ void dft(float2 *a, float sgn) {
+  dft_generic<R>(a, sgn);
+}
+
+template <>
+__device__ __forceinline__ void dft<2>(float2 *a, float) {
+  const float2 x0 = a[0], x1 = a[1];
+  a[0] = cadd(x0, x1);
+  a[1] = csub(x0, x1);
+}
+
+template <>
+__device__ __forceinline__ void dft<4>(float2 *a, float sgn) {
+  const float2 s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+  const float2 s13 = cadd(a[1], a[3]), d13 = cmul_isgn(csub(a[1], a[3]), sgn);
+  a[0] = cadd(s02, s13);
+  a[2] = csub(s02, s13);
+  a[1] = cadd(d02, d13);
+  a[3] = csub(d02, d13);
+}
+
+template <>
+__device__ __forceinline__ void dft<8>(float2 *a, float sgn) {
+  // 8 = 2 x 4: two radix-4 DFTs on even/odd samples, twiddle, radix-2.
+  float2 e[4] = {a[0], a[2], a[4], a[6]};
+  float2 o[4] = {a[1], a[3], a[5], a[7]};
+  dft<4>(e, sgn);
+  dft<4>(o, sgn);
+  constexpr float r2 = 0.70710678118654752440f;
+  // o[q] *= exp(sgn*2*pi*i*q/8)
+  o[1] = make_float2(r2 * (o[1].x - sgn * o[1].y), r2 * (o[1].y + sgn * o[1].x));
+  o[2] = cmul_isgn(o[2], sgn);
+  o[3] = make_float2(r2 * (-o[3].x - sgn * o[3].y), r2 * (-o[3].y + sgn * o[3].x));
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    a[q] = cadd(e[q], o[q]);
+    a[q + 4] = csub(e[q], o[q]);
+  }
+}
+
+// Largest butterfly count per thread a plan may need (host-side check only;
+// threads loop over their butterflies one at a time).
+constexpr int kMaxElemsPerThread = 1 << 20;
+
+template <int R>
+__device__ __forceinline__ void dif_pass(float2 *buf, int ntrans, int n, int span,
+                                         const float2 *__restrict__ tw, float sgn, int tid,
+                                         int nthr) {
+  const int stride = span / R;
+  const int nbt = n / R;
+  const int total = ntrans * nbt;
+  for (int b = tid; b < total; b += nthr) {
+    const int t = b / nbt;
+    const int r = b - t * nbt;
+    const int blk = r / stride;
+    const int j = r - blk * stride;
+    float2 *base = buf + (size_t)t * n + (size_t)blk * span + j;
+    float2 v[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) v[m] = base[m * stride];
+    dft<R>(v, sgn);
+    if (j > 0) {
+      const float2 *w = tw + (size_t)j * (R - 1);
+#pragma unroll
+      for (int q = 1; q < R; ++q) v[q] = cmul(v[q], w[q - 1]);
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) base[q * stride] = v[q];
+  }
+  __syncthreads();
+}
+
+// Position of bin k in the digit-reversed LDS result.
+__device__ __forceinline__ int fft_pos(const FftPlanDev &plan, int k) {
+  int pos = 0;
+  for (int p = 0; p < plan.npass; ++p) {
+    const int r = plan.radix[p];
+    const int q = k / r;
+    pos += (k - q * r) * plan.stride[p];
+    k = q;
+  }
+  return pos;
+}
+
+// Radix sets a kernel compiles butterflies for (bit r = radix r).  Kernels on
+// 2,3,5,7-smooth lengths leave out the large odd primes so 1024-thread
+// workgroups stay within 128 VGPRs.
+constexpr unsigned kRadixSmooth = (1u << 2) | (1u << 3) | (1u << 4) | (1u << 5) | (1u << 7) | (1u << 8);
+constexpr unsigned kRadixAll = kRadixSmooth | (1u << 23) | (1u << 29);
+
+// Runs every pass of `plan` over `ntrans` transforms held in `buf` (LDS).
+// sgn must match the sign the twiddle table was built with.
+template <unsigned kMask = kRadixSmooth>
+__device__ inline void lds_fft(float2 *buf, int ntrans, const FftPlanDev &plan,
+                               const float2 *__restrict__ tw, float sgn) {
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  for (int p = 0; p < plan.npass; ++p) {
+    const int r = plan.radix[p];
+    const int span = plan.span[p];
+    const float2 *t = tw + plan.tw_off[p];
+#define FXC_PASS(R)                                                           \
+  case R:                                                                     \
+    if constexpr ((kMask >> R) & 1u) dif_pass<R>(buf, ntrans, plan.n, span, t, sgn, tid, nthr); \
+    break;
+    switch (r) {
+      FXC_PASS(2)
+      FXC_PASS(3)
+      FXC_PASS(4)
+      FXC_PASS(5)
+      FXC_PASS(7)
+      FXC_PASS(8)
+      FXC_PASS(23)
+      FXC_PASS(29)
+      default: break;  // plans are only built from the radices above
+    }
+#undef FXC_PASS
+  }
+}
+
+// True when every radix of `plan` is in kMask (host check before launch).
+inline bool plan_in_mask(const FftPlanDev &plan, unsigned mask) {
+  for (int p = 0; p < plan.npass; ++p)
+    if (!((mask >> plan.radix[p]) & 1u)) return false;
+  return true;
+}
+
+}  // namespace fxc

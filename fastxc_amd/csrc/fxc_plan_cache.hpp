@@ -1,0 +1,16 @@
+// Process-wide cache of FFT plans keyed by (device, length, sign).  Plans own
+// small device twiddle tables; they live until fxc_release_plans() or exit.
+#pragma once
+
+#include "fxc_fft_real.hpp"
+
+namespace fxc {
+
+// Complex values one workgroup can stage in LDS (160 KB on gfx950).
+constexpr int kLdsComplexCapacity = 20480;
+
+int plan_cache_complex(int n, int sign, const FftPlan **out);
+int plan_cache_real(int n, int sign, const RealFftPlan **out);
+void plan_cache_release();
+
+}  // namespace fxc

@@ -1,0 +1,776 @@
+// xc_fast hot path: windowed cross-spectrum stack and lag synthesis.
+//
+// Replaces, for one timestamp and one GPU:
+//   * accumulateStepXc2DKernel launched once per window
+//     (native/xc/kernels/cuda.kernels.cu:16-36, native/xc/worker/async_workers.cu:66-83):
+//       stack[p,k] += conj(S[src_p,k]) * S[rec_p,k] * (1/nstep)
+//   * finalize_xc_batch (native/xc/worker/executor.cu:6-45): (-1)^k shift, C2R of
+//     length nfft = 2(nspec-1), x 1/(nfft*dt), keep lags [nfft/2-H, nfft/2+H].
+//
+// MI355X design.  The window sum is a per-bin Hermitian contraction
+// C_k = A_k^H B_k over the window axis, so pairs are tiled 32 x 32 stations and
+// each workgroup computes one tile for 16 consecutive bins, reading every
+// station spectrum once per tile instead of once per pair-window.  Three
+// arithmetic forms share the tiling:
+//   FP32    plain fp32 FMA on the vector ALU (the parity path);
+//   BF16X3  spectra split into bf16 hi+lo, three MFMA products
+//           (hi*hi + hi*lo + lo*hi) with fp32 accumulation;
+//   FP16    one fp16 MFMA product with a power-of-two scale per station.
+// The cross-spectra go to an HBM workspace in [pair][bin] rows; the lag
+// kernel folds each row into an 18522-point complex inverse FFT held wholly
+// in LDS and writes only the 2H+1 lags.
+#include "fxc_plan_cache.hpp"
+#include "fastxc_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+namespace fxc {
+
+constexpr int kTile = 32;          // stations per tile side
+constexpr int kBinChunk = 16;      // bins per contraction workgroup
+constexpr int kTilePairs = kTile * kTile;
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Spectra split for the MFMA forms.  Input: [nstep][nsta][nspec] complex64
+// (the StepPack payload order).  Output per (station, bin): K' = 2*Kh 16-bit
+// values [Re w=0..Kh-1 | Im w=0..Kh-1], windows >= nstep zero, in
+// [nsta][nspec][K'] order, for hi and (BF16X3) lo parts.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned short bf16_bits_rne(float x) {
+  unsigned int u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);  // inf/nan kept
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_bits_to_float(unsigned short b) {
+  return __uint_as_float(((unsigned int)b) << 16);
+}
+
+__global__ void station_absmax_kernel(const float2 *__restrict__ spec, int nstep, int nsta,
+                                      int nspec, unsigned int *__restrict__ amax_bits) {
+  const int s = blockIdx.y;
+  float m = 0.f;
+  for (int w = 0; w < nstep; ++w) {
+    const float2 *row = spec + ((size_t)w * nsta + s) * nspec;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nspec; k += gridDim.x * blockDim.x) {
+      const float2 v = row[k];
+      m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+  }
+  // wave reduction then one atomic per wave (positive floats order as ints)
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(amax_bits + s, __float_as_uint(m));
+}
+
+// scale_s = 2^-e with 2^(e-1) <= max < 2^e, so |S/scale| <= 1 (exact power).
+__global__ void station_scale_kernel(const unsigned int *__restrict__ amax_bits, int nsta,
+                                     float *__restrict__ inv_scale, float *__restrict__ scale) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsta) return;
+  const float m = __uint_as_float(amax_bits[s]);
+  int e = 0;
+  if (m > 0.f && isfinite(m)) (void)frexpf(m, &e);
+  inv_scale[s] = ldexpf(1.f, -e);
+  scale[s] = ldexpf(1.f, e);
+}
+
+template <int kMode>
+__global__ void split_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec,
+                             int kh, const float *__restrict__ inv_scale,
+                             unsigned short *__restrict__ hi, unsigned short *__restrict__ lo) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  if (k >= nspec) return;
+  const int kp = 2 * kh;
+  const size_t obase = ((size_t)s * nspec + k) * kp;
+  const float sc = (kMode == FXC_PREC_FP16) ? inv_scale[s] : 1.f;
+  for (int c = 0; c < kp; c += 8) {
+    unsigned short h[8], l[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = c + j;
+      const int w = e < kh ? e : e - kh;
+      float v = 0.f;
+      if (w < nstep) {
+        const float2 z = spec[((size_t)w * nsta + s) * nspec + k];
+        v = (e < kh ? z.x : z.y) * sc;
+      }
+      if (kMode == FXC_PREC_FP16) {
+        const _Float16 f = (_Float16)v;
+        h[j] = __builtin_bit_cast(unsigned short, f);
+        l[j] = 0;
+      } else {
+        const unsigned short hb = bf16_bits_rne(v);
+        h[j] = hb;
+        l[j] = bf16_bits_rne(v - bf16_bits_to_float(hb));
+      }
+    }
+    uint4 hv, lv;
+    hv.x = h[0] | ((unsigned)h[1] << 16); hv.y = h[2] | ((unsigned)h[3] << 16);
+    hv.z = h[4] | ((unsigned)h[5] << 16); hv.w = h[6] | ((unsigned)h[7] << 16);
+    *(uint4 *)(hi + obase + c) = hv;
+    if (kMode == FXC_PREC_BF16X3) {
+      lv.x = l[0] | ((unsigned)l[1] << 16); lv.y = l[2] | ((unsigned)l[3] << 16);
+      lv.z = l[4] | ((unsigned)l[5] << 16); lv.w = l[6] | ((unsigned)l[7] << 16);
+      *(uint4 *)(lo + obase + c) = lv;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tile descriptors: tile t covers src stations [src_base, +32) and rec
+// stations [rec_base, +32); rows[t][i][j] is the workspace row of pair
+// (src_base+i, rec_base+j) or -1.
+// ---------------------------------------------------------------------------
+struct TileDesc {
+  int src_base;
+  int rec_base;
+};
+
+// FP32 contraction: thread = (bin, 8x8 micro-tile of pairs).
+__global__ void __launch_bounds__(256)
+contract_fp32_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec,
+                     const TileDesc *__restrict__ tiles, const int *__restrict__ tile_rows,
+                     float2 *__restrict__ xspec, float scale) {
+  __shared__ int rows[kTilePairs];
+  __shared__ int active[16];
+  const int tile = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int *trow = tile_rows + (size_t)tile * kTilePairs;
+  for (int i = tid; i < kTilePairs; i += blockDim.x) rows[i] = trow[i];
+  if (tid < 16) active[tid] = 0;
+  __syncthreads();
+  for (int i = tid; i < kTilePairs; i += blockDim.x) {
+    if (rows[i] >= 0) {
+      const int mi = (i >> 5) >> 3, mj = (i & 31) >> 3;
+      active[mi * 4 + mj] = 1;
+    }
+  }
+  __syncthreads();
+  const int b = tid & 15;
+  const int g = tid >> 4;
+  const int mi = g >> 2, mj = g & 3;
+  const int k = blockIdx.y * kBinChunk + b;
+  if (k >= nspec || !active[g]) return;
+  const TileDesc td = tiles[tile];
+  const int s0 = td.src_base + mi * 8;
+  const int r0 = td.rec_base + mj * 8;
+  float2 acc[8][8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[a][c] = make_float2(0.f, 0.f);
+  for (int w = 0; w < nstep; ++w) {
+    const float2 *base = spec + (size_t)w * nsta * nspec + k;
+    float2 sv[8], rv[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      sv[a] = (s0 + a < nsta) ? base[(size_t)(s0 + a) * nspec] : make_float2(0.f, 0.f);
+      rv[a] = (r0 + a < nsta) ? base[(size_t)(r0 + a) * nspec] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        // conj(s) * r
+        acc[a][c].x += sv[a].x * rv[c].x + sv[a].y * rv[c].y;
+        acc[a][c].y += sv[a].x * rv[c].y - sv[a].y * rv[c].x;
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int row = rows[(mi * 8 + a) * kTile + mj * 8 + c];
+      if (row >= 0) xspec[(size_t)row * nspec + k] = cscale(acc[a][c], scale);
+    }
+  }
+}
+
+// MFMA contraction.  8 waves: wave = (subtile 0..3 of 16x16 stations, bin
+// half).  Per bin: Cr = A.B1, Ci = A.B2 with a_s = [Re | Im] and
+// b2_s = [Im | -Re], so conj(a)*b = Cr + i Ci.  Results are staged in LDS as
+// [pair][bin] and written as 128-byte rows.
+template <int kMode>
+__device__ __forceinline__ void mfma_k32(const unsigned short *pa, const unsigned short *pb,
+                                         f32x4_t &acc, bool neg) {
+  uint4 av = *(const uint4 *)pa;
+  uint4 bv = *(const uint4 *)pb;
+  if (neg) {
+    bv.x ^= 0x80008000u; bv.y ^= 0x80008000u; bv.z ^= 0x80008000u; bv.w ^= 0x80008000u;
+  }
+  if constexpr (kMode == FXC_PREC_FP16) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, av),
+                                                 __builtin_bit_cast(f16x8_t, bv), acc, 0, 0, 0);
+  } else {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, av),
+                                                  __builtin_bit_cast(bf16x8_t, bv), acc, 0, 0, 0);
+  }
+}
+
+template <int kMode>
+__device__ __forceinline__ void mfma_k16(const unsigned short *pa, const unsigned short *pb,
+                                         f32x4_t &acc, bool neg) {
+  uint2 av = *(const uint2 *)pa;
+  uint2 bv = *(const uint2 *)pb;
+  if (neg) {
+    bv.x ^= 0x80008000u; bv.y ^= 0x80008000u;
+  }
+  if constexpr (kMode == FXC_PREC_FP16) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4_t, av),
+                                                __builtin_bit_cast(f16x4_t, bv), acc, 0, 0, 0);
+  } else {
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4_t, av),
+                                                    __builtin_bit_cast(short4_t, bv), acc, 0, 0, 0);
+  }
+}
+
+// One (operand-part) product over the whole K' = 2*kh for one bin.
+template <int kMode>
+__device__ __forceinline__ void mfma_bin(const unsigned short *a, const unsigned short *b,
+                                         int kh, int lane_g, f32x4_t &cr, f32x4_t &ci) {
+  const int kp = 2 * kh;
+  int e = 0;
+  for (; e + 32 <= kp; e += 32) {
+    const int ea = e + 8 * lane_g;
+    mfma_k32<kMode>(a + ea, b + ea, cr, false);
+    const bool hi_half = ea >= kh;
+    mfma_k32<kMode>(a + ea, b + (hi_half ? ea - kh : ea + kh), ci, hi_half);
+  }
+  if (e < kp) {
+    const int ea = e + 4 * lane_g;
+    mfma_k16<kMode>(a + ea, b + ea, cr, false);
+    const bool hi_half = ea >= kh;
+    mfma_k16<kMode>(a + ea, b + (hi_half ? ea - kh : ea + kh), ci, hi_half);
+  }
+}
+
+template <int kMode>
+__global__ void __launch_bounds__(512)
+contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short *__restrict__ lo,
+                     int nsta, int nspec, int kh, const float *__restrict__ sta_scale,
+                     const TileDesc *__restrict__ tiles, const int *__restrict__ tile_rows,
+                     float2 *__restrict__ xspec, float scale) {
+  extern __shared__ float2 stage[];  // [kTilePairs][kBinChunk]
+  __shared__ int rows[kTilePairs];
+  const int tile = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int *trow = tile_rows + (size_t)tile * kTilePairs;
+  for (int i = tid; i < kTilePairs; i += blockDim.x) rows[i] = trow[i];
+  const TileDesc td = tiles[tile];
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int sub = wave & 3;
+  const int half = wave >> 2;
+  const int si = sub >> 1, sj = sub & 1;
+  const int lane_r = lane & 15, lane_g = lane >> 4;
+  const int s_sta = td.src_base + si * 16 + lane_r;
+  const int r_sta = td.rec_base + sj * 16 + lane_r;
+  const bool r_ok = r_sta < nsta;
+  // Stations past the array end load a valid station; their rows/columns
+  // belong to pairs that are never stored.
+  const int s_ld = s_sta < nsta ? s_sta : nsta - 1;
+  const int r_ld = r_ok ? r_sta : nsta - 1;
+  const int kp = 2 * kh;
+  const int k0 = blockIdx.y * kBinChunk;
+  for (int bb = 0; bb < kBinChunk / 2; ++bb) {
+    const int b = half * (kBinChunk / 2) + bb;
+    const int k = k0 + b;
+    f32x4_t cr = {0.f, 0.f, 0.f, 0.f};
+    f32x4_t ci = {0.f, 0.f, 0.f, 0.f};
+    if (k < nspec) {
+      const size_t ao = ((size_t)s_ld * nspec + k) * kp;
+      const size_t bo = ((size_t)r_ld * nspec + k) * kp;
+      const unsigned short *ah = hi + ao;
+      const unsigned short *bh = hi + bo;
+      mfma_bin<kMode>(ah, bh, kh, lane_g, cr, ci);
+      if constexpr (kMode == FXC_PREC_BF16X3) {
+        const unsigned short *al = lo + ao;
+        const unsigned short *bl = lo + bo;
+        mfma_bin<kMode>(ah, bl, kh, lane_g, cr, ci);
+        mfma_bin<kMode>(al, bh, kh, lane_g, cr, ci);
+      }
+    }
+    // D layout: lane holds rows 4*lane_g + i, column lane_r of the 16x16 block.
+    float col_scale = scale;
+    if constexpr (kMode == FXC_PREC_FP16) col_scale *= (r_ok ? sta_scale[r_sta] : 0.f);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int a = si * 16 + 4 * lane_g + i;
+      const int c = sj * 16 + lane_r;
+      float sc = col_scale;
+      if constexpr (kMode == FXC_PREC_FP16) {
+        const int sa = td.src_base + a;
+        sc *= (sa < nsta) ? sta_scale[sa] : 0.f;
+      }
+      stage[(a * kTile + c) * kBinChunk + b] = make_float2(cr[i] * sc, ci[i] * sc);
+    }
+  }
+  __syncthreads();
+  // 16 consecutive threads write one pair's 16 bins (128 bytes).
+  const int b = tid & 15;
+  const int k = k0 + b;
+  if (k < nspec) {
+    for (int p = tid >> 4; p < kTilePairs; p += blockDim.x >> 4) {
+      const int row = rows[p];
+      if (row >= 0) xspec[(size_t)row * nspec + k] = stage[p * kBinChunk + b];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Lag synthesis: one workgroup per cross-spectrum row.
+//   out[m] = C2R(X)[(m - H) mod nfft] * (1/(nfft*dt)),  m = 0..2H
+// which equals the reference's (-1)^k shift + C2R + extraction of
+// [nfft/2 - H, nfft/2 + H].  The scale is evaluated as the reference does:
+// weight = 1.0/(double)((float)nfft * dt), product in double, rounded to float.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024)
+lag_kernel(const float2 *__restrict__ xspec, int nspec, const int *__restrict__ row_out,
+           float *__restrict__ cc, int half_cc, double weight, FftPlanDev plan,
+           const float2 *__restrict__ tw, const float2 *__restrict__ fold) {
+  extern __shared__ float2 lds[];
+  const int row = blockIdx.x;
+  const float2 *X = xspec + (size_t)row * nspec;
+  const int M = plan.n;
+  c2r_fold(lds, M, fold, [&](int k) { return X[k]; });
+  lds_fft(lds, 1, plan, tw, 1.f);
+  const int out_row = row_out ? row_out[row] : row;
+  float *y = cc + (size_t)out_row * (2 * half_cc + 1);
+  for (int m = threadIdx.x; m <= 2 * half_cc; m += blockDim.x) {
+    const int l = m - half_cc;
+    const int par = l & 1;
+    int n = (l - par) / 2;
+    n = n < 0 ? n + M : n;
+    const float2 z = lds[fft_pos(plan, n)];
+    const float v = par ? z.y : z.x;
+    y[m] = (float)((double)v * weight);
+  }
+}
+
+// Copies workspace rows to their task rows (cross-spectra output order).
+__global__ void scatter_rows_kernel(const float2 *__restrict__ src, const int *__restrict__ row_task,
+                                    int nspec, float2 *__restrict__ dst) {
+  const size_t r = blockIdx.x;
+  const size_t t = (size_t)row_task[r];
+  for (int k = threadIdx.x; k < nspec; k += blockDim.x)
+    dst[t * nspec + k] = src[r * nspec + k];
+}
+
+}  // namespace fxc
+
+using namespace fxc;
+
+// ---------------------------------------------------------------------------
+// Plan object behind the C ABI.
+// ---------------------------------------------------------------------------
+struct fxc_xc_plan {
+  int nspec = 0, nstep = 0, nfft = 0, half_cc = 0, cc_size = 0;
+  float dt = 0.f;
+  int precision = FXC_PREC_FP32;
+  int kh = 0;
+  long long chunk_rows_max = 0;  // cross-spectrum rows per workspace fill
+  // pairs
+  int nsta = 0;
+  long long npairs = 0;
+  std::vector<TileDesc> h_tiles;
+  std::vector<long long> chunk_tile_begin;  // tile index where each chunk starts
+  std::vector<long long> chunk_row_begin;   // first workspace row id of each chunk
+  TileDesc *d_tiles = nullptr;
+  int *d_tile_rows = nullptr;     // chunk-relative rows
+  int *d_row_task = nullptr;      // global row -> task index
+  // workspaces
+  float2 *d_xspec = nullptr;
+  size_t xspec_rows = 0;
+  unsigned short *d_hi = nullptr, *d_lo = nullptr;
+  size_t split_elems = 0;
+  unsigned int *d_amax = nullptr;
+  float *d_inv_scale = nullptr, *d_scale = nullptr;
+  int scale_cap = 0;
+  // optional per-kernel timing with events on the launch stream
+  bool timing = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;  // (kind, start/stop)
+  double t_ms[3] = {0, 0, 0};  // split, contract, lag
+  long long t_launch[3] = {0, 0, 0};
+};
+
+enum { kTimeSplit = 0, kTimeContract = 1, kTimeLag = 2 };
+
+static int xc_mark(fxc_xc_plan *p, int kind, bool start, hipStream_t st) {
+  if (!p->timing) return FXC_OK;
+  if (start) {
+    hipEvent_t a, b;
+    FXC_HIP(hipEventCreate(&a));
+    FXC_HIP(hipEventCreate(&b));
+    p->events.push_back({kind, {a, b}});
+    FXC_HIP(hipEventRecord(a, st));
+  } else {
+    FXC_HIP(hipEventRecord(p->events.back().second.second, st));
+  }
+  return FXC_OK;
+}
+
+static void xc_free_pairs(fxc_xc_plan *p) {
+  dfree(p->d_tiles);
+  dfree(p->d_tile_rows);
+  dfree(p->d_row_task);
+  p->d_tiles = nullptr;
+  p->d_tile_rows = nullptr;
+  p->d_row_task = nullptr;
+  p->h_tiles.clear();
+  p->chunk_tile_begin.clear();
+  p->chunk_row_begin.clear();
+  p->npairs = 0;
+}
+
+extern "C" int fxc_xc_plan_create(fxc_xc_plan **out, int nspec, int nstep, float dt, int half_cc,
+                                  int precision) {
+  FXC_CHECK_ARG(out != nullptr, "fxc_xc_plan_create: null output");
+  *out = nullptr;
+  FXC_CHECK_ARG(nspec >= 2 && nstep >= 1 && dt > 0.f && half_cc >= 0,
+                "fxc_xc_plan_create: invalid shape nspec=%d nstep=%d dt=%g half_cc=%d", nspec,
+                nstep, (double)dt, half_cc);
+  FXC_CHECK_ARG(precision == FXC_PREC_FP32 || precision == FXC_PREC_BF16X3 ||
+                    precision == FXC_PREC_FP16,
+                "fxc_xc_plan_create: unknown precision %d", precision);
+  const int nfft = 2 * (nspec - 1);
+  FXC_CHECK_ARG(2 * half_cc + 1 <= nfft, "fxc_xc_plan_create: 2*half_cc+1 exceeds nfft");
+  FXC_CHECK_ARG(nfft / 2 <= kLdsComplexCapacity,
+                "fxc_xc_plan_create: nfft %d exceeds the single-workgroup FFT size", nfft);
+  if (precision != FXC_PREC_FP32)
+    FXC_CHECK_ARG(nstep <= 128, "fxc_xc_plan_create: MFMA forms support up to 128 windows");
+  const RealFftPlan *rp = nullptr;
+  FXC_TRY(plan_cache_real(nfft, 1, &rp));
+  FXC_CHECK_ARG(max_transforms_per_wg(rp->half.dev, 1024, kLdsComplexCapacity) >= 1,
+                "fxc_xc_plan_create: nfft %d does not fit one workgroup", nfft);
+  fxc_xc_plan *p = new fxc_xc_plan();
+  p->nspec = nspec;
+  p->nstep = nstep;
+  p->nfft = nfft;
+  p->half_cc = half_cc;
+  p->cc_size = 2 * half_cc + 1;
+  p->dt = dt;
+  p->precision = precision;
+  p->kh = (nstep + 7) / 8 * 8;
+  p->chunk_rows_max = 0;
+  *out = p;
+  return FXC_OK;
+}
+
+static void xc_drop_events(fxc_xc_plan *p) {
+  for (auto &e : p->events) {
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+  p->events.clear();
+}
+
+// Enables (1) or disables (0) per-kernel event timing and clears the totals.
+extern "C" int fxc_xc_set_timing(fxc_xc_plan *p, int enable) {
+  FXC_CHECK_ARG(p, "fxc_xc_set_timing: null plan");
+  xc_drop_events(p);
+  p->timing = enable != 0;
+  for (int i = 0; i < 3; ++i) {
+    p->t_ms[i] = 0;
+    p->t_launch[i] = 0;
+  }
+  return FXC_OK;
+}
+
+// Waits for the recorded events and returns accumulated kernel milliseconds
+// (split, contraction, lag synthesis) and launch counts since the last reset.
+extern "C" int fxc_xc_get_timing(fxc_xc_plan *p, double *ms3, long long *launches3) {
+  FXC_CHECK_ARG(p && ms3, "fxc_xc_get_timing: invalid arguments");
+  for (auto &e : p->events) {
+    FXC_HIP(hipEventSynchronize(e.second.second));
+    float ms = 0.f;
+    FXC_HIP(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+    p->t_ms[e.first] += ms;
+    p->t_launch[e.first] += 1;
+  }
+  xc_drop_events(p);
+  for (int i = 0; i < 3; ++i) {
+    ms3[i] = p->t_ms[i];
+    if (launches3) launches3[i] = p->t_launch[i];
+  }
+  return FXC_OK;
+}
+
+extern "C" int fxc_xc_plan_destroy(fxc_xc_plan *p) {
+  if (!p) return FXC_OK;
+  xc_drop_events(p);
+  xc_free_pairs(p);
+  dfree(p->d_xspec);
+  dfree(p->d_hi);
+  dfree(p->d_lo);
+  dfree(p->d_amax);
+  dfree(p->d_inv_scale);
+  dfree(p->d_scale);
+  delete p;
+  return FXC_OK;
+}
+
+extern "C" int fxc_xc_set_chunk_rows(fxc_xc_plan *p, long long rows) {
+  FXC_CHECK_ARG(p && rows >= 0, "fxc_xc_set_chunk_rows: invalid arguments");
+  p->chunk_rows_max = rows;
+  return FXC_OK;
+}
+
+// Builds the 32x32 station tiles for the task list (src[t], rec[t]) and the
+// row maps.  Tasks keep their index as the output row of fxc_xc_correlate.
+extern "C" int fxc_xc_set_pairs(fxc_xc_plan *p, int nsta, const int *h_src, const int *h_rec,
+                                long long npairs) {
+  FXC_CHECK_ARG(p && nsta >= 1 && npairs >= 0 && (npairs == 0 || (h_src && h_rec)),
+                "fxc_xc_set_pairs: invalid arguments");
+  xc_free_pairs(p);
+  p->nsta = nsta;
+  p->npairs = npairs;
+  if (npairs == 0) return FXC_OK;
+  const int nblk = (nsta + kTile - 1) / kTile;
+  // tile key -> list of (local pair index, task)
+  std::map<long long, std::vector<std::pair<int, long long>>> by_tile;
+  for (long long t = 0; t < npairs; ++t) {
+    const int s = h_src[t], r = h_rec[t];
+    FXC_CHECK_ARG(s >= 0 && s < nsta && r >= 0 && r < nsta,
+                  "fxc_xc_set_pairs: pair %lld station index out of range", t);
+    const long long key = (long long)(s / kTile) * nblk + (r / kTile);
+    by_tile[key].push_back({(s % kTile) * kTile + (r % kTile), t});
+  }
+  const long long cap = p->chunk_rows_max > 0 ? p->chunk_rows_max : npairs;
+  std::vector<int> rows;
+  std::vector<int> row_task;
+  rows.reserve(by_tile.size() * kTilePairs);
+  row_task.reserve(npairs);
+  long long chunk_rows = 0;
+  for (auto &kv : by_tile) {
+    auto &lst = kv.second;
+    std::sort(lst.begin(), lst.end());
+    // a local pair listed twice (duplicate task) keeps its first row; later
+    // duplicates are computed by their own tile copy
+    std::vector<std::vector<std::pair<int, long long>>> copies(1);
+    for (auto &e : lst) {
+      size_t c = 0;
+      while (c < copies.size() && !copies[c].empty() && copies[c].back().first == e.first) ++c;
+      if (c == copies.size()) copies.emplace_back();
+      copies[c].push_back(e);
+    }
+    for (auto &cp : copies) {
+      if (cp.empty()) continue;
+      if (p->h_tiles.empty() || chunk_rows + (long long)cp.size() > cap) {
+        p->chunk_tile_begin.push_back((long long)p->h_tiles.size());
+        p->chunk_row_begin.push_back((long long)row_task.size());
+        chunk_rows = 0;
+      }
+      TileDesc td;
+      td.src_base = (int)(kv.first / nblk) * kTile;
+      td.rec_base = (int)(kv.first % nblk) * kTile;
+      p->h_tiles.push_back(td);
+      const size_t off = rows.size();
+      rows.resize(off + kTilePairs, -1);
+      for (auto &e : cp) {
+        rows[off + e.first] = (int)chunk_rows++;
+        row_task.push_back((int)e.second);
+      }
+    }
+  }
+  p->chunk_tile_begin.push_back((long long)p->h_tiles.size());
+  p->chunk_row_begin.push_back((long long)row_task.size());
+  FXC_TRY(dmalloc((void **)&p->d_tiles, p->h_tiles.size() * sizeof(TileDesc)));
+  FXC_TRY(dmalloc((void **)&p->d_tile_rows, rows.size() * sizeof(int)));
+  FXC_TRY(dmalloc((void **)&p->d_row_task, row_task.size() * sizeof(int)));
+  FXC_HIP(hipMemcpy(p->d_tiles, p->h_tiles.data(), p->h_tiles.size() * sizeof(TileDesc),
+                    hipMemcpyHostToDevice));
+  FXC_HIP(hipMemcpy(p->d_tile_rows, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice));
+  FXC_HIP(hipMemcpy(p->d_row_task, row_task.data(), row_task.size() * sizeof(int),
+                    hipMemcpyHostToDevice));
+  return FXC_OK;
+}
+
+extern "C" int fxc_xc_pairs_info(const fxc_xc_plan *p, long long *ntiles, long long *nchunks,
+                                 long long *max_chunk_rows) {
+  FXC_CHECK_ARG(p, "fxc_xc_pairs_info: null plan");
+  const long long nc = p->chunk_tile_begin.empty() ? 0 : (long long)p->chunk_tile_begin.size() - 1;
+  long long mx = 0;
+  for (long long c = 0; c < nc; ++c)
+    mx = std::max(mx, p->chunk_row_begin[c + 1] - p->chunk_row_begin[c]);
+  if (ntiles) *ntiles = (long long)p->h_tiles.size();
+  if (nchunks) *nchunks = nc;
+  if (max_chunk_rows) *max_chunk_rows = mx;
+  return FXC_OK;
+}
+
+static int xc_ensure_workspace(fxc_xc_plan *p, size_t rows) {
+  if (rows > p->xspec_rows) {
+    dfree(p->d_xspec);
+    p->d_xspec = nullptr;
+    p->xspec_rows = 0;
+    FXC_TRY(dmalloc((void **)&p->d_xspec, rows * (size_t)p->nspec * sizeof(float2)));
+    p->xspec_rows = rows;
+  }
+  if (p->precision != FXC_PREC_FP32) {
+    const size_t elems = (size_t)p->nsta * p->nspec * (size_t)(2 * p->kh);
+    if (elems > p->split_elems) {
+      dfree(p->d_hi);
+      dfree(p->d_lo);
+      p->d_hi = p->d_lo = nullptr;
+      p->split_elems = 0;
+      FXC_TRY(dmalloc((void **)&p->d_hi, elems * sizeof(unsigned short)));
+      if (p->precision == FXC_PREC_BF16X3)
+        FXC_TRY(dmalloc((void **)&p->d_lo, elems * sizeof(unsigned short)));
+      p->split_elems = elems;
+    }
+    if (p->nsta > p->scale_cap) {
+      dfree(p->d_amax);
+      dfree(p->d_inv_scale);
+      dfree(p->d_scale);
+      FXC_TRY(dmalloc((void **)&p->d_amax, p->nsta * sizeof(unsigned int)));
+      FXC_TRY(dmalloc((void **)&p->d_inv_scale, p->nsta * sizeof(float)));
+      FXC_TRY(dmalloc((void **)&p->d_scale, p->nsta * sizeof(float)));
+      p->scale_cap = p->nsta;
+    }
+  }
+  return FXC_OK;
+}
+
+// Split pass for the MFMA forms (no-op for FP32).
+static int xc_prepare_spectra(fxc_xc_plan *p, const float2 *d_spec, hipStream_t st) {
+  if (p->precision == FXC_PREC_FP32) return FXC_OK;
+  FXC_TRY(xc_mark(p, kTimeSplit, true, st));
+  const dim3 grid((unsigned)ceil_div(p->nspec, 256), (unsigned)p->nsta);
+  if (p->precision == FXC_PREC_FP16) {
+    FXC_HIP(hipMemsetAsync(p->d_amax, 0, p->nsta * sizeof(unsigned int), st));
+    hipLaunchKernelGGL(station_absmax_kernel, dim3(8, p->nsta), dim3(256), 0, st, d_spec,
+                       p->nstep, p->nsta, p->nspec, p->d_amax);
+    hipLaunchKernelGGL(station_scale_kernel, dim3((unsigned)ceil_div(p->nsta, 256)), dim3(256), 0,
+                       st, p->d_amax, p->nsta, p->d_inv_scale, p->d_scale);
+    hipLaunchKernelGGL(split_kernel<FXC_PREC_FP16>, grid, dim3(256), 0, st, d_spec, p->nstep,
+                       p->nsta, p->nspec, p->kh, p->d_inv_scale, p->d_hi, p->d_lo);
+  } else {
+    hipLaunchKernelGGL(split_kernel<FXC_PREC_BF16X3>, grid, dim3(256), 0, st, d_spec, p->nstep,
+                       p->nsta, p->nspec, p->kh, (const float *)nullptr, p->d_hi, p->d_lo);
+  }
+  FXC_HIP(hipGetLastError());
+  FXC_TRY(xc_mark(p, kTimeSplit, false, st));
+  return FXC_OK;
+}
+
+static int xc_contract_chunk(fxc_xc_plan *p, const float2 *d_spec, long long c, float2 *dst,
+                             hipStream_t st) {
+  const long long t0 = p->chunk_tile_begin[c], t1 = p->chunk_tile_begin[c + 1];
+  const unsigned ntl = (unsigned)(t1 - t0);
+  const unsigned nbc = (unsigned)ceil_div(p->nspec, kBinChunk);
+  const float scale = 1.0f / (float)p->nstep;
+  const TileDesc *tiles = p->d_tiles + t0;
+  const int *trows = p->d_tile_rows + t0 * kTilePairs;
+  FXC_TRY(xc_mark(p, kTimeContract, true, st));
+  if (p->precision == FXC_PREC_FP32) {
+    hipLaunchKernelGGL(contract_fp32_kernel, dim3(ntl, nbc), dim3(256), 0, st, d_spec, p->nstep,
+                       p->nsta, p->nspec, tiles, trows, dst, scale);
+  } else {
+    const size_t lds = (size_t)kTilePairs * kBinChunk * sizeof(float2);
+    if (p->precision == FXC_PREC_BF16X3) {
+      FXC_HIP(hipFuncSetAttribute((const void *)contract_mfma_kernel<FXC_PREC_BF16X3>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(contract_mfma_kernel<FXC_PREC_BF16X3>, dim3(ntl, nbc), dim3(512), lds, st,
+                         p->d_hi, p->d_lo, p->nsta, p->nspec, p->kh, (const float *)nullptr, tiles,
+                         trows, dst, scale);
+    } else {
+      FXC_HIP(hipFuncSetAttribute((const void *)contract_mfma_kernel<FXC_PREC_FP16>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(contract_mfma_kernel<FXC_PREC_FP16>, dim3(ntl, nbc), dim3(512), lds, st,
+                         p->d_hi, p->d_lo, p->nsta, p->nspec, p->kh, p->d_scale, tiles, trows, dst,
+                         scale);
+    }
+  }
+  FXC_HIP(hipGetLastError());
+  FXC_TRY(xc_mark(p, kTimeContract, false, st));
+  return FXC_OK;
+}
+
+static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long nrows, const int *row_out,
+                        float *d_cc, hipStream_t st) {
+  if (nrows == 0) return FXC_OK;
+  const RealFftPlan *rp = nullptr;
+  FXC_TRY(plan_cache_real(p->nfft, 1, &rp));
+  const size_t lds = (size_t)(p->nfft / 2) * sizeof(float2);
+  FXC_HIP(hipFuncSetAttribute((const void *)lag_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds));
+  // reference weight: 1.0 / (width * dt) with width converted to float
+  const double weight = 1.0 / (double)((float)p->nfft * p->dt);
+  FXC_TRY(xc_mark(p, kTimeLag, true, st));
+  hipLaunchKernelGGL(lag_kernel, dim3((unsigned)nrows), dim3(1024), lds, st, xs, p->nspec, row_out,
+                     d_cc, p->half_cc, weight, rp->half.dev, rp->half.d_tw, rp->d_fold);
+  FXC_HIP(hipGetLastError());
+  FXC_TRY(xc_mark(p, kTimeLag, false, st));
+  return FXC_OK;
+}
+
+// Cross-spectra only (the windowed stack), rows in task order:
+// d_xspec[task][k] = (1/nstep) sum_w conj(S[w][src][k]) S[w][rec][k].
+extern "C" int fxc_xc_cross_spectra(fxc_xc_plan *p, const float *d_spec, float *d_xspec,
+                                    void *stream) {
+  FXC_CHECK_ARG(p && d_spec && d_xspec, "fxc_xc_cross_spectra: invalid arguments");
+  FXC_CHECK_ARG(p->chunk_tile_begin.size() >= 1, "fxc_xc_cross_spectra: no pairs set");
+  if (p->npairs == 0) return FXC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  long long maxrows = 0;
+  FXC_TRY(fxc_xc_pairs_info(p, nullptr, nullptr, &maxrows));
+  FXC_TRY(xc_ensure_workspace(p, (size_t)maxrows));
+  FXC_TRY(xc_prepare_spectra(p, (const float2 *)d_spec, st));
+  const long long nc = (long long)p->chunk_tile_begin.size() - 1;
+  for (long long c = 0; c < nc; ++c) {
+    FXC_TRY(xc_contract_chunk(p, (const float2 *)d_spec, c, p->d_xspec, st));
+    const long long r0 = p->chunk_row_begin[c], r1 = p->chunk_row_begin[c + 1];
+    if (r1 > r0) {
+      hipLaunchKernelGGL(scatter_rows_kernel, dim3((unsigned)(r1 - r0)), dim3(256), 0, st,
+                         p->d_xspec, p->d_row_task + r0, p->nspec, (float2 *)d_xspec);
+      FXC_HIP(hipGetLastError());
+    }
+  }
+  return FXC_OK;
+}
+
+// Lag synthesis only: cc[r] = lags of the cross-spectrum row r.
+extern "C" int fxc_xc_lags(fxc_xc_plan *p, const float *d_xspec, long long nrows, float *d_cc,
+                           void *stream) {
+  FXC_CHECK_ARG(p && d_xspec && d_cc && nrows >= 0, "fxc_xc_lags: invalid arguments");
+  return xc_lags_rows(p, (const float2 *)d_xspec, nrows, nullptr, d_cc, (hipStream_t)stream);
+}
+
+// Full correlation: cc[task][0..2H] for every task set by fxc_xc_set_pairs.
+extern "C" int fxc_xc_correlate(fxc_xc_plan *p, const float *d_spec, float *d_cc, void *stream) {
+  FXC_CHECK_ARG(p && d_spec && d_cc, "fxc_xc_correlate: invalid arguments");
+  FXC_CHECK_ARG(p->chunk_tile_begin.size() >= 1, "fxc_xc_correlate: no pairs set");
+  if (p->npairs == 0) return FXC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  long long maxrows = 0;
+  FXC_TRY(fxc_xc_pairs_info(p, nullptr, nullptr, &maxrows));
+  FXC_TRY(xc_ensure_workspace(p, (size_t)maxrows));
+  FXC_TRY(xc_prepare_spectra(p, (const float2 *)d_spec, st));
+  const long long nc = (long long)p->chunk_tile_begin.size() - 1;
+  for (long long c = 0; c < nc; ++c) {
+    FXC_TRY(xc_contract_chunk(p, (const float2 *)d_spec, c, p->d_xspec, st));
+    const long long r0 = p->chunk_row_begin[c], r1 = p->chunk_row_begin[c + 1];
+    FXC_TRY(xc_lags_rows(p, p->d_xspec, r1 - r0, p->d_row_task + r0, d_cc, st));
+  }
+  return FXC_OK;
+}
+
+extern "C" int fxc_xc_plan_info(const fxc_xc_plan *p, int *nfft, int *half_cc, int *cc_size) {
+  FXC_CHECK_ARG(p, "fxc_xc_plan_info: null plan");
+  if (nfft) *nfft = p->nfft;
+  if (half_cc) *half_cc = p->half_cc;
+  if (cc_size) *cc_size = p->cc_size;
+  return FXC_OK;
+}

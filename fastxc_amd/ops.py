@@ -1,0 +1,159 @@
+"""Torch-facing wrappers over the C ABI (device tensors in, device tensors out).
+
+Each wrapper validates shapes on the host before any launch, then enqueues the
+HIP kernels on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import PRECISIONS, check, lib
+
+
+def _stream_ptr(device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _require(t: torch.Tensor, dtype, name: str) -> None:
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+def fft_c2c(x: torch.Tensor, inverse: bool = False, scale: float = 1.0) -> torch.Tensor:
+    """Batched complex FFT along the last axis of a complex64 tensor."""
+    _require(x, torch.complex64, "x")
+    n = x.shape[-1]
+    batch = x.numel() // n if n else 0
+    out = torch.empty_like(x)
+    check(lib().fxc_fft_c2c(x.data_ptr(), n, out.data_ptr(), n, n, batch, int(inverse),
+                            float(scale), _stream_ptr(x.device)), "fxc_fft_c2c")
+    return out
+
+
+def fft_r2c(x: torch.Tensor, n: int | None = None, scale: float = 1.0) -> torch.Tensor:
+    """Batched real FFT (rows zero-padded to n) -> bins 0..n/2 as complex64."""
+    _require(x, torch.float32, "x")
+    in_len = x.shape[-1]
+    n = in_len if n is None else n
+    batch = x.numel() // in_len if in_len else 0
+    out = torch.empty(x.shape[:-1] + (n // 2 + 1,), dtype=torch.complex64, device=x.device)
+    check(lib().fxc_fft_r2c(x.data_ptr(), in_len, in_len, out.data_ptr(), n // 2 + 1, n, batch,
+                            float(scale), _stream_ptr(x.device)), "fxc_fft_r2c")
+    return out
+
+
+def fft_c2r(X: torch.Tensor, n: int, out_len: int | None = None, scale: float = 1.0) -> torch.Tensor:
+    """Batched real inverse FFT of bins 0..n/2 (unnormalised); first out_len samples."""
+    _require(X, torch.complex64, "X")
+    if X.shape[-1] != n // 2 + 1:
+        raise ValueError("X must hold n/2+1 bins")
+    out_len = n if out_len is None else out_len
+    batch = X.numel() // X.shape[-1]
+    out = torch.empty(X.shape[:-1] + (out_len,), dtype=torch.float32, device=X.device)
+    check(lib().fxc_fft_c2r(X.data_ptr(), n // 2 + 1, out.data_ptr(), out_len, out_len, n, batch,
+                            float(scale), _stream_ptr(X.device)), "fxc_fft_c2r")
+    return out
+
+
+class XcPlan:
+    """One timestamp's windowed cross-correlation (xc_fast hot path).
+
+    spectra: complex64 [nstep, nsta, nspec] (StepPack payload order).
+    """
+
+    def __init__(self, nspec: int, nstep: int, dt: float, half_cc: int,
+                 precision: str = "fp32", chunk_rows: int = 0):
+        self._h = ctypes.c_void_p()
+        check(lib().fxc_xc_plan_create(ctypes.byref(self._h), nspec, nstep, float(dt), half_cc,
+                                       PRECISIONS[precision]), "fxc_xc_plan_create")
+        self.nspec, self.nstep, self.dt, self.half_cc = nspec, nstep, dt, half_cc
+        self.cc_size = 2 * half_cc + 1
+        self.nfft = 2 * (nspec - 1)
+        self.precision = precision
+        self.npairs = 0
+        if chunk_rows:
+            check(lib().fxc_xc_set_chunk_rows(self._h, int(chunk_rows)), "fxc_xc_set_chunk_rows")
+
+    def close(self):
+        if self._h:
+            lib().fxc_xc_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_pairs(self, nsta: int, src, rec) -> None:
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        rec = np.ascontiguousarray(rec, dtype=np.int32)
+        if src.shape != rec.shape or src.ndim != 1:
+            raise ValueError("src and rec must be equal-length 1-D arrays")
+        if src.size and (src.min() < 0 or rec.min() < 0 or src.max() >= nsta or rec.max() >= nsta):
+            raise ValueError("station index out of range")
+        check(lib().fxc_xc_set_pairs(self._h, int(nsta), src.ctypes.data, rec.ctypes.data,
+                                     int(src.size)), "fxc_xc_set_pairs")
+        self.nsta = nsta
+        self.npairs = int(src.size)
+
+    def pairs_info(self):
+        a, b, c = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        check(lib().fxc_xc_pairs_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+              "fxc_xc_pairs_info")
+        return {"ntiles": a.value, "nchunks": b.value, "max_chunk_rows": c.value}
+
+    def _check_spec(self, spec: torch.Tensor) -> None:
+        _require(spec, torch.complex64, "spec")
+        if tuple(spec.shape) != (self.nstep, self.nsta, self.nspec):
+            raise ValueError(f"spec shape {tuple(spec.shape)} != "
+                             f"{(self.nstep, self.nsta, self.nspec)}")
+
+    def cross_spectra(self, spec: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        self._check_spec(spec)
+        if out is None:
+            out = torch.empty((self.npairs, self.nspec), dtype=torch.complex64, device=spec.device)
+        _require(out, torch.complex64, "out")
+        check(lib().fxc_xc_cross_spectra(self._h, spec.data_ptr(), out.data_ptr(),
+                                         _stream_ptr(spec.device)), "fxc_xc_cross_spectra")
+        return out
+
+    def lags(self, xspec: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        _require(xspec, torch.complex64, "xspec")
+        if xspec.shape[-1] != self.nspec:
+            raise ValueError("xspec rows must hold nspec bins")
+        rows = xspec.numel() // self.nspec
+        if out is None:
+            out = torch.empty((rows, self.cc_size), dtype=torch.float32, device=xspec.device)
+        _require(out, torch.float32, "out")
+        check(lib().fxc_xc_lags(self._h, xspec.data_ptr(), rows, out.data_ptr(),
+                                _stream_ptr(xspec.device)), "fxc_xc_lags")
+        return out
+
+    def set_timing(self, enable: bool) -> None:
+        check(lib().fxc_xc_set_timing(self._h, int(enable)), "fxc_xc_set_timing")
+
+    def get_timing(self):
+        ms = (ctypes.c_double * 3)()
+        n = (ctypes.c_longlong * 3)()
+        check(lib().fxc_xc_get_timing(self._h, ms, n), "fxc_xc_get_timing")
+        return {"split_ms": ms[0], "contract_ms": ms[1], "lag_ms": ms[2],
+                "split_launches": n[0], "contract_launches": n[1], "lag_launches": n[2]}
+
+    def correlate(self, spec: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        self._check_spec(spec)
+        if out is None:
+            out = torch.empty((self.npairs, self.cc_size), dtype=torch.float32, device=spec.device)
+        _require(out, torch.float32, "out")
+        if tuple(out.shape) != (self.npairs, self.cc_size):
+            raise ValueError("out must be [npairs, 2*half_cc+1]")
+        check(lib().fxc_xc_correlate(self._h, spec.data_ptr(), out.data_ptr(),
+                                     _stream_ptr(spec.device)), "fxc_xc_correlate")
+        return out

@@ -1,0 +1,93 @@
+/*
+ * fastxc_amd.h -- C ABI of the MI355X (gfx950) FastXC kernels.
+ *
+ * The reference FastXC has no library boundary: Python runs the four stage
+ * executables (fastxc/runtime/command_runner.py:147-155) and everything
+ * crosses as files.  This header is the thin C ABI the replacement stage
+ * binaries call, one entry point per reference hot-path routine it replaces
+ * (cited per declaration).  Conventions:
+ *   - every function returns 0 on success, non-zero on error; the message is
+ *     available from fxc_last_error() (thread-local);
+ *   - buffers are caller-owned device pointers (complex data is interleaved
+ *     float re/im); `stream` is a hipStream_t or NULL for the default stream;
+ *   - work is enqueued on `stream` and is complete when the stream is.
+ */
+#ifndef FASTXC_AMD_H
+#define FASTXC_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failed call on this thread. */
+const char *fxc_last_error(void);
+/* Frees cached FFT plans (twiddle tables) of every device. */
+void fxc_release_plans(void);
+
+/* ---------------------------------------------------------------- FFT --
+ * Batched transforms on the workgroup LDS FFT (lengths whose n (complex) or
+ * n/2 (real) factor into 2,3,5,7,23,29 and fit 20480 complex values).
+ * Replace the cuFFT plans of native/sac2spec/device_memory.cu:23-32,
+ * native/xc/memory/memory.cu:478-483, native/pws/worker/gpu.cu:197-227 and
+ * native/tfpws/core/tfpws_workspace.cu:190-235.  Unnormalised; `scale`
+ * multiplies every output. */
+int fxc_fft_c2c(const float *d_in, long long in_dist, float *d_out, long long out_dist, int n,
+                long long batch, int inverse, float scale, void *stream);
+/* Real forward transform: in_len samples per row (zero-padded to n); writes
+ * bins 0..n/2. */
+int fxc_fft_r2c(const float *d_in, long long in_dist, int in_len, float *d_out,
+                long long out_dist, int n, long long batch, float scale, void *stream);
+/* Real inverse transform of bins 0..n/2; writes the first out_len samples
+ * (imaginary parts of bins 0 and n/2 are ignored). */
+int fxc_fft_c2r(const float *d_in, long long in_dist, float *d_out, long long out_dist,
+                int out_len, int n, long long batch, float scale, void *stream);
+
+/* ------------------------------------------------------------- xc_fast --
+ * Windowed cross-correlation of one timestamp.
+ * Spectra layout: [nstep][nsta][nspec] complex64, the StepPack v3 payload
+ * order (native/xc/io/input.cu:46-67).  Output: [npairs][2*half_cc+1] float,
+ * row t for task t = (src[t], rec[t]) in the caller's order. */
+#define FXC_PREC_FP32 0   /* fp32 vector FMA (reference arithmetic) */
+#define FXC_PREC_BF16X3 1 /* split-bf16 hi+lo MFMA, fp32 accumulation */
+#define FXC_PREC_FP16 2   /* fp16 MFMA with per-station power-of-2 scale */
+
+typedef struct fxc_xc_plan fxc_xc_plan;
+
+/* nfft = 2(nspec-1); half_cc = llround(max_lag/dt) clamped to nspec-1, as
+ * finalize_shape (native/xc/io/input.cu:1033-1054). */
+int fxc_xc_plan_create(fxc_xc_plan **plan, int nspec, int nstep, float dt, int half_cc,
+                       int precision);
+int fxc_xc_plan_destroy(fxc_xc_plan *plan);
+int fxc_xc_plan_info(const fxc_xc_plan *plan, int *nfft, int *half_cc, int *cc_size);
+/* Bounds the cross-spectrum workspace (rows of nspec complex); 0 = all pairs. */
+int fxc_xc_set_chunk_rows(fxc_xc_plan *plan, long long rows);
+/* Task list of the job: src/rec station indices (host arrays).  Replaces the
+ * index lists of build_tasks_for_job (native/xc/worker/scheduler.cu:65-152). */
+int fxc_xc_set_pairs(fxc_xc_plan *plan, int nsta, const int *h_src, const int *h_rec,
+                     long long npairs);
+int fxc_xc_pairs_info(const fxc_xc_plan *plan, long long *ntiles, long long *nchunks,
+                      long long *max_chunk_rows);
+/* Windowed stack only: d_xspec[t][k] = (1/nstep) sum_w conj(S[w][src][k]) S[w][rec][k].
+ * Replaces the per-window accumulateStepXc2DKernel loop
+ * (native/xc/worker/async_workers.cu:66-83, kernels/cuda.kernels.cu:16-36). */
+int fxc_xc_cross_spectra(fxc_xc_plan *plan, const float *d_spec, float *d_xspec, void *stream);
+/* Lag synthesis only: (-1)^k shift, C2R(nfft), x 1/(nfft*dt), lags
+ * [nfft/2-H, nfft/2+H] of each cross-spectrum row.  Replaces
+ * finalize_xc_batch (native/xc/worker/executor.cu:6-45). */
+int fxc_xc_lags(fxc_xc_plan *plan, const float *d_xspec, long long nrows, float *d_cc,
+                void *stream);
+/* Per-kernel timing with HIP events recorded on the launch stream (no
+ * reference counterpart; used by the benchmark).  get_timing waits for the
+ * events and returns milliseconds and launch counts for the split,
+ * contraction and lag-synthesis kernels since the last set_timing. */
+int fxc_xc_set_timing(fxc_xc_plan *plan, int enable);
+int fxc_xc_get_timing(fxc_xc_plan *plan, double *ms3, long long *launches3);
+/* Stack + lag synthesis for every task (execute_row_batch_job,
+ * native/xc/worker/async_workers.cu:18-86, minus file I/O). */
+int fxc_xc_correlate(fxc_xc_plan *plan, const float *d_spec, float *d_cc, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTXC_AMD_H */
