@@ -53,21 +53,33 @@ const char *last_error();
 // Workgroup FFT plan (device side).  A transform of length n = prod(radix)
 // runs as in-place mixed-radix decimation-in-frequency passes in LDS.  Pass p
 // splits spans of length span[p] into radix[p] sub-spans of length
-// stride[p] = span[p]/radix[p]; its twiddles w^(j*q), j < stride[p],
-// 1 <= q < radix[p], sit at tw[tw_off[p] + j*(radix[p]-1) + q-1] with
-// w = exp(sign*2*pi*i/span[p]).  The result is left in digit-reversed order:
-// bin k is read from position fft_pos(plan, k) = sum_p d_p*stride[p], where
-// k = d_0 + radix[0]*(d_1 + radix[1]*(d_2 + ...)).
+// stride[p] = span[p]/radix[p]; butterfly j of a span is twiddled by
+// w_n^(j*q*tstep[p]) (tstep = n/span, w_n = exp(sign*2*pi*i/n)).  The result
+// is left in digit-reversed order: bin k is at fft_pos(plan, k) =
+// sum_p d_p*stride[p], where k = d_0 + radix[0]*(d_1 + radix[1]*(d_2 + ...)).
+// Twiddles come from a two-level table w_n^e = coarse[e >> 7] * fine[e & 127]
+// (fine: 128 entries, coarse: ceil(n/128)), computed in FP64 on the host and
+// staged in LDS.  Integer divisions use multiply-high by a precomputed
+// ceil(2^32/d), exact for operands below 2^16.
 constexpr int kMaxPasses = 16;
+constexpr int kTwFine = 128;
 
 struct FftPlanDev {
   int n;
   int npass;
+  int ncoarse;
+  unsigned nmagic;             // ceil(2^32/n), 0 when n == 1
   int radix[kMaxPasses];
   int span[kMaxPasses];
   int stride[kMaxPasses];
-  int tw_off[kMaxPasses];
+  int tstep[kMaxPasses];
+  unsigned smagic[kMaxPasses];  // ceil(2^32/stride), 0 when stride == 1
+  unsigned bmagic[kMaxPasses];  // ceil(2^32/(n/radix))
+  unsigned rmagic[kMaxPasses];  // ceil(2^32/radix)
 };
+
+// float2 entries of the LDS twiddle region of a length-n plan.
+inline __host__ __device__ int tw_lds_count(int n) { return kTwFine + (n + kTwFine - 1) / kTwFine; }
 
 // Host-side owner of a workgroup FFT plan and its twiddle table.
 struct FftPlan {

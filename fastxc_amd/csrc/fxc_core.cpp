@@ -65,24 +65,37 @@ int fft_plan_create(FftPlan *plan, int n, int sign) {
     set_error("fft length %d has a prime factor without a butterfly", n);
     return FXC_ERR_UNSUPPORTED;
   }
-  std::vector<float2> tw;
+  auto magic = [](int d) -> unsigned {
+    return d <= 1 ? 0u : (unsigned)(((1ull << 32) + (unsigned long long)d - 1) / (unsigned long long)d);
+  };
+  if (n >= (1 << 16)) {
+    set_error("fft length %d exceeds the workgroup FFT limit", n);
+    return FXC_ERR_UNSUPPORTED;
+  }
   int span = n;
   for (int p = 0; p < d.npass; ++p) {
     const int r = d.radix[p];
     const int stride = span / r;
     d.span[p] = span;
     d.stride[p] = stride;
-    d.tw_off[p] = (int)tw.size();
-    const double base = sign * 2.0 * M_PI / (double)span;
-    for (int j = 0; j < stride; ++j) {
-      for (int q = 1; q < r; ++q) {
-        // reduce the integer phase before converting to an angle
-        const long long ph = ((long long)j * q) % (long long)span;
-        const double a = base * (double)ph;
-        tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
-      }
-    }
+    d.tstep[p] = n / span;
+    d.smagic[p] = magic(stride);
+    d.bmagic[p] = magic(n / r);
+    d.rmagic[p] = magic(r);
     span = stride;
+  }
+  d.nmagic = magic(n);
+  d.ncoarse = (n + kTwFine - 1) / kTwFine;
+  // two-level table: fine[u] = w^u (u < 128), coarse[v] = w^(128 v)
+  std::vector<float2> tw(kTwFine + d.ncoarse);
+  for (int u = 0; u < kTwFine; ++u) {
+    const double a = sign * 2.0 * M_PI * (double)(u % n) / (double)n;
+    tw[u] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  for (int v = 0; v < d.ncoarse; ++v) {
+    const long long e = ((long long)v * kTwFine) % n;
+    const double a = sign * 2.0 * M_PI * (double)e / (double)n;
+    tw[kTwFine + v] = make_float2((float)std::cos(a), (float)std::sin(a));
   }
   plan->dev = d;
   plan->sign = sign;

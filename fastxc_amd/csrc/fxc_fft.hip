@@ -24,11 +24,16 @@ int real_fft_plan_create(RealFftPlan *plan, int n, int sign) {
   plan->n = n;
   FXC_TRY(fft_plan_create(&plan->half, n / 2, sign));
   const int M = n / 2;
-  std::vector<float2> fold(M + 1);
-  for (int k = 0; k <= M; ++k) {
-    const double a = sign * 2.0 * M_PI * (double)k / (double)n;
-    fold[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  std::vector<float2> fold(fold_lds_count(M));
+  for (int u = 0; u < kTwFine; ++u) {
+    const double a = sign * 2.0 * M_PI * (double)u / (double)n;
+    fold[u] = make_float2((float)std::cos(a), (float)std::sin(a));
   }
+  for (int v = 0; v + kTwFine < (int)fold.size(); ++v) {
+    const double a = sign * 2.0 * M_PI * (double)((long long)v * kTwFine % n) / (double)n;
+    fold[kTwFine + v] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  plan->fold_count = (int)fold.size();
   FXC_TRY(dmalloc((void **)&plan->d_fold, fold.size() * sizeof(float2)));
   FXC_HIP(hipMemcpy(plan->d_fold, fold.data(), fold.size() * sizeof(float2), hipMemcpyHostToDevice));
   return FXC_OK;
@@ -42,7 +47,8 @@ void real_fft_plan_destroy(RealFftPlan *plan) {
 
 int max_transforms_per_wg(const FftPlanDev &plan, int nthreads, int lds_complex_capacity) {
   (void)nthreads;  // threads loop over butterflies; only LDS capacity limits
-  return lds_complex_capacity / (plan.n > 0 ? plan.n : 1);
+  const int room = lds_complex_capacity - tw_lds_count(plan.n);
+  return room / (plan.n > 0 ? plan.n : 1);
 }
 
 constexpr int kFftThreads = 1024;
@@ -57,12 +63,14 @@ c2c_batch_kernel(const float2 *__restrict__ in, long long in_dist, float2 *__res
   const long long t0 = (long long)blockIdx.x * per_wg;
   const int nt = (int)((ntr - t0) < per_wg ? (ntr - t0) : per_wg);
   const int n = plan.n;
+  float2 *twl = lds + (size_t)per_wg * n;
+  for (int i = threadIdx.x; i < kTwFine + plan.ncoarse; i += blockDim.x) twl[i] = tw[i];
   for (int i = threadIdx.x; i < nt * n; i += blockDim.x) {
     const int t = i / n, j = i - t * n;
     lds[i] = in[(t0 + t) * in_dist + j];
   }
   __syncthreads();
-  lds_fft<kMask>(lds, nt, plan, tw, sgn);
+  lds_fft<kMask>(lds, nt, plan, twl, sgn);
   for (int i = threadIdx.x; i < nt * n; i += blockDim.x) {
     const int t = i / n, j = i - t * n;
     out[(t0 + t) * out_dist + j] = cscale(lds[t * n + fft_pos(plan, j)], scale);
@@ -78,9 +86,12 @@ r2c_batch_kernel(const float *__restrict__ in, long long in_dist, int in_len,
   const float *x = in + row * in_dist;
   float2 *y = out + row * out_dist;
   const int M = plan.n;
+  stage_real_tables(lds, plan, tw, fold);
+  const float2 *twl = lds + M + 1;
+  const float2 *fl = twl + tw_lds_count(M);
   real_pack(lds, M, [&](int i) { return i < in_len ? x[i] : 0.f; });
-  lds_fft(lds, 1, plan, tw, -1.f);
-  r2c_unfold(lds, plan, fold, [&](int k, float2 v) { y[k] = cscale(v, scale); });
+  lds_fft(lds, 1, plan, twl, -1.f);
+  r2c_unfold(lds, plan, fl, [&](int k, float2 v) { y[k] = cscale(v, scale); });
 }
 
 __global__ void __launch_bounds__(kFftThreads)
@@ -92,8 +103,11 @@ c2r_batch_kernel(const float2 *__restrict__ in, long long in_dist, float *__rest
   const float2 *X = in + row * in_dist;
   float *y = out + row * out_dist;
   const int M = plan.n;
-  c2r_fold(lds, M, fold, [&](int k) { return X[k]; });
-  lds_fft(lds, 1, plan, tw, 1.f);
+  stage_real_tables(lds, plan, tw, fold);
+  const float2 *twl = lds + M + 1;
+  const float2 *fl = twl + tw_lds_count(M);
+  c2r_fold(lds, M, fl, [&](int k) { return X[k]; });
+  lds_fft(lds, 1, plan, twl, 1.f);
   c2r_unpack(lds, plan, [&](int i, float v) {
     if (i < out_len) y[i] = v * scale;
   });
@@ -118,7 +132,7 @@ extern "C" int fxc_fft_c2c(const float *d_in, long long in_dist, float *d_out, l
   const int nthr = smooth ? kFftThreads : kFftThreadsOddPrime;
   const int per = max_transforms_per_wg(p->dev, nthr, kLdsComplexCapacity);
   FXC_CHECK_ARG(per >= 1, "fxc_fft_c2c: length %d does not fit one workgroup", n);
-  const size_t lds = (size_t)per * n * sizeof(float2);
+  const size_t lds = ((size_t)per * n + tw_lds_count(n)) * sizeof(float2);
   const unsigned grid = (unsigned)ceil_div((size_t)batch, (size_t)per);
   if (smooth) {
     FXC_TRY(set_max_lds((const void *)c2c_batch_kernel<kRadixSmooth, kFftThreads>, lds));
@@ -144,7 +158,8 @@ extern "C" int fxc_fft_r2c(const float *d_in, long long in_dist, int in_len, flo
   FXC_TRY(plan_cache_real(n, -1, &p));
   FXC_CHECK_ARG(max_transforms_per_wg(p->half.dev, kFftThreads, kLdsComplexCapacity) >= 1,
                 "fxc_fft_r2c: length %d does not fit one workgroup", n);
-  const size_t lds = (size_t)(n / 2) * sizeof(float2);
+  const size_t lds = (size_t)real_lds_count(n / 2) * sizeof(float2);
+  FXC_CHECK_ARG(real_lds_count(n / 2) <= kLdsComplexCapacity, "fxc_fft_r2c: length %d too large", n);
   FXC_TRY(set_max_lds((const void *)r2c_batch_kernel, lds));
   hipLaunchKernelGGL(r2c_batch_kernel, dim3((unsigned)batch), dim3(kFftThreads), lds,
                      (hipStream_t)stream, d_in, in_dist, in_len, (float2 *)d_out, out_dist,
@@ -163,7 +178,8 @@ extern "C" int fxc_fft_c2r(const float *d_in, long long in_dist, float *d_out, l
   FXC_TRY(plan_cache_real(n, 1, &p));
   FXC_CHECK_ARG(max_transforms_per_wg(p->half.dev, kFftThreads, kLdsComplexCapacity) >= 1,
                 "fxc_fft_c2r: length %d does not fit one workgroup", n);
-  const size_t lds = (size_t)(n / 2) * sizeof(float2);
+  const size_t lds = (size_t)real_lds_count(n / 2) * sizeof(float2);
+  FXC_CHECK_ARG(real_lds_count(n / 2) <= kLdsComplexCapacity, "fxc_fft_c2r: length %d too large", n);
   FXC_TRY(set_max_lds((const void *)c2r_batch_kernel, lds));
   hipLaunchKernelGGL(c2r_batch_kernel, dim3((unsigned)batch), dim3(kFftThreads), lds,
                      (hipStream_t)stream, (const float2 *)d_in, in_dist, d_out, out_dist, out_len,

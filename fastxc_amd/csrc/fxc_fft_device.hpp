@@ -156,28 +156,91 @@ __device__ __forceinline__ void dft<8>(float2 *a, float sgn) {
 // threads loop over their butterflies one at a time).
 constexpr int kMaxElemsPerThread = 1 << 20;
 
+__device__ __forceinline__ unsigned udiv_magic(unsigned x, unsigned d, unsigned magic) {
+  return d <= 1u ? x : __umulhi(x, magic);
+}
+
+// w_n^e from the LDS two-level table (tw = [fine(128) | coarse]).
+__device__ __forceinline__ float2 tw_lookup(const float2 *tw, int e) {
+  return cmul(tw[kTwFine + (e >> 7)], tw[e & (kTwFine - 1)]);
+}
+
+// Multiplies v[1..R-1] by w^(q*e), q = 1..R-1 (exponents modulo n).
 template <int R>
-__device__ __forceinline__ void dif_pass(float2 *buf, int ntrans, int n, int span,
-                                         const float2 *__restrict__ tw, float sgn, int tid,
-                                         int nthr) {
-  const int stride = span / R;
+__device__ __forceinline__ void apply_twiddles(float2 *v, const float2 *tw, int e, int n) {
+  if constexpr (R <= 8) {
+    const float2 w = tw_lookup(tw, e);
+    float2 wq = w;
+    v[1] = cmul(v[1], w);
+#pragma unroll
+    for (int q = 2; q < R; ++q) {
+      wq = cmul(wq, w);
+      v[q] = cmul(v[q], wq);
+    }
+  } else {
+    int eq = 0;
+#pragma unroll
+    for (int q = 1; q < R; ++q) {
+      eq += e;
+      if (eq >= n) eq -= n;
+      v[q] = cmul(v[q], tw_lookup(tw, eq));
+    }
+  }
+}
+
+// One in-place DIF pass (butterfly, then twiddle) over ntrans transforms.
+template <int R>
+__device__ __forceinline__ void dif_pass(float2 *buf, int ntrans, const FftPlanDev &plan, int p,
+                                         const float2 *tw, float sgn, int tid, int nthr) {
+  const int n = plan.n;
+  const int span = plan.span[p];
+  const int stride = plan.stride[p];
+  const int tstep = plan.tstep[p];
+  const unsigned smagic = plan.smagic[p];
+  const unsigned bmagic = plan.bmagic[p];
   const int nbt = n / R;
   const int total = ntrans * nbt;
   for (int b = tid; b < total; b += nthr) {
-    const int t = b / nbt;
+    const int t = ntrans > 1 ? (int)udiv_magic((unsigned)b, (unsigned)nbt, bmagic) : 0;
     const int r = b - t * nbt;
-    const int blk = r / stride;
+    const int blk = (int)udiv_magic((unsigned)r, (unsigned)stride, smagic);
     const int j = r - blk * stride;
-    float2 *base = buf + (size_t)t * n + (size_t)blk * span + j;
+    float2 *base = buf + t * n + blk * span + j;
     float2 v[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) v[m] = base[m * stride];
     dft<R>(v, sgn);
-    if (j > 0) {
-      const float2 *w = tw + (size_t)j * (R - 1);
+    if (j > 0) apply_twiddles<R>(v, tw, j * tstep, n);
 #pragma unroll
-      for (int q = 1; q < R; ++q) v[q] = cmul(v[q], w[q - 1]);
-    }
+    for (int q = 0; q < R; ++q) base[q * stride] = v[q];
+  }
+  __syncthreads();
+}
+
+// One in-place DIT pass (twiddle, then butterfly): the exact inverse
+// ordering of dif_pass, consuming digit-reversed data.
+template <int R>
+__device__ __forceinline__ void dit_pass(float2 *buf, int ntrans, const FftPlanDev &plan, int p,
+                                         const float2 *tw, float sgn, int tid, int nthr) {
+  const int n = plan.n;
+  const int span = plan.span[p];
+  const int stride = plan.stride[p];
+  const int tstep = plan.tstep[p];
+  const unsigned smagic = plan.smagic[p];
+  const unsigned bmagic = plan.bmagic[p];
+  const int nbt = n / R;
+  const int total = ntrans * nbt;
+  for (int b = tid; b < total; b += nthr) {
+    const int t = ntrans > 1 ? (int)udiv_magic((unsigned)b, (unsigned)nbt, bmagic) : 0;
+    const int r = b - t * nbt;
+    const int blk = (int)udiv_magic((unsigned)r, (unsigned)stride, smagic);
+    const int j = r - blk * stride;
+    float2 *base = buf + t * n + blk * span + j;
+    float2 v[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) v[m] = base[m * stride];
+    if (j > 0) apply_twiddles<R>(v, tw, j * tstep, n);
+    dft<R>(v, sgn);
 #pragma unroll
     for (int q = 0; q < R; ++q) base[q * stride] = v[q];
   }
@@ -189,11 +252,18 @@ __device__ __forceinline__ int fft_pos(const FftPlanDev &plan, int k) {
   int pos = 0;
   for (int p = 0; p < plan.npass; ++p) {
     const int r = plan.radix[p];
-    const int q = k / r;
+    const int q = (int)udiv_magic((unsigned)k, (unsigned)r, plan.rmagic[p]);
     pos += (k - q * r) * plan.stride[p];
     k = q;
   }
   return pos;
+}
+
+// Copies the plan's twiddle table (global) into LDS; ends with a barrier.
+__device__ inline void stage_twiddles(float2 *tw_lds, const FftPlanDev &plan,
+                                      const float2 *__restrict__ tw_global) {
+  for (int i = threadIdx.x; i < kTwFine + plan.ncoarse; i += blockDim.x) tw_lds[i] = tw_global[i];
+  __syncthreads();
 }
 
 // Radix sets a kernel compiles butterflies for (bit r = radix r).  Kernels on
@@ -202,33 +272,39 @@ __device__ __forceinline__ int fft_pos(const FftPlanDev &plan, int k) {
 constexpr unsigned kRadixSmooth = (1u << 2) | (1u << 3) | (1u << 4) | (1u << 5) | (1u << 7) | (1u << 8);
 constexpr unsigned kRadixAll = kRadixSmooth | (1u << 23) | (1u << 29);
 
-// Runs every pass of `plan` over `ntrans` transforms held in `buf` (LDS).
-// sgn must match the sign the twiddle table was built with.
+#define FXC_RADIX_SWITCH(FN)                                                  \
+  switch (plan.radix[p]) {                                                    \
+    case 2: if constexpr ((kMask >> 2) & 1u) FN<2>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break;   \
+    case 3: if constexpr ((kMask >> 3) & 1u) FN<3>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break;   \
+    case 4: if constexpr ((kMask >> 4) & 1u) FN<4>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break;   \
+    case 5: if constexpr ((kMask >> 5) & 1u) FN<5>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break;   \
+    case 7: if constexpr ((kMask >> 7) & 1u) FN<7>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break;   \
+    case 8: if constexpr ((kMask >> 8) & 1u) FN<8>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break;   \
+    case 23: if constexpr ((kMask >> 23) & 1u) FN<23>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break; \
+    case 29: if constexpr ((kMask >> 29) & 1u) FN<29>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break; \
+    default: break;                                                           \
+  }
+
+// Natural-order input -> digit-reversed output (tw: LDS twiddle table).
 template <unsigned kMask = kRadixSmooth>
-__device__ inline void lds_fft(float2 *buf, int ntrans, const FftPlanDev &plan,
-                               const float2 *__restrict__ tw, float sgn) {
+__device__ inline void lds_fft(float2 *buf, int ntrans, const FftPlanDev &plan, const float2 *tw,
+                               float sgn) {
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   for (int p = 0; p < plan.npass; ++p) {
-    const int r = plan.radix[p];
-    const int span = plan.span[p];
-    const float2 *t = tw + plan.tw_off[p];
-#define FXC_PASS(R)                                                           \
-  case R:                                                                     \
-    if constexpr ((kMask >> R) & 1u) dif_pass<R>(buf, ntrans, plan.n, span, t, sgn, tid, nthr); \
-    break;
-    switch (r) {
-      FXC_PASS(2)
-      FXC_PASS(3)
-      FXC_PASS(4)
-      FXC_PASS(5)
-      FXC_PASS(7)
-      FXC_PASS(8)
-      FXC_PASS(23)
-      FXC_PASS(29)
-      default: break;  // plans are only built from the radices above
-    }
-#undef FXC_PASS
+    FXC_RADIX_SWITCH(dif_pass)
+  }
+}
+
+// Digit-reversed input -> natural-order output (the inverse ordering of
+// lds_fft with the same plan radices; tw built for the sign sgn).
+template <unsigned kMask = kRadixSmooth>
+__device__ inline void lds_fft_dit(float2 *buf, int ntrans, const FftPlanDev &plan,
+                                   const float2 *tw, float sgn) {
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  for (int p = plan.npass - 1; p >= 0; --p) {
+    FXC_RADIX_SWITCH(dit_pass)
   }
 }
 

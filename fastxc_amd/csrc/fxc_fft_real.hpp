@@ -14,8 +14,33 @@ namespace fxc {
 struct RealFftPlan {
   int n = 0;            // real length (even)
   FftPlan half;         // complex plan of length n/2
-  float2 *d_fold = nullptr;  // exp(sign*2*pi*i*k/n), k = 0..n/2
+  // exp(sign*2*pi*i*k/n), k = 0..n/2, as a two-level table
+  // [fine(128) | coarse(ceil((n/2+1)/128))], staged in LDS by the kernels
+  float2 *d_fold = nullptr;
+  int fold_count = 0;
 };
+
+// LDS layout of a single real transform of length n = 2M:
+// [data: M+1 complex][twiddles of the M-point plan][fold table].
+inline __host__ __device__ int fold_lds_count(int M) { return kTwFine + (M + 1 + kTwFine - 1) / kTwFine; }
+inline __host__ __device__ int real_lds_count(int M) { return M + 1 + tw_lds_count(M) + fold_lds_count(M); }
+
+__device__ __forceinline__ float2 fold_lookup(const float2 *fold, int k) {
+  return cmul(fold[kTwFine + (k >> 7)], fold[k & (kTwFine - 1)]);
+}
+
+// Copies the twiddle and fold tables into their LDS slots (after M+1 data
+// slots) and synchronises the workgroup.
+__device__ inline void stage_real_tables(float2 *lds, const FftPlanDev &plan,
+                                         const float2 *__restrict__ tw,
+                                         const float2 *__restrict__ fold) {
+  const int M = plan.n;
+  float2 *twl = lds + M + 1;
+  float2 *fl = twl + tw_lds_count(M);
+  for (int i = threadIdx.x; i < kTwFine + plan.ncoarse; i += blockDim.x) twl[i] = tw[i];
+  for (int i = threadIdx.x; i < fold_lds_count(M); i += blockDim.x) fl[i] = fold[i];
+  __syncthreads();
+}
 
 int real_fft_plan_create(RealFftPlan *plan, int n, int sign);
 void real_fft_plan_destroy(RealFftPlan *plan);
@@ -25,6 +50,7 @@ int max_transforms_per_wg(const FftPlanDev &plan, int nthreads, int lds_complex_
 // lds[j] = ld(2j) + i*ld(2j+1), j < M.
 template <class LoadReal>
 __device__ inline void real_pack(float2 *lds, int M, const LoadReal &ld) {
+#pragma unroll 8
   for (int j = threadIdx.x; j < M; j += blockDim.x) lds[j] = make_float2(ld(2 * j), ld(2 * j + 1));
   __syncthreads();
 }
@@ -34,8 +60,8 @@ __device__ inline void real_pack(float2 *lds, int M, const LoadReal &ld) {
 //   X[k] = E_k + w_k O_k, E_k = (Z[k] + conj Z[M-k])/2, O_k = -i (Z[k] - conj Z[M-k])/2,
 // with w_k = exp(-2*pi*i*k/N) and Z[M] = Z[0].
 template <class StoreSpec>
-__device__ inline void r2c_unfold(const float2 *lds, const FftPlanDev &plan,
-                                  const float2 *__restrict__ fold, const StoreSpec &st) {
+__device__ inline void r2c_unfold(const float2 *lds, const FftPlanDev &plan, const float2 *fold,
+                                  const StoreSpec &st) {
   const int M = plan.n;
   for (int k = threadIdx.x; k <= M; k += blockDim.x) {
     const float2 a = lds[fft_pos(plan, k == M ? 0 : k)];
@@ -43,7 +69,7 @@ __device__ inline void r2c_unfold(const float2 *lds, const FftPlanDev &plan,
     const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
     const float2 d = csub(a, b);
     const float2 o = make_float2(0.5f * d.y, -0.5f * d.x);
-    float2 x = cadd(e, cmul(fold[k], o));
+    float2 x = cadd(e, cmul(fold_lookup(fold, k), o));
     if (k == 0 || k == M) x.y = 0.f;
     st(k, x);
   }
@@ -53,8 +79,8 @@ __device__ inline void r2c_unfold(const float2 *lds, const FftPlanDev &plan,
 // an inverse FFT of length M (w_k = exp(+2*pi*i*k/N)).  The imaginary parts
 // of X[0] and X[M] are ignored, as a real inverse transform requires.
 template <class LoadSpec>
-__device__ inline void c2r_fold(float2 *lds, int M, const float2 *__restrict__ fold,
-                                const LoadSpec &ld) {
+__device__ inline void c2r_fold(float2 *lds, int M, const float2 *fold, const LoadSpec &ld) {
+#pragma unroll 8
   for (int k = threadIdx.x; k < M; k += blockDim.x) {
     float2 a = ld(k);
     float2 b = cconj(ld(M - k));
@@ -63,7 +89,7 @@ __device__ inline void c2r_fold(float2 *lds, int M, const float2 *__restrict__ f
       b.y = 0.f;
     }
     const float2 s = cadd(a, b);
-    const float2 d = cmul(fold[k], csub(a, b));
+    const float2 d = cmul(fold_lookup(fold, k), csub(a, b));
     lds[k] = make_float2(s.x - d.y, s.y + d.x);
   }
   __syncthreads();

@@ -6,7 +6,8 @@
 
 namespace fxc {
 
-// Complex values one workgroup can stage in LDS (160 KB on gfx950).
+// Complex values one workgroup can stage in LDS (160 KB on gfx950),
+// including the FFT twiddle table.
 constexpr int kLdsComplexCapacity = 20480;
 
 int plan_cache_complex(int n, int sign, const FftPlan **out);

@@ -43,7 +43,8 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 // ---------------------------------------------------------------------------
 // Spectra split for the MFMA forms.  Input: [nstep][nsta][nspec] complex64
 // (the StepPack payload order).  Output per (station, bin): K' = 2*Kh 16-bit
-// values [Re w=0..Kh-1 | Im w=0..Kh-1], windows >= nstep zero, in
+// values [Re w=0..Kh-1 | Im w=0..Kh-1] (Kh = nstep rounded up to 16),
+// windows >= nstep zero, in
 // [nsta][nspec][K'] order, for hi and (BF16X3) lo parts.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ unsigned short bf16_bits_rne(float x) {
@@ -137,14 +138,30 @@ struct TileDesc {
   int rec_base;
 };
 
+// Workgroup -> (tile, bin chunk).  Workgroups are dispatched round-robin over
+// the 8 XCDs, so 8 consecutive workgroups take 8 consecutive bin chunks of one
+// tile: each XCD's L2 then serves one chunk of every station's spectra, and
+// each pair row receives 8 x 128 contiguous bytes at once.
+constexpr int kChunkGroup = 8;
+
+__device__ __forceinline__ bool tile_chunk(int ntiles, int nchunks, int &tile, int &chunk) {
+  const int per_group = ntiles * kChunkGroup;
+  const int g = blockIdx.x / per_group;
+  const int r = blockIdx.x - g * per_group;
+  tile = r / kChunkGroup;
+  chunk = g * kChunkGroup + (r - tile * kChunkGroup);
+  return chunk < nchunks;
+}
+
 // FP32 contraction: thread = (bin, 8x8 micro-tile of pairs).
 __global__ void __launch_bounds__(256)
 contract_fp32_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec,
                      const TileDesc *__restrict__ tiles, const int *__restrict__ tile_rows,
-                     float2 *__restrict__ xspec, float scale) {
+                     float2 *__restrict__ xspec, long long xpitch, int ntiles, float scale) {
   __shared__ int rows[kTilePairs];
   __shared__ int active[16];
-  const int tile = blockIdx.x;
+  int tile, chunk;
+  if (!tile_chunk(ntiles, (nspec + kBinChunk - 1) / kBinChunk, tile, chunk)) return;
   const int tid = threadIdx.x;
   const int *trow = tile_rows + (size_t)tile * kTilePairs;
   for (int i = tid; i < kTilePairs; i += blockDim.x) rows[i] = trow[i];
@@ -160,7 +177,7 @@ contract_fp32_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int n
   const int b = tid & 15;
   const int g = tid >> 4;
   const int mi = g >> 2, mj = g & 3;
-  const int k = blockIdx.y * kBinChunk + b;
+  const int k = chunk * kBinChunk + b;
   if (k >= nspec || !active[g]) return;
   const TileDesc td = tiles[tile];
   const int s0 = td.src_base + mi * 8;
@@ -170,13 +187,23 @@ contract_fp32_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int n
   for (int a = 0; a < 8; ++a)
 #pragma unroll
     for (int c = 0; c < 8; ++c) acc[a][c] = make_float2(0.f, 0.f);
-  for (int w = 0; w < nstep; ++w) {
-    const float2 *base = spec + (size_t)w * nsta * nspec + k;
-    float2 sv[8], rv[8];
+  // software pipeline: loads of window w+1 are in flight while window w is
+  // accumulated
+  const size_t wstride = (size_t)nsta * nspec;
+  float2 sv[8], rv[8], sn[8], rn[8];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      sv[a] = (s0 + a < nsta) ? base[(size_t)(s0 + a) * nspec] : make_float2(0.f, 0.f);
-      rv[a] = (r0 + a < nsta) ? base[(size_t)(r0 + a) * nspec] : make_float2(0.f, 0.f);
+  for (int a = 0; a < 8; ++a) {
+    sv[a] = (s0 + a < nsta) ? spec[(size_t)(s0 + a) * nspec + k] : make_float2(0.f, 0.f);
+    rv[a] = (r0 + a < nsta) ? spec[(size_t)(r0 + a) * nspec + k] : make_float2(0.f, 0.f);
+  }
+  for (int w = 0; w < nstep; ++w) {
+    if (w + 1 < nstep) {
+      const float2 *base = spec + (size_t)(w + 1) * wstride + k;
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        sn[a] = (s0 + a < nsta) ? base[(size_t)(s0 + a) * nspec] : make_float2(0.f, 0.f);
+        rn[a] = (r0 + a < nsta) ? base[(size_t)(r0 + a) * nspec] : make_float2(0.f, 0.f);
+      }
     }
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
@@ -187,13 +214,18 @@ contract_fp32_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int n
         acc[a][c].y += sv[a].x * rv[c].y - sv[a].y * rv[c].x;
       }
     }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      sv[a] = sn[a];
+      rv[a] = rn[a];
+    }
   }
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int row = rows[(mi * 8 + a) * kTile + mj * 8 + c];
-      if (row >= 0) xspec[(size_t)row * nspec + k] = cscale(acc[a][c], scale);
+      if (row >= 0) xspec[(size_t)row * xpitch + k] = cscale(acc[a][c], scale);
     }
   }
 }
@@ -202,69 +234,73 @@ contract_fp32_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int n
 // half).  Per bin: Cr = A.B1, Ci = A.B2 with a_s = [Re | Im] and
 // b2_s = [Im | -Re], so conj(a)*b = Cr + i Ci.  Results are staged in LDS as
 // [pair][bin] and written as 128-byte rows.
-template <int kMode>
-__device__ __forceinline__ void mfma_k32(const unsigned short *pa, const unsigned short *pb,
-                                         f32x4_t &acc, bool neg) {
-  uint4 av = *(const uint4 *)pa;
-  uint4 bv = *(const uint4 *)pb;
-  if (neg) {
-    bv.x ^= 0x80008000u; bv.y ^= 0x80008000u; bv.z ^= 0x80008000u; bv.w ^= 0x80008000u;
-  }
-  if constexpr (kMode == FXC_PREC_FP16) {
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, av),
-                                                 __builtin_bit_cast(f16x8_t, bv), acc, 0, 0, 0);
-  } else {
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, av),
-                                                  __builtin_bit_cast(bf16x8_t, bv), acc, 0, 0, 0);
-  }
+// Operand fragments of one lane for one bin and one split part (hi or lo).
+// K' = 2*kh = 32*NK32 with kh a multiple of 16, so every K step is one
+// 16x16x32 MFMA: a dependent accumulate chain never mixes MFMA shapes.
+// Source side: A; receiver side: B1 = [Re|Im] and B2 = [Im|-Re].
+template <int NK32>
+struct SrcFrag {
+  uint4 a32[NK32];
+};
+template <int NK32>
+struct RecFrag {
+  uint4 b32[NK32], c32[NK32];
+};
+
+template <int NK32>
+__device__ __forceinline__ void load_src(const unsigned short *a, int lane_g, SrcFrag<NK32> &o) {
+#pragma unroll
+  for (int s = 0; s < NK32; ++s) o.a32[s] = *(const uint4 *)(a + 32 * s + 8 * lane_g);
 }
 
-template <int kMode>
-__device__ __forceinline__ void mfma_k16(const unsigned short *pa, const unsigned short *pb,
-                                         f32x4_t &acc, bool neg) {
-  uint2 av = *(const uint2 *)pa;
-  uint2 bv = *(const uint2 *)pb;
-  if (neg) {
-    bv.x ^= 0x80008000u; bv.y ^= 0x80008000u;
-  }
-  if constexpr (kMode == FXC_PREC_FP16) {
-    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4_t, av),
-                                                __builtin_bit_cast(f16x4_t, bv), acc, 0, 0, 0);
-  } else {
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4_t, av),
-                                                    __builtin_bit_cast(short4_t, bv), acc, 0, 0, 0);
-  }
-}
-
-// One (operand-part) product over the whole K' = 2*kh for one bin.
-template <int kMode>
-__device__ __forceinline__ void mfma_bin(const unsigned short *a, const unsigned short *b,
-                                         int kh, int lane_g, f32x4_t &cr, f32x4_t &ci) {
-  const int kp = 2 * kh;
-  int e = 0;
-  for (; e + 32 <= kp; e += 32) {
-    const int ea = e + 8 * lane_g;
-    mfma_k32<kMode>(a + ea, b + ea, cr, false);
+template <int NK32>
+__device__ __forceinline__ void load_rec(const unsigned short *b, int lane_g, RecFrag<NK32> &o) {
+  constexpr int kh = 16 * NK32;
+#pragma unroll
+  for (int s = 0; s < NK32; ++s) {
+    const int ea = 32 * s + 8 * lane_g;
     const bool hi_half = ea >= kh;
-    mfma_k32<kMode>(a + ea, b + (hi_half ? ea - kh : ea + kh), ci, hi_half);
-  }
-  if (e < kp) {
-    const int ea = e + 4 * lane_g;
-    mfma_k16<kMode>(a + ea, b + ea, cr, false);
-    const bool hi_half = ea >= kh;
-    mfma_k16<kMode>(a + ea, b + (hi_half ? ea - kh : ea + kh), ci, hi_half);
+    o.b32[s] = *(const uint4 *)(b + ea);
+    uint4 c = *(const uint4 *)(b + (hi_half ? ea - kh : ea + kh));
+    if (hi_half) {
+      c.x ^= 0x80008000u; c.y ^= 0x80008000u; c.z ^= 0x80008000u; c.w ^= 0x80008000u;
+    }
+    o.c32[s] = c;
   }
 }
 
 template <int kMode>
+__device__ __forceinline__ void mfma32(uint4 a, uint4 b, f32x4_t &acc) {
+  if constexpr (kMode == FXC_PREC_FP16) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, a),
+                                                 __builtin_bit_cast(f16x8_t, b), acc, 0, 0, 0);
+  } else {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                  __builtin_bit_cast(bf16x8_t, b), acc, 0, 0, 0);
+  }
+}
+
+// cr += A.B1, ci += A.B2 over all K steps of one bin.
+template <int kMode, int NK32>
+__device__ __forceinline__ void mfma_bin(const SrcFrag<NK32> &x, const RecFrag<NK32> &y,
+                                         f32x4_t &cr, f32x4_t &ci) {
+#pragma unroll
+  for (int s = 0; s < NK32; ++s) {
+    mfma32<kMode>(x.a32[s], y.b32[s], cr);
+    mfma32<kMode>(x.a32[s], y.c32[s], ci);
+  }
+}
+
+template <int kMode, int NK32>
 __global__ void __launch_bounds__(512)
 contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short *__restrict__ lo,
-                     int nsta, int nspec, int kh, const float *__restrict__ sta_scale,
+                     int nsta, int nspec, const float *__restrict__ sta_scale,
                      const TileDesc *__restrict__ tiles, const int *__restrict__ tile_rows,
-                     float2 *__restrict__ xspec, float scale) {
+                     float2 *__restrict__ xspec, long long xpitch, int ntiles, float scale) {
   extern __shared__ float2 stage[];  // [kTilePairs][kBinChunk]
   __shared__ int rows[kTilePairs];
-  const int tile = blockIdx.x;
+  int tile, chunk;
+  if (!tile_chunk(ntiles, (nspec + kBinChunk - 1) / kBinChunk, tile, chunk)) return;
   const int tid = threadIdx.x;
   const int *trow = tile_rows + (size_t)tile * kTilePairs;
   for (int i = tid; i < kTilePairs; i += blockDim.x) rows[i] = trow[i];
@@ -282,39 +318,53 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
   // belong to pairs that are never stored.
   const int s_ld = s_sta < nsta ? s_sta : nsta - 1;
   const int r_ld = r_ok ? r_sta : nsta - 1;
-  const int kp = 2 * kh;
-  const int k0 = blockIdx.y * kBinChunk;
-  for (int bb = 0; bb < kBinChunk / 2; ++bb) {
-    const int b = half * (kBinChunk / 2) + bb;
-    const int k = k0 + b;
-    f32x4_t cr = {0.f, 0.f, 0.f, 0.f};
-    f32x4_t ci = {0.f, 0.f, 0.f, 0.f};
-    if (k < nspec) {
+  constexpr int kp = 32 * NK32;
+  constexpr bool kSplit = kMode == FXC_PREC_BF16X3;
+  const int k0 = chunk * kBinChunk;
+  // bins whose operands are loaded ahead together, sized to ~160 VGPRs of fragments
+  constexpr int kFragRegs = 12 * NK32 * (kSplit ? 2 : 1);
+  constexpr int kGroupRaw = 160 / kFragRegs;
+  constexpr int kGroup = kGroupRaw >= 4 ? 4 : (kGroupRaw >= 2 ? 2 : 1);
+  float col_scale = scale;
+  if constexpr (kMode == FXC_PREC_FP16) col_scale *= (r_ok ? sta_scale[r_sta] : 0.f);
+  for (int g0 = 0; g0 < kBinChunk / 2; g0 += kGroup) {
+    SrcFrag<NK32> sh[kGroup], sl[kSplit ? kGroup : 1];
+    RecFrag<NK32> rh[kGroup], rl[kSplit ? kGroup : 1];
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) {
+      int k = k0 + half * (kBinChunk / 2) + g0 + i;
+      k = k < nspec ? k : nspec - 1;  // tail bins compute a valid bin, never stored
       const size_t ao = ((size_t)s_ld * nspec + k) * kp;
       const size_t bo = ((size_t)r_ld * nspec + k) * kp;
-      const unsigned short *ah = hi + ao;
-      const unsigned short *bh = hi + bo;
-      mfma_bin<kMode>(ah, bh, kh, lane_g, cr, ci);
-      if constexpr (kMode == FXC_PREC_BF16X3) {
-        const unsigned short *al = lo + ao;
-        const unsigned short *bl = lo + bo;
-        mfma_bin<kMode>(ah, bl, kh, lane_g, cr, ci);
-        mfma_bin<kMode>(al, bh, kh, lane_g, cr, ci);
+      load_src(hi + ao, lane_g, sh[i]);
+      load_rec(hi + bo, lane_g, rh[i]);
+      if constexpr (kSplit) {
+        load_src(lo + ao, lane_g, sl[i]);
+        load_rec(lo + bo, lane_g, rl[i]);
       }
     }
-    // D layout: lane holds rows 4*lane_g + i, column lane_r of the 16x16 block.
-    float col_scale = scale;
-    if constexpr (kMode == FXC_PREC_FP16) col_scale *= (r_ok ? sta_scale[r_sta] : 0.f);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int a = si * 16 + 4 * lane_g + i;
-      const int c = sj * 16 + lane_r;
-      float sc = col_scale;
-      if constexpr (kMode == FXC_PREC_FP16) {
-        const int sa = td.src_base + a;
-        sc *= (sa < nsta) ? sta_scale[sa] : 0.f;
+    for (int i = 0; i < kGroup; ++i) {
+      const int b = half * (kBinChunk / 2) + g0 + i;
+      f32x4_t cr = {0.f, 0.f, 0.f, 0.f};
+      f32x4_t ci = {0.f, 0.f, 0.f, 0.f};
+      mfma_bin<kMode>(sh[i], rh[i], cr, ci);  // hi.hi
+      if constexpr (kSplit) {
+        mfma_bin<kMode>(sh[i], rl[i], cr, ci);  // hi.lo
+        mfma_bin<kMode>(sl[i], rh[i], cr, ci);  // lo.hi
       }
-      stage[(a * kTile + c) * kBinChunk + b] = make_float2(cr[i] * sc, ci[i] * sc);
+      // D layout: lane holds rows 4*lane_g + r, column lane_r of the 16x16 block.
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = si * 16 + 4 * lane_g + r;
+        const int c = sj * 16 + lane_r;
+        float sc = col_scale;
+        if constexpr (kMode == FXC_PREC_FP16) {
+          const int sa = td.src_base + a;
+          sc *= (sa < nsta) ? sta_scale[sa] : 0.f;
+        }
+        stage[(a * kTile + c) * kBinChunk + b] = make_float2(cr[r] * sc, ci[r] * sc);
+      }
     }
   }
   __syncthreads();
@@ -324,48 +374,98 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
   if (k < nspec) {
     for (int p = tid >> 4; p < kTilePairs; p += blockDim.x >> 4) {
       const int row = rows[p];
-      if (row >= 0) xspec[(size_t)row * nspec + k] = stage[p * kBinChunk + b];
+      if (row >= 0) xspec[(size_t)row * xpitch + k] = stage[p * kBinChunk + b];
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Lag synthesis: one workgroup per cross-spectrum row.
+// Lag synthesis.  Persistent: each 1024-thread workgroup owns one CU's LDS
+// and walks rows blockIdx.x, +gridDim.x, ...; the next row is loaded into
+// registers while the current row's FFT runs, so HBM reads overlap the LDS
+// passes.  Per row:
 //   out[m] = C2R(X)[(m - H) mod nfft] * (1/(nfft*dt)),  m = 0..2H
 // which equals the reference's (-1)^k shift + C2R + extraction of
 // [nfft/2 - H, nfft/2 + H].  The scale is evaluated as the reference does:
 // weight = 1.0/(double)((float)nfft * dt), product in double, rounded to float.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024)
-lag_kernel(const float2 *__restrict__ xspec, int nspec, const int *__restrict__ row_out,
-           float *__restrict__ cc, int half_cc, double weight, FftPlanDev plan,
-           const float2 *__restrict__ tw, const float2 *__restrict__ fold) {
+constexpr int kLagThreads = 1024;
+constexpr int kLagRegs = (kLdsComplexCapacity + kLagThreads - 1) / kLagThreads;
+
+__device__ __forceinline__ void lag_prefetch(float2 (&pre)[kLagRegs], const float2 *__restrict__ X,
+                                             int nspec) {
+#pragma unroll
+  for (int i = 0; i < kLagRegs; ++i) {
+    const int k = threadIdx.x + i * kLagThreads;
+    if (k < nspec) pre[i] = X[k];
+  }
+}
+
+__global__ void __launch_bounds__(kLagThreads)
+lag_kernel(const float2 *__restrict__ xspec, long long xpitch, int nspec, long long nrows,
+           const int *__restrict__ row_out, float *__restrict__ cc, int half_cc, double weight,
+           FftPlanDev plan, const float2 *__restrict__ tw, const float2 *__restrict__ fold) {
   extern __shared__ float2 lds[];
-  const int row = blockIdx.x;
-  const float2 *X = xspec + (size_t)row * nspec;
-  const int M = plan.n;
-  c2r_fold(lds, M, fold, [&](int k) { return X[k]; });
-  lds_fft(lds, 1, plan, tw, 1.f);
-  const int out_row = row_out ? row_out[row] : row;
-  float *y = cc + (size_t)out_row * (2 * half_cc + 1);
-  for (int m = threadIdx.x; m <= 2 * half_cc; m += blockDim.x) {
-    const int l = m - half_cc;
-    const int par = l & 1;
-    int n = (l - par) / 2;
-    n = n < 0 ? n + M : n;
-    const float2 z = lds[fft_pos(plan, n)];
-    const float v = par ? z.y : z.x;
-    y[m] = (float)((double)v * weight);
+  const int M = plan.n;  // nspec == M + 1
+  stage_real_tables(lds, plan, tw, fold);
+  const float2 *twl = lds + M + 1;
+  const float2 *fl = twl + tw_lds_count(M);
+  float2 pre[kLagRegs];
+  long long row = blockIdx.x;
+  if (row < nrows) lag_prefetch(pre, xspec + row * xpitch, nspec);
+  for (; row < nrows; row += gridDim.x) {
+#pragma unroll
+    for (int i = 0; i < kLagRegs; ++i) {
+      const int k = threadIdx.x + i * kLagThreads;
+      if (k < nspec) lds[k] = pre[i];
+    }
+    __syncthreads();
+    const long long nxt = row + gridDim.x;
+    if (nxt < nrows) lag_prefetch(pre, xspec + nxt * xpitch, nspec);
+    // fold pairs (k, M-k) in place: Z[k] = (X[k] + conj X[M-k]) + i w_k (X[k] - conj X[M-k])
+    for (int k = threadIdx.x; 2 * k <= M; k += blockDim.x) {
+      if (k == 0) {
+        const float a = lds[0].x, b = lds[M].x;  // imaginary parts of DC/Nyquist ignored
+        lds[0] = make_float2(a + b, a - b);
+      } else {
+        const float2 a = lds[k];
+        const float2 b = lds[M - k];
+        const float2 s1 = cadd(a, cconj(b));
+        const float2 d1 = cmul(fold_lookup(fl, k), csub(a, cconj(b)));
+        const float2 zk = make_float2(s1.x - d1.y, s1.y + d1.x);
+        if (2 * k != M) {
+          const float2 s2 = cadd(b, cconj(a));
+          const float2 d2 = cmul(fold_lookup(fl, M - k), csub(b, cconj(a)));
+          lds[M - k] = make_float2(s2.x - d2.y, s2.y + d2.x);
+        }
+        lds[k] = zk;
+      }
+    }
+    __syncthreads();
+    lds_fft(lds, 1, plan, twl, 1.f);
+    const int out_row = row_out ? row_out[row] : (int)row;
+    float *y = cc + (size_t)out_row * (2 * half_cc + 1);
+    for (int m = threadIdx.x; m <= 2 * half_cc; m += blockDim.x) {
+      const int l = m - half_cc;
+      const int par = l & 1;
+      int n = (l - par) / 2;
+      n = n < 0 ? n + M : n;
+      const float2 z = lds[fft_pos(plan, n)];
+      const float v = par ? z.y : z.x;
+      y[m] = (float)((double)v * weight);
+    }
+    __syncthreads();
   }
 }
 
 // Copies workspace rows to their task rows (cross-spectra output order).
-__global__ void scatter_rows_kernel(const float2 *__restrict__ src, const int *__restrict__ row_task,
-                                    int nspec, float2 *__restrict__ dst) {
+__global__ void scatter_rows_kernel(const float2 *__restrict__ src, long long src_pitch,
+                                    const int *__restrict__ row_task, int nspec,
+                                    float2 *__restrict__ dst) {
   const size_t r = blockIdx.x;
   const size_t t = (size_t)row_task[r];
   for (int k = threadIdx.x; k < nspec; k += blockDim.x)
-    dst[t * nspec + k] = src[r * nspec + k];
+    dst[t * nspec + k] = src[r * src_pitch + k];
 }
 
 }  // namespace fxc
@@ -377,6 +477,7 @@ using namespace fxc;
 // ---------------------------------------------------------------------------
 struct fxc_xc_plan {
   int nspec = 0, nstep = 0, nfft = 0, half_cc = 0, cc_size = 0;
+  long long xpitch = 0;  // workspace row pitch (complex), a multiple of 16 (128 bytes)
   float dt = 0.f;
   int precision = FXC_PREC_FP32;
   int kh = 0;
@@ -452,7 +553,7 @@ extern "C" int fxc_xc_plan_create(fxc_xc_plan **out, int nspec, int nstep, float
     FXC_CHECK_ARG(nstep <= 128, "fxc_xc_plan_create: MFMA forms support up to 128 windows");
   const RealFftPlan *rp = nullptr;
   FXC_TRY(plan_cache_real(nfft, 1, &rp));
-  FXC_CHECK_ARG(max_transforms_per_wg(rp->half.dev, 1024, kLdsComplexCapacity) >= 1,
+  FXC_CHECK_ARG(real_lds_count(nfft / 2) <= kLdsComplexCapacity,
                 "fxc_xc_plan_create: nfft %d does not fit one workgroup", nfft);
   fxc_xc_plan *p = new fxc_xc_plan();
   p->nspec = nspec;
@@ -462,7 +563,8 @@ extern "C" int fxc_xc_plan_create(fxc_xc_plan **out, int nspec, int nstep, float
   p->cc_size = 2 * half_cc + 1;
   p->dt = dt;
   p->precision = precision;
-  p->kh = (nstep + 7) / 8 * 8;
+  p->xpitch = (nspec + 15) / 16 * 16;
+  p->kh = (nstep + 15) / 16 * 16;  // K' = 2*kh is a whole number of 16x16x32 steps
   p->chunk_rows_max = 0;
   *out = p;
   return FXC_OK;
@@ -615,7 +717,7 @@ static int xc_ensure_workspace(fxc_xc_plan *p, size_t rows) {
     dfree(p->d_xspec);
     p->d_xspec = nullptr;
     p->xspec_rows = 0;
-    FXC_TRY(dmalloc((void **)&p->d_xspec, rows * (size_t)p->nspec * sizeof(float2)));
+    FXC_TRY(dmalloc((void **)&p->d_xspec, rows * (size_t)p->xpitch * sizeof(float2)));
     p->xspec_rows = rows;
   }
   if (p->precision != FXC_PREC_FP32) {
@@ -665,52 +767,91 @@ static int xc_prepare_spectra(fxc_xc_plan *p, const float2 *d_spec, hipStream_t 
   return FXC_OK;
 }
 
+template <int kMode, int KH>
+static int launch_mfma_kh(dim3 grid, size_t lds, hipStream_t st, const unsigned short *hi,
+                          const unsigned short *lo, int nsta, int nspec, const float *sc,
+                          const TileDesc *tiles, const int *trows, float2 *dst, long long xpitch,
+                          float scale) {
+  auto kern = contract_mfma_kernel<kMode, KH / 16>;
+  const dim3 grid1(grid.x);
+  FXC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds));
+  hipLaunchKernelGGL(kern, grid1, dim3(512), lds, st, hi, lo, nsta, nspec, sc, tiles, trows, dst,
+                     xpitch, (int)(grid.y), scale);
+  return FXC_OK;
+}
+
+template <int kMode>
+static int launch_mfma_mode(int kh, dim3 grid, size_t lds, hipStream_t st,
+                            const unsigned short *hi, const unsigned short *lo, int nsta, int nspec,
+                            const float *sc, const TileDesc *tiles, const int *trows, float2 *dst,
+                            long long xpitch, float scale) {
+#define FXC_KH(V) \
+  case V: return launch_mfma_kh<kMode, V>(grid, lds, st, hi, lo, nsta, nspec, sc, tiles, trows, dst, xpitch, scale);
+  switch (kh) {
+    FXC_KH(16) FXC_KH(32) FXC_KH(48) FXC_KH(64) FXC_KH(80) FXC_KH(96) FXC_KH(112) FXC_KH(128)
+    default: break;
+  }
+#undef FXC_KH
+  set_error("MFMA contraction: unsupported window padding %d", kh);
+  return FXC_ERR_UNSUPPORTED;
+}
+
+static int launch_mfma_contract(int precision, int kh, dim3 grid, size_t lds, hipStream_t st,
+                                const unsigned short *hi, const unsigned short *lo, int nsta,
+                                int nspec, const float *sc, const TileDesc *tiles,
+                                const int *trows, float2 *dst, long long xpitch, float scale) {
+  if (precision == FXC_PREC_BF16X3)
+    return launch_mfma_mode<FXC_PREC_BF16X3>(kh, grid, lds, st, hi, lo, nsta, nspec, sc, tiles,
+                                             trows, dst, xpitch, scale);
+  return launch_mfma_mode<FXC_PREC_FP16>(kh, grid, lds, st, hi, lo, nsta, nspec, sc, tiles, trows,
+                                         dst, xpitch, scale);
+}
+
 static int xc_contract_chunk(fxc_xc_plan *p, const float2 *d_spec, long long c, float2 *dst,
                              hipStream_t st) {
   const long long t0 = p->chunk_tile_begin[c], t1 = p->chunk_tile_begin[c + 1];
   const unsigned ntl = (unsigned)(t1 - t0);
   const unsigned nbc = (unsigned)ceil_div(p->nspec, kBinChunk);
+  // 1-D grid covering ntl tiles x nbc chunks in groups of kChunkGroup chunks
+  const unsigned nwg = ntl * (unsigned)(ceil_div(nbc, kChunkGroup) * kChunkGroup);
   const float scale = 1.0f / (float)p->nstep;
   const TileDesc *tiles = p->d_tiles + t0;
   const int *trows = p->d_tile_rows + t0 * kTilePairs;
   FXC_TRY(xc_mark(p, kTimeContract, true, st));
   if (p->precision == FXC_PREC_FP32) {
-    hipLaunchKernelGGL(contract_fp32_kernel, dim3(ntl, nbc), dim3(256), 0, st, d_spec, p->nstep,
-                       p->nsta, p->nspec, tiles, trows, dst, scale);
+    hipLaunchKernelGGL(contract_fp32_kernel, dim3(nwg), dim3(256), 0, st, d_spec, p->nstep,
+                       p->nsta, p->nspec, tiles, trows, dst, p->xpitch, (int)ntl, scale);
   } else {
     const size_t lds = (size_t)kTilePairs * kBinChunk * sizeof(float2);
-    if (p->precision == FXC_PREC_BF16X3) {
-      FXC_HIP(hipFuncSetAttribute((const void *)contract_mfma_kernel<FXC_PREC_BF16X3>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL(contract_mfma_kernel<FXC_PREC_BF16X3>, dim3(ntl, nbc), dim3(512), lds, st,
-                         p->d_hi, p->d_lo, p->nsta, p->nspec, p->kh, (const float *)nullptr, tiles,
-                         trows, dst, scale);
-    } else {
-      FXC_HIP(hipFuncSetAttribute((const void *)contract_mfma_kernel<FXC_PREC_FP16>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL(contract_mfma_kernel<FXC_PREC_FP16>, dim3(ntl, nbc), dim3(512), lds, st,
-                         p->d_hi, p->d_lo, p->nsta, p->nspec, p->kh, p->d_scale, tiles, trows, dst,
-                         scale);
-    }
+    // grid.y carries the tile count to the launcher (the launch itself is 1-D)
+    FXC_TRY(launch_mfma_contract(p->precision, p->kh, dim3(nwg, ntl), lds, st, p->d_hi, p->d_lo,
+                                 p->nsta, p->nspec, p->d_scale, tiles, trows, dst, p->xpitch,
+                                 scale));
   }
   FXC_HIP(hipGetLastError());
   FXC_TRY(xc_mark(p, kTimeContract, false, st));
   return FXC_OK;
 }
 
-static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long nrows, const int *row_out,
-                        float *d_cc, hipStream_t st) {
+static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long long nrows,
+                        const int *row_out, float *d_cc, hipStream_t st) {
   if (nrows == 0) return FXC_OK;
   const RealFftPlan *rp = nullptr;
   FXC_TRY(plan_cache_real(p->nfft, 1, &rp));
-  const size_t lds = (size_t)(p->nfft / 2) * sizeof(float2);
+  const size_t lds = (size_t)real_lds_count(p->nfft / 2) * sizeof(float2);
   FXC_HIP(hipFuncSetAttribute((const void *)lag_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
+  int dev = 0, ncu = 0;
+  FXC_HIP(hipGetDevice(&dev));
+  FXC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const long long grid = std::min<long long>(nrows, (long long)std::max(ncu, 1));
   // reference weight: 1.0 / (width * dt) with width converted to float
   const double weight = 1.0 / (double)((float)p->nfft * p->dt);
   FXC_TRY(xc_mark(p, kTimeLag, true, st));
-  hipLaunchKernelGGL(lag_kernel, dim3((unsigned)nrows), dim3(1024), lds, st, xs, p->nspec, row_out,
-                     d_cc, p->half_cc, weight, rp->half.dev, rp->half.d_tw, rp->d_fold);
+  hipLaunchKernelGGL(lag_kernel, dim3((unsigned)grid), dim3(kLagThreads), lds, st, xs, xpitch,
+                     p->nspec, nrows, row_out, d_cc, p->half_cc, weight, rp->half.dev,
+                     rp->half.d_tw, rp->d_fold);
   FXC_HIP(hipGetLastError());
   FXC_TRY(xc_mark(p, kTimeLag, false, st));
   return FXC_OK;
@@ -734,7 +875,7 @@ extern "C" int fxc_xc_cross_spectra(fxc_xc_plan *p, const float *d_spec, float *
     const long long r0 = p->chunk_row_begin[c], r1 = p->chunk_row_begin[c + 1];
     if (r1 > r0) {
       hipLaunchKernelGGL(scatter_rows_kernel, dim3((unsigned)(r1 - r0)), dim3(256), 0, st,
-                         p->d_xspec, p->d_row_task + r0, p->nspec, (float2 *)d_xspec);
+                         p->d_xspec, p->xpitch, p->d_row_task + r0, p->nspec, (float2 *)d_xspec);
       FXC_HIP(hipGetLastError());
     }
   }
@@ -745,7 +886,8 @@ extern "C" int fxc_xc_cross_spectra(fxc_xc_plan *p, const float *d_spec, float *
 extern "C" int fxc_xc_lags(fxc_xc_plan *p, const float *d_xspec, long long nrows, float *d_cc,
                            void *stream) {
   FXC_CHECK_ARG(p && d_xspec && d_cc && nrows >= 0, "fxc_xc_lags: invalid arguments");
-  return xc_lags_rows(p, (const float2 *)d_xspec, nrows, nullptr, d_cc, (hipStream_t)stream);
+  return xc_lags_rows(p, (const float2 *)d_xspec, p->nspec, nrows, nullptr, d_cc,
+                      (hipStream_t)stream);
 }
 
 // Full correlation: cc[task][0..2H] for every task set by fxc_xc_set_pairs.
@@ -762,7 +904,7 @@ extern "C" int fxc_xc_correlate(fxc_xc_plan *p, const float *d_spec, float *d_cc
   for (long long c = 0; c < nc; ++c) {
     FXC_TRY(xc_contract_chunk(p, (const float2 *)d_spec, c, p->d_xspec, st));
     const long long r0 = p->chunk_row_begin[c], r1 = p->chunk_row_begin[c + 1];
-    FXC_TRY(xc_lags_rows(p, p->d_xspec, r1 - r0, p->d_row_task + r0, d_cc, st));
+    FXC_TRY(xc_lags_rows(p, p->d_xspec, p->xpitch, r1 - r0, p->d_row_task + r0, d_cc, st));
   }
   return FXC_OK;
 }

@@ -27,7 +27,8 @@ def _all_pairs(nsta):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16"])
-@pytest.mark.parametrize("nsta,nstep,seg,nfft,H", [(40, 5, 800, 882 * 2, 100), (9, 24, 1500, 1764, 300)])
+@pytest.mark.parametrize("nsta,nstep,seg,nfft,H", [(40, 5, 800, 882 * 2, 100), (9, 24, 1500, 1764, 300),
+                                                    (33, 47, 600, 1764, 200)])
 def test_correlate_matches_oracle(gpu, precision, nsta, nstep, seg, nfft, H):
     import torch
     from fastxc_amd.ops import XcPlan
@@ -83,7 +84,9 @@ def test_delayed_copy_lag_position_exact(gpu):
     plan.set_pairs(2, np.array([0], np.int32), np.array([1], np.int32))
     cc = plan.correlate(torch.from_numpy(spec).to(gpu)).cpu().numpy()[0]
     want = xc.correlate(spec, [0], [1], nfft, H, dt)[0]
-    assert int(np.argmax(cc)) == int(np.argmax(want)) == H - d
+    # receiver delayed by d (rec(t) = src(t - d)): conj on the source side puts
+    # the peak at lag +d, i.e. output index H + d
+    assert int(np.argmax(cc)) == int(np.argmax(want)) == H + d
 
 
 def test_chunked_matches_unchunked(gpu):
