@@ -21,6 +21,15 @@ PRECISIONS = {"fp32": FXC_PREC_FP32, "bf16x3": FXC_PREC_BF16X3, "fp16": FXC_PREC
 _lib = None
 
 
+class S2sConfig(ctypes.Structure):
+    """fxc_s2s_config (include/fastxc_amd.h)."""
+    _fields_ = [("npts", c_int), ("seg", c_int), ("shift", c_int), ("nfft_filter", c_int),
+                ("nfft_out", c_int), ("num_ch", c_int), ("delta", c_float),
+                ("freq_low", c_float), ("idx", c_int * 4), ("whiten", c_int),
+                ("normalize", c_int), ("phase_only", c_int), ("filter_count", c_int),
+                ("responses", c_void_p), ("freq_lows", c_void_p)]
+
+
 class FxcError(RuntimeError):
     pass
 
@@ -46,6 +55,11 @@ _SIGNATURES = {
     "fxc_xc_lags": (c_int, [c_void_p, c_void_p, c_longlong, c_void_p, c_void_p]),
     "fxc_xc_correlate": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "fxc_xc_set_timing": (c_int, [c_void_p, c_int]),
+    "fxc_s2s_plan_create": (c_int, [POINTER(c_void_p), c_void_p]),
+    "fxc_s2s_plan_destroy": (c_int, [c_void_p]),
+    "fxc_s2s_plan_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int),
+                                  POINTER(c_int)]),
+    "fxc_s2s_process": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "fxc_xc_get_timing": (c_int, [c_void_p, POINTER(ctypes.c_double), POINTER(c_longlong)]),
 }
 

@@ -91,9 +91,11 @@ struct FftPlan {
 
 // Factor n into radices this library implements; returns false when n has
 // a prime factor with no butterfly (then a Bluestein plan is needed).
-bool factor_radices(int n, int *radix, int *npass);
+// Radices above max_radix are not used (kernels with tight register budgets
+// ask for small butterflies).
+bool factor_radices(int n, int *radix, int *npass, int max_radix = 32);
 // Builds the plan and uploads its FP64-computed twiddle table.
-int fft_plan_create(FftPlan *plan, int n, int sign);
+int fft_plan_create(FftPlan *plan, int n, int sign, int max_radix = 32);
 void fft_plan_destroy(FftPlan *plan);
 
 // Device memory helpers that record an error on failure.

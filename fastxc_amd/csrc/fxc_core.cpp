@@ -35,14 +35,16 @@ void dfree(void *ptr) {
   if (ptr) (void)hipFree(ptr);
 }
 
-// Radix preference: large butterflies first keeps the pass count low; the
-// odd primes 23 and 29 cover the correlation lengths 2001 = 3*23*29.
-bool factor_radices(int n, int *radix, int *npass) {
-  static const int prefs[] = {8, 7, 5, 4, 3, 2, 23, 29};
+// Radix preference: large register-resident butterflies first keeps the LDS
+// pass count low (18522 = 21*21*21*2, 18000 = 16*15*15*5, 18144 = 21*16*9*6);
+// the odd primes 23 and 29 cover the correlation length 2001 = 3*23*29.
+bool factor_radices(int n, int *radix, int *npass, int max_radix) {
+  static const int prefs[] = {21, 16, 15, 14, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2, 23, 29};
   int m = n;
   int cnt = 0;
   if (n < 1) return false;
   for (int r : prefs) {
+    if (r > max_radix && r != 23 && r != 29) continue;
     while (m % r == 0 && m > 1) {
       if (cnt >= kMaxPasses) return false;
       radix[cnt++] = r;
@@ -58,10 +60,10 @@ bool factor_radices(int n, int *radix, int *npass) {
   return true;
 }
 
-int fft_plan_create(FftPlan *plan, int n, int sign) {
+int fft_plan_create(FftPlan *plan, int n, int sign, int max_radix) {
   FftPlanDev d{};
   d.n = n;
-  if (!factor_radices(n, d.radix, &d.npass)) {
+  if (!factor_radices(n, d.radix, &d.npass, max_radix)) {
     set_error("fft length %d has a prime factor without a butterfly", n);
     return FXC_ERR_UNSUPPORTED;
   }

@@ -8,9 +8,10 @@
 // every consumer reads it through fft_pos(), which the fused kernels need
 // anyway because they only read the bins or lags they keep.
 //
-// Butterflies are written for radices 2, 3, 4, 5, 7, 8 and the odd primes the
-// correlation lengths need (23, 29); all constants are computed at compile
-// time in double precision and rounded once to float.
+// Butterflies are written for radices 2, 3, 4, 5, 7, 8, the composites 6, 9,
+// 10, 12, 14, 15, 16, 21 (two register-resident stages each) and the odd
+// primes the correlation lengths need (23, 29); all constants are computed at
+// compile time in double precision and rounded once to float.
 #pragma once
 
 #include "fxc_common.hpp"
@@ -152,6 +153,48 @@ __device__ __forceinline__ void dft<8>(float2 *a, float sgn) {
   }
 }
 
+// Composite DFT of R = R1*R2 values in registers (one Cooley-Tukey split:
+// R2 DFTs of length R1, internal twiddles exp(sgn*2*pi*i*n2*k1/R) from a
+// compile-time table, R1 DFTs of length R2):
+//   X[k1 + R1*k2] = sum_n2 W_R2^(n2 k2) W_R^(n2 k1) sum_n1 a[R2 n1 + n2] W_R1^(n1 k1).
+template <int R1, int R2>
+__device__ __forceinline__ void dft_composite(float2 *a, float sgn) {
+  constexpr int R = R1 * R2;
+  constexpr RootTable<R> tb{};
+  float2 t[R2][R1];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    float2 x[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) x[n1] = a[R2 * n1 + n2];
+    dft<R1>(x, sgn);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const int idx = (n2 * k1) % R;
+      if (idx != 0) x[k1] = cmul(x[k1], make_float2(tb.c[idx], sgn * tb.s[idx]));
+      t[n2][k1] = x[k1];
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    float2 y[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) y[n2] = t[n2][k1];
+    dft<R2>(y, sgn);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) a[k1 + R1 * k2] = y[k2];
+  }
+}
+
+template <> __device__ __forceinline__ void dft<6>(float2 *a, float sgn) { dft_composite<2, 3>(a, sgn); }
+template <> __device__ __forceinline__ void dft<9>(float2 *a, float sgn) { dft_composite<3, 3>(a, sgn); }
+template <> __device__ __forceinline__ void dft<10>(float2 *a, float sgn) { dft_composite<2, 5>(a, sgn); }
+template <> __device__ __forceinline__ void dft<12>(float2 *a, float sgn) { dft_composite<4, 3>(a, sgn); }
+template <> __device__ __forceinline__ void dft<14>(float2 *a, float sgn) { dft_composite<2, 7>(a, sgn); }
+template <> __device__ __forceinline__ void dft<15>(float2 *a, float sgn) { dft_composite<3, 5>(a, sgn); }
+template <> __device__ __forceinline__ void dft<16>(float2 *a, float sgn) { dft_composite<4, 4>(a, sgn); }
+template <> __device__ __forceinline__ void dft<21>(float2 *a, float sgn) { dft_composite<3, 7>(a, sgn); }
+
 // Largest butterfly count per thread a plan may need (host-side check only;
 // threads loop over their butterflies one at a time).
 constexpr int kMaxElemsPerThread = 1 << 20;
@@ -269,7 +312,9 @@ __device__ inline void stage_twiddles(float2 *tw_lds, const FftPlanDev &plan,
 // Radix sets a kernel compiles butterflies for (bit r = radix r).  Kernels on
 // 2,3,5,7-smooth lengths leave out the large odd primes so 1024-thread
 // workgroups stay within 128 VGPRs.
-constexpr unsigned kRadixSmooth = (1u << 2) | (1u << 3) | (1u << 4) | (1u << 5) | (1u << 7) | (1u << 8);
+constexpr unsigned kRadixSmooth = (1u << 2) | (1u << 3) | (1u << 4) | (1u << 5) | (1u << 6) |
+                                  (1u << 7) | (1u << 8) | (1u << 9) | (1u << 10) | (1u << 12) |
+                                  (1u << 14) | (1u << 15) | (1u << 16) | (1u << 21);
 constexpr unsigned kRadixAll = kRadixSmooth | (1u << 23) | (1u << 29);
 
 #define FXC_RADIX_SWITCH(FN)                                                  \
@@ -280,6 +325,14 @@ constexpr unsigned kRadixAll = kRadixSmooth | (1u << 23) | (1u << 29);
     case 5: if constexpr ((kMask >> 5) & 1u) FN<5>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break;   \
     case 7: if constexpr ((kMask >> 7) & 1u) FN<7>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break;   \
     case 8: if constexpr ((kMask >> 8) & 1u) FN<8>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break;   \
+    case 6: if constexpr ((kMask >> 6) & 1u) FN<6>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break;   \
+    case 9: if constexpr ((kMask >> 9) & 1u) FN<9>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break;   \
+    case 10: if constexpr ((kMask >> 10) & 1u) FN<10>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break; \
+    case 12: if constexpr ((kMask >> 12) & 1u) FN<12>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break; \
+    case 14: if constexpr ((kMask >> 14) & 1u) FN<14>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break; \
+    case 15: if constexpr ((kMask >> 15) & 1u) FN<15>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break; \
+    case 16: if constexpr ((kMask >> 16) & 1u) FN<16>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break; \
+    case 21: if constexpr ((kMask >> 21) & 1u) FN<21>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break; \
     case 23: if constexpr ((kMask >> 23) & 1u) FN<23>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break; \
     case 29: if constexpr ((kMask >> 29) & 1u) FN<29>(buf, ntrans, plan, p, tw, sgn, tid, nthr); break; \
     default: break;                                                           \

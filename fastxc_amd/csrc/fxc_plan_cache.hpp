@@ -10,8 +10,9 @@ namespace fxc {
 // including the FFT twiddle table.
 constexpr int kLdsComplexCapacity = 20480;
 
-int plan_cache_complex(int n, int sign, const FftPlan **out);
-int plan_cache_real(int n, int sign, const RealFftPlan **out);
+// max_radix bounds the butterfly size (part of the cache key).
+int plan_cache_complex(int n, int sign, const FftPlan **out, int max_radix = 32);
+int plan_cache_real(int n, int sign, const RealFftPlan **out, int max_radix = 32);
 void plan_cache_release();
 
 }  // namespace fxc

@@ -1,0 +1,786 @@
+// sac2spec hot path: per station-window preprocessing, time normalisation,
+// spectral whitening and the output spectrum.
+//
+// Replaces ComputeWorkerBatch (native/sac2spec/worker/compute.cu:8-272) and
+// the ~30 element-wise kernels and 4-5 cuFFT passes it launches per batch.
+// Rows are station-windows ordered [step][file], file = group*nch + ch; a
+// frame (step, group) owns nch consecutive rows that share the normalisation
+// and whitening weights (core/normalization.cu:91-213, core/spectrum.cu:178-305).
+//
+// MI355X design: every per-row stage is one workgroup holding the row's whole
+// transform in LDS.  Real data sit in LDS as floats, which is the packed
+// complex sequence z[j] = x[2j] + i x[2j+1] of the half-length FFT, so
+// packing is free.  Forward transforms are DIF (natural in, digit-reversed
+// out), inverse transforms DIT (digit-reversed in, natural out): the
+// point-wise filter between them runs in digit-reversed order and no
+// permutation pass exists.  Demean/detrend/taper, ×dt, sanitising, |H|^2,
+// the normalisation divide and clip, whitening divide and taper, and the
+// phase-only option are all folded into those passes.  Sums the reference
+// takes in FP64 (mean, trend, running means) stay FP64.
+#include "fxc_plan_cache.hpp"
+#include "fastxc_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace fxc {
+
+constexpr int kS2sThreads = 1024;
+constexpr float kMaxVal = 20.f;        // MAXVAL, include/config.h:13
+constexpr float kMinVal = 1e-30f;      // MINVAL as a float argument
+constexpr double kMinValD = 1e-30;     // MINVAL in double comparisons
+
+enum { kLoadPlain = 0, kLoadRunabs = 1, kLoadOnebit = 2 };
+
+struct RealTables {
+  FftPlanDev plan;    // half-length complex plan (radix structure)
+  const float2 *tw_fwd;
+  const float2 *tw_inv;
+  const float2 *fold;  // exp(-2 pi i k / n), two-level, k <= n/2
+};
+
+__device__ __forceinline__ bool cfinite(float2 v) { return isfinite(v.x) && isfinite(v.y); }
+
+// Block-wide double sums (three at once).  Uses a small static LDS area.
+__device__ inline void block_sum3(double &a, double &b, double &c) {
+  __shared__ double red[3][kS2sThreads / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off);
+    b += __shfl_down(b, off);
+    c += __shfl_down(c, off);
+  }
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = a;
+    red[1][w] = b;
+    red[2][w] = c;
+  }
+  __syncthreads();
+  a = b = c = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+    a += red[0][i];
+    b += red[1][i];
+    c += red[2][i];
+  }
+}
+
+// LDS tables: [twiddles fwd | twiddles inv | fold] after `data_slots`.
+__device__ inline const float2 *stage_tables(float2 *lds, int data_slots, const RealTables &t,
+                                             const float2 **tw_inv, const float2 **fold) {
+  const int M = t.plan.n;
+  const int ntw = kTwFine + t.plan.ncoarse;
+  float2 *a = lds + data_slots;
+  float2 *b = a + ntw;
+  float2 *c = b + ntw;
+  for (int i = threadIdx.x; i < ntw; i += blockDim.x) {
+    a[i] = t.tw_fwd[i];
+    b[i] = t.tw_inv[i];
+  }
+  for (int i = threadIdx.x; i < fold_lds_count(M); i += blockDim.x) c[i] = t.fold[i];
+  *tw_inv = b;
+  *fold = c;
+  return a;
+}
+
+__host__ __device__ inline int tables_count(int M) { return 2 * tw_lds_count(M) + fold_lds_count(M); }
+
+// After a forward DIF of the packed real sequence (length 2M), visits the
+// spectrum pairwise: f(k, X[k], X[M-k]) returns the new values (Y[k],
+// Y[M-k]), which are refolded in place for a DIT inverse (w = conj fold).
+// k = 0 carries (X[0], X[M]).  Pairs are disjoint, so no barrier is needed
+// inside; one barrier at the end.
+template <class F>
+__device__ inline void spectral_pairs(float2 *lds, const FftPlanDev &plan, const float2 *fold,
+                                      const F &f) {
+  const int M = plan.n;
+  for (int k = threadIdx.x; 2 * k <= M; k += blockDim.x) {
+    const int pk = fft_pos(plan, k);
+    const int pm = fft_pos(plan, k == 0 ? 0 : M - k);
+    const float2 a = lds[pk];
+    const float2 b = cconj(lds[pm]);
+    float2 xk, xm;
+    if (k == 0) {
+      xk = make_float2(a.x + a.y, 0.f);  // X[0]
+      xm = make_float2(a.x - a.y, 0.f);  // X[M]
+    } else {
+      const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+      const float2 d = csub(a, b);
+      const float2 o = make_float2(0.5f * d.y, -0.5f * d.x);
+      xk = cadd(e, cmul(fold_lookup(fold, k), o));
+      // X[M-k] from the same pair: swap roles of a and b
+      const float2 a2 = lds[pm];
+      const float2 b2 = cconj(lds[pk]);
+      const float2 e2 = make_float2(0.5f * (a2.x + b2.x), 0.5f * (a2.y + b2.y));
+      const float2 d2 = csub(a2, b2);
+      const float2 o2 = make_float2(0.5f * d2.y, -0.5f * d2.x);
+      xm = cadd(e2, cmul(fold_lookup(fold, M - k), o2));
+    }
+    float2 yk, ym;
+    f(k, xk, xm, yk, ym);
+    // refold for the inverse: Z[k] = (Y[k] + conj Y[M-k]) + i conj(w_k) (Y[k] - conj Y[M-k])
+    if (k == 0) {
+      lds[pk] = make_float2(yk.x + ym.x, yk.x - ym.x);
+    } else {
+      const float2 s1 = cadd(yk, cconj(ym));
+      const float2 d1 = cmul(cconj(fold_lookup(fold, k)), csub(yk, cconj(ym)));
+      const float2 zk = make_float2(s1.x - d1.y, s1.y + d1.x);
+      if (2 * k != M) {
+        const float2 s2 = cadd(ym, cconj(yk));
+        const float2 d2 = cmul(cconj(fold_lookup(fold, M - k)), csub(ym, cconj(yk)));
+        lds[pm] = make_float2(s2.x - d2.y, s2.y + d2.x);
+      }
+      lds[pk] = zk;
+    }
+  }
+  __syncthreads();
+}
+
+// After a forward DIF, emits X[k] (k = 0..M) in natural order: st(k, X[k]).
+template <class S>
+__device__ inline void spectral_emit(const float2 *lds, const FftPlanDev &plan, const float2 *fold,
+                                     const S &st) {
+  r2c_unfold(lds, plan, fold, st);
+}
+
+// Loads one row of the time series into the float view of LDS applying the
+// pending time-domain normalisation (reference order: x/w with 0 for an
+// invalid weight, non-finite -> 0, clip to +-MAXVAL; or sign(x)), then zero
+// pads to 2M samples.
+__device__ inline void load_row(float *ldsf, int twoM, const float *__restrict__ x, int seg,
+                                int mode, const float *__restrict__ w) {
+  for (int i = threadIdx.x; i < twoM; i += blockDim.x) {
+    float v = 0.f;
+    if (i < seg) {
+      v = x[i];
+      if (mode == kLoadRunabs) {
+        const float d = w[i];
+        v = (!isfinite(d) || d < kMinValD) ? 0.f : v / d;
+        if (!isfinite(v)) v = 0.f;
+        v = v > kMaxVal ? kMaxVal : (v < -kMaxVal ? -kMaxVal : v);
+      } else if (mode == kLoadOnebit) {
+        v = v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f);
+      }
+    }
+    ldsf[i] = v;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// K1: window cut + preprocess + broad-band prefilter (compute.cu:34-86).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kS2sThreads)
+s2s_prefilter_kernel(const float *__restrict__ traces, int npts, int nfiles,
+                     const int *__restrict__ steps, int shift, int seg, int taper_size,
+                     RealTables tf, const float *__restrict__ resp0, float dt, double inv_weight,
+                     float *__restrict__ xout) {
+  extern __shared__ float2 lds[];
+  float *ldsf = (float *)lds;
+  const FftPlanDev &plan = tf.plan;
+  const int M = plan.n;
+  const float2 *tw_inv, *fold;
+  const float2 *tw_fwd = stage_tables(lds, M + 1, tf, &tw_inv, &fold);
+  const int row = blockIdx.x;
+  const int step = row / nfiles, file = row - step * nfiles;
+  const long long off = (long long)steps[step] * shift;
+  const float *src = traces + (size_t)file * npts;
+  // load + non-finite -> 0 + FP64 sum
+  double s = 0.0, unused1 = 0.0, unused2 = 0.0;
+  for (int i = threadIdx.x; i < 2 * M; i += blockDim.x) {
+    float v = 0.f;
+    if (i < seg && off + i < npts) v = src[off + i];
+    if (!isfinite(v)) v = 0.f;
+    ldsf[i] = v;
+    if (i < seg) s += v;
+  }
+  block_sum3(s, unused1, unused2);
+  // demean: x -= sum/W (double), then sums for the linear trend
+  const double mean = s / seg;
+  double s1 = 0.0, si = 0.0;
+  for (int i = threadIdx.x; i < seg; i += blockDim.x) {
+    const float v = (float)((double)ldsf[i] - mean);
+    ldsf[i] = v;
+    s1 += v;
+    si += (double)v * i;
+  }
+  unused1 = 0.0;
+  block_sum3(s1, si, unused1);
+  const double W = seg;
+  const double a12 = 0.5 * W * (W - 1);
+  const double a11 = a12 * (2 * W - 1) / 3.;
+  const double a22 = W;
+  const double det = a11 * a22 - a12 * a12;
+  const double ca = (a22 * si - a12 * s1) / det;
+  const double cb = (a11 * s1 - a12 * si) / det;
+  for (int i = threadIdx.x; i < seg; i += blockDim.x) {
+    float v = (float)((double)ldsf[i] - (ca * i + cb));
+    float factor = 1.f;
+    if (i < taper_size)
+      factor = (float)(0.5 * (1.0 - cos(M_PI * i / taper_size)));
+    else if (i >= seg - taper_size)
+      factor = (float)(0.5 * (1.0 - cos(M_PI * (seg - i) / taper_size)));
+    ldsf[i] = v * factor;
+  }
+  __syncthreads();
+  lds_fft(lds, 1, plan, tw_fwd, -1.f);
+  spectral_pairs(lds, plan, fold, [&](int k, float2 xk, float2 xm, float2 &yk, float2 &ym) {
+    xk = cscale(xk, dt);
+    xm = cscale(xm, dt);
+    if (!cfinite(xk)) xk = make_float2(0.f, 0.f);
+    if (!cfinite(xm)) xm = make_float2(0.f, 0.f);
+    yk = cscale(xk, resp0[k]);
+    ym = cscale(xm, resp0[k == 0 ? M : M - k]);
+  });
+  lds_fft_dit(lds, 1, plan, tw_inv, 1.f);
+  float *y = xout + (size_t)row * seg;
+  for (int i = threadIdx.x; i < seg; i += blockDim.x) y[i] = (float)((double)ldsf[i] * inv_weight);
+}
+
+// ---------------------------------------------------------------------------
+// Running-mean weights shared by a frame's channels:
+//   w[t] = sum_ch (1/W) sum_{i=t-W/2}^{t-W/2+W-1} |x_ch[i]|  (clipped edges)
+// (smoothRowsRollingKernel, kernels/real_matrix.cu:155-191, summed by
+// sum2DKernel and clamped by clampmin2DKernel).  Each thread slides an FP64
+// window over a contiguous block of outputs.  `cplx` selects |X| of complex
+// rows (whitening) instead of |x|.
+// ---------------------------------------------------------------------------
+constexpr int kWeightChunk = 128;  // consecutive outputs per thread
+
+template <bool kCplx>
+__global__ void __launch_bounds__(256)
+s2s_weight_kernel(const void *__restrict__ data, long long row_pitch, int width, int nch, int win,
+                  float *__restrict__ wout, long long w_pitch) {
+  const int frame = blockIdx.y;
+  const int t0 = (blockIdx.x * blockDim.x + threadIdx.x) * kWeightChunk;
+  if (t0 >= width) return;
+  const int t1 = min(width, t0 + kWeightChunk);
+  const double inv = 1.0 / (double)win;
+  float *w = wout + (long long)frame * w_pitch;
+  for (int ch = 0; ch < nch; ++ch) {
+    const long long r = (long long)frame * nch + ch;
+    auto val = [&](int i) -> double {
+      if (kCplx) {
+        const float2 z = ((const float2 *)data)[r * row_pitch + i];
+        return (double)sqrtf(z.x * z.x + z.y * z.y);
+      }
+      return (double)fabsf(((const float *)data)[r * row_pitch + i]);
+    };
+    int start = t0 - win / 2;
+    int end = start + win - 1;
+    double sum = 0.0;
+    for (int i = max(start, 0); i <= min(end, width - 1); ++i) sum += val(i);
+    for (int t = t0; t < t1; ++t) {
+      const float v = (float)(inv * sum);
+      w[t] = ch == 0 ? v : w[t] + v;  // this thread owns outputs t0..t1-1
+      if (start >= 0 && start < width) sum -= val(start);
+      ++start;
+      ++end;
+      if (end >= 0 && end < width) sum += val(end);
+    }
+  }
+  for (int t = t0; t < t1; ++t) {
+    float v = w[t];
+    if (!isfinite(v) || v < kMinVal) v = kMinVal;
+    w[t] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Whitening forward: load (normalised) row, R2C(seg) x dt, sanitise; write
+// X[k] and |X[k]|, k < seg/2+1 (core/spectrum.cu:178-230 up to the weight).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kS2sThreads)
+s2s_whiten_fwd_kernel(const float *__restrict__ x, int seg, int nch, int mode,
+                      const float *__restrict__ w, RealTables ts, float dt,
+                      float2 *__restrict__ X, float *__restrict__ amp, long long fpitch) {
+  extern __shared__ float2 lds[];
+  const FftPlanDev &plan = ts.plan;
+  const int M = plan.n;
+  const float2 *tw_inv, *fold;
+  const float2 *tw_fwd = stage_tables(lds, M + 1, ts, &tw_inv, &fold);
+  const int row = blockIdx.x;
+  const int frame = row / nch;
+  load_row((float *)lds, 2 * M, x + (size_t)row * seg, seg, mode,
+           w ? w + (size_t)frame * seg : nullptr);
+  lds_fft(lds, 1, plan, tw_fwd, -1.f);
+  float2 *Xr = X + (size_t)row * fpitch;
+  float *ar = amp + (size_t)row * fpitch;
+  spectral_emit(lds, plan, fold, [&](int k, float2 v) {
+    v = cscale(v, dt);
+    if (!cfinite(v)) v = make_float2(0.f, 0.f);
+    Xr[k] = v;
+    ar[k] = sqrtf(v.x * v.x + v.y * v.y);
+  });
+}
+
+// Spectral taper factor (specTaper2DKernel, kernels/taper.cu:8-69, np = 1).
+__device__ inline float spec_taper(int col, int i1, int i2, int i3, int i4) {
+  if (col < i1) return 0.f;
+  if (col < i2) return (float)((1.0 - cos(M_PI / (i2 - i1) * (col - i1))) / 2.0);
+  if (col >= i3 && col < i4) return (float)((1.0 + cos(M_PI / (i4 - i3) * (col - i3))) / 2.0);
+  if (col >= i4) return 0.f;
+  return 1.f;
+}
+
+// ---------------------------------------------------------------------------
+// Whitening inverse (+ optionally the output spectrum): X/ww (0 if the weight
+// is invalid), taper, sanitise, C2R(seg) x 1/(seg dt) (spectrum.cu:231-305,
+// compute.cu:203-217); then either store x, or zero-pad to nfft_out, R2C x dt,
+// sanitise, optional phase-only and store nspec bins (compute.cu:226-264).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kS2sThreads)
+s2s_whiten_inv_kernel(const float2 *__restrict__ X, long long fpitch, const float *__restrict__ ww,
+                      int nch, int seg, int i1, int i2, int i3, int i4, RealTables ts,
+                      double inv_weight_seg, float *__restrict__ xout, RealTables to, float dt,
+                      int phase_only, float2 *__restrict__ out, int nspec) {
+  extern __shared__ float2 lds[];
+  const FftPlanDev &ps = ts.plan;
+  const int Ms = ps.n;
+  const bool do_out = out != nullptr;
+  const int Mo = do_out ? to.plan.n : Ms;
+  const int data = (Ms > Mo ? Ms : Mo) + 1;
+  const float2 *tws_inv, *fold_s;
+  (void)stage_tables(lds, data, ts, &tws_inv, &fold_s);
+  const float2 *two_inv = nullptr, *fold_o = nullptr, *two_fwd = nullptr;
+  if (do_out) two_fwd = stage_tables(lds, data + tables_count(Ms), to, &two_inv, &fold_o);
+  __syncthreads();
+  const int row = blockIdx.x;
+  const int frame = row / nch;
+  const float2 *Xr = X + (size_t)row * fpitch;
+  const float *wr = ww + (size_t)frame * fpitch;
+  const int nbin = Ms + 1;
+  const int t4 = i4 < nbin ? i4 : nbin;
+  auto whitened = [&](int k) -> float2 {
+    float2 v = Xr[k];
+    const float d = wr[k];
+    v = (!isfinite(d) || d < kMinValD) ? make_float2(0.f, 0.f) : make_float2(v.x / d, v.y / d);
+    v = cscale(v, spec_taper(k, i1, i2, i3, t4));
+    if (!cfinite(v)) v = make_float2(0.f, 0.f);
+    return v;
+  };
+  // fold into digit-reversed positions for the DIT inverse
+  for (int k = threadIdx.x; k < Ms; k += blockDim.x) {
+    float2 a = whitened(k);
+    float2 b = cconj(whitened(Ms - k));
+    if (k == 0) {
+      a.y = 0.f;
+      b.y = 0.f;
+    }
+    const float2 s = cadd(a, b);
+    const float2 d = cmul(cconj(fold_lookup(fold_s, k)), csub(a, b));
+    lds[fft_pos(ps, k)] = make_float2(s.x - d.y, s.y + d.x);
+  }
+  __syncthreads();
+  lds_fft_dit(lds, 1, ps, tws_inv, 1.f);
+  float *ldsf = (float *)lds;
+  if (!do_out) {
+    float *y = xout + (size_t)row * seg;
+    for (int i = threadIdx.x; i < seg; i += blockDim.x) y[i] = (float)((double)ldsf[i] * inv_weight_seg);
+    return;
+  }
+  for (int i = threadIdx.x; i < 2 * Mo; i += blockDim.x)
+    ldsf[i] = i < seg ? (float)((double)ldsf[i] * inv_weight_seg) : 0.f;
+  __syncthreads();
+  lds_fft(lds, 1, to.plan, two_fwd, -1.f);
+  float2 *o = out + (size_t)row * nspec;
+  spectral_emit(lds, to.plan, fold_o, [&](int k, float2 v) {
+    if (k >= nspec) return;
+    v = cscale(v, dt);
+    if (!cfinite(v)) v = make_float2(0.f, 0.f);
+    if (phase_only) {
+      const float a = hypotf(v.x, v.y);
+      v = (!isfinite(a) || a <= kMinVal) ? make_float2(0.f, 0.f) : cscale(v, 1.f / a);
+      if (!cfinite(v)) v = make_float2(0.f, 0.f);
+    }
+    o[k] = v;
+  });
+}
+
+// Output spectrum without whitening-after: load (normalised) row, zero-pad,
+// R2C(nfft_out) x dt, sanitise, optional phase-only (compute.cu:226-264).
+__global__ void __launch_bounds__(kS2sThreads)
+s2s_output_kernel(const float *__restrict__ x, int seg, int nch, int mode,
+                  const float *__restrict__ w, RealTables to, float dt, int phase_only,
+                  float2 *__restrict__ out, int nspec) {
+  extern __shared__ float2 lds[];
+  const FftPlanDev &plan = to.plan;
+  const int M = plan.n;
+  const float2 *tw_inv, *fold;
+  const float2 *tw_fwd = stage_tables(lds, M + 1, to, &tw_inv, &fold);
+  const int row = blockIdx.x;
+  const int frame = row / nch;
+  load_row((float *)lds, 2 * M, x + (size_t)row * seg, seg, mode,
+           w ? w + (size_t)frame * seg : nullptr);
+  lds_fft(lds, 1, plan, tw_fwd, -1.f);
+  float2 *o = out + (size_t)row * nspec;
+  spectral_emit(lds, plan, fold, [&](int k, float2 v) {
+    if (k >= nspec) return;
+    v = cscale(v, dt);
+    if (!cfinite(v)) v = make_float2(0.f, 0.f);
+    if (phase_only) {
+      const float a = hypotf(v.x, v.y);
+      v = (!isfinite(a) || a <= kMinVal) ? make_float2(0.f, 0.f) : cscale(v, 1.f / a);
+      if (!cfinite(v)) v = make_float2(0.f, 0.f);
+    }
+    o[k] = v;
+  });
+}
+
+// RUN-ABS-MF base spectrum: R2C(nfft_filter) x dt, sanitised (normalization.cu:285-329).
+__global__ void __launch_bounds__(kS2sThreads)
+s2s_mf_base_kernel(const float *__restrict__ x, int seg, RealTables tf, float dt,
+                   float2 *__restrict__ base, long long bpitch) {
+  extern __shared__ float2 lds[];
+  const FftPlanDev &plan = tf.plan;
+  const int M = plan.n;
+  const float2 *tw_inv, *fold;
+  const float2 *tw_fwd = stage_tables(lds, M + 1, tf, &tw_inv, &fold);
+  const int row = blockIdx.x;
+  load_row((float *)lds, 2 * M, x + (size_t)row * seg, seg, kLoadPlain, nullptr);
+  lds_fft(lds, 1, plan, tw_fwd, -1.f);
+  float2 *b = base + (size_t)row * bpitch;
+  spectral_emit(lds, plan, fold, [&](int k, float2 v) {
+    v = cscale(v, dt);
+    if (!cfinite(v)) v = make_float2(0.f, 0.f);
+    b[k] = v;
+  });
+}
+
+// RUN-ABS-MF band i: C2R(base x |H_i|^2) x 1/(nfft_filter dt), first seg
+// samples (normalization.cu:331-370).
+__global__ void __launch_bounds__(kS2sThreads)
+s2s_mf_band_kernel(const float2 *__restrict__ base, long long bpitch, RealTables tf,
+                   const float *__restrict__ resp, double inv_weight, int seg,
+                   float *__restrict__ xout) {
+  extern __shared__ float2 lds[];
+  const FftPlanDev &plan = tf.plan;
+  const int M = plan.n;
+  const float2 *tw_inv, *fold;
+  (void)stage_tables(lds, M + 1, tf, &tw_inv, &fold);
+  __syncthreads();
+  const int row = blockIdx.x;
+  const float2 *b = base + (size_t)row * bpitch;
+  for (int k = threadIdx.x; k < M; k += blockDim.x) {
+    float2 a = cscale(b[k], resp[k]);
+    float2 c = cconj(cscale(b[M - k], resp[M - k]));
+    if (k == 0) {
+      a.y = 0.f;
+      c.y = 0.f;
+    }
+    const float2 s = cadd(a, c);
+    const float2 d = cmul(cconj(fold_lookup(fold, k)), csub(a, c));
+    lds[fft_pos(plan, k)] = make_float2(s.x - d.y, s.y + d.x);
+  }
+  __syncthreads();
+  lds_fft_dit(lds, 1, plan, tw_inv, 1.f);
+  const float *ldsf = (const float *)lds;
+  float *y = xout + (size_t)row * seg;
+  for (int i = threadIdx.x; i < seg; i += blockDim.x) y[i] = (float)((double)ldsf[i] * inv_weight);
+}
+
+// total += runabs-normalised band signal (normalization.cu:371-385).
+__global__ void s2s_mf_accum_kernel(const float *__restrict__ xb, const float *__restrict__ w,
+                                    int seg, int nch, long long rows, float *__restrict__ total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * seg) return;
+  const long long row = i / seg;
+  const int t = (int)(i - row * seg);
+  const long long frame = row / nch;
+  float v = xb[i];
+  const float d = w[frame * seg + t];
+  v = (!isfinite(d) || d < kMinValD) ? 0.f : v / d;
+  if (!isfinite(v)) v = 0.f;
+  v = v > kMaxVal ? kMaxVal : (v < -kMaxVal ? -kMaxVal : v);
+  total[i] = total[i] + v;
+}
+
+}  // namespace fxc
+
+using namespace fxc;
+
+// ---------------------------------------------------------------------------
+// Plan object behind the C ABI.
+// ---------------------------------------------------------------------------
+struct fxc_s2s_plan {
+  fxc_s2s_config cfg{};
+  int nspec = 0;
+  int fwidth = 0;          // seg/2 + 1
+  long long fpitch = 0;    // complex row pitch of whitening spectra
+  long long bpitch = 0;    // complex row pitch of the MF base spectrum
+  int taper_size = 0;
+  int runabs_win = 0;
+  int whiten_win = 0;
+  std::vector<int> mf_win;
+  float *d_resp = nullptr;  // [filter_count][nfft_filter/2 + 1]
+  RealTables tf{}, ts{}, to{};
+  // workspace (grows on demand)
+  long long cap_rows = 0, cap_frames = 0;
+  float *d_x = nullptr, *d_w = nullptr, *d_xb = nullptr, *d_total = nullptr;
+  float2 *d_X = nullptr, *d_base = nullptr;
+  float *d_amp = nullptr, *d_ww = nullptr;
+  int *d_steps = nullptr;
+  int cap_steps = 0;
+};
+
+static int make_tables(int n, RealTables *t) {
+  const RealFftPlan *f = nullptr, *i = nullptr;
+  FXC_TRY(plan_cache_real(n, -1, &f));
+  FXC_TRY(plan_cache_real(n, 1, &i));
+  t->plan = f->half.dev;
+  t->tw_fwd = f->half.d_tw;
+  t->tw_inv = i->half.d_tw;
+  t->fold = f->d_fold;
+  return FXC_OK;
+}
+
+static size_t lds_bytes(int data_slots, int M) {
+  return ((size_t)data_slots + tables_count(M)) * sizeof(float2);
+}
+
+template <class K>
+static int set_lds(K kern, size_t bytes) {
+  FXC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bytes));
+  return FXC_OK;
+}
+
+extern "C" int fxc_s2s_plan_create(fxc_s2s_plan **out, const fxc_s2s_config *cfg) {
+  FXC_CHECK_ARG(out && cfg, "fxc_s2s_plan_create: null argument");
+  *out = nullptr;
+  const fxc_s2s_config &c = *cfg;
+  FXC_CHECK_ARG(c.npts >= 1 && c.seg >= 2 && c.seg % 2 == 0 && c.shift >= 1 && c.num_ch >= 1,
+                "fxc_s2s_plan_create: invalid lengths npts=%d seg=%d shift=%d nch=%d", c.npts,
+                c.seg, c.shift, c.num_ch);
+  FXC_CHECK_ARG(c.nfft_filter >= c.seg && c.nfft_filter % 2 == 0 && c.nfft_out >= c.seg &&
+                    c.nfft_out % 2 == 0,
+                "fxc_s2s_plan_create: FFT lengths must be even and >= seg");
+  FXC_CHECK_ARG(c.delta > 0.f && c.freq_low > 0.f, "fxc_s2s_plan_create: invalid delta/freq_low");
+  FXC_CHECK_ARG(c.whiten >= 0 && c.whiten <= 3 && c.normalize >= 0 && c.normalize <= 3,
+                "fxc_s2s_plan_create: whiten must be 0..3 and normalize 0..3");
+  FXC_CHECK_ARG(c.filter_count >= 1 && c.responses && c.freq_lows,
+                "fxc_s2s_plan_create: at least one filter response is required");
+  FXC_CHECK_ARG(c.normalize != 1 || c.filter_count >= 2,
+                "fxc_s2s_plan_create: RUN-ABS-MF needs sub-band filters");
+  const int M = std::max(std::max(c.nfft_filter, c.seg), c.nfft_out) / 2;
+  FXC_CHECK_ARG(2 * (M + 1) + 2 * tables_count(M) <= kLdsComplexCapacity ||
+                    std::max(c.seg / 2, c.nfft_out / 2) + 1 + tables_count(c.seg / 2) +
+                            tables_count(c.nfft_out / 2) <=
+                        kLdsComplexCapacity,
+                "fxc_s2s_plan_create: window too long for the single-workgroup FFT");
+  fxc_s2s_plan *p = new fxc_s2s_plan();
+  p->cfg = c;
+  p->cfg.responses = nullptr;
+  p->cfg.freq_lows = nullptr;
+  p->nspec = c.nfft_out / 2 + 1;
+  p->fwidth = c.seg / 2 + 1;
+  p->fpitch = (p->fwidth + 15) / 16 * 16;
+  p->bpitch = ((c.nfft_filter / 2 + 1) + 15) / 16 * 16;
+  // taper length (size_t)(2.0f*(1.0f/freq_low)/delta), preprocess.cu:91
+  p->taper_size = (int)(2.0f * (1.0f / c.freq_low) / c.delta);
+  // RUN-ABS window 2*int(1.0/(f*dt))+1 with f = freq_low*0.667f (compute.cu:173,
+  // normalization.cu:126)
+  const float fra = c.freq_low * 0.667f;
+  p->runabs_win = std::max(2 * (int)(1.0 / (double)(fra * c.delta)) + 1, 1);
+  for (int i = 0; i < c.filter_count; ++i)
+    p->mf_win.push_back(std::max(2 * (int)(1.0 / (double)(c.freq_lows[i] * c.delta)) + 1, 1));
+  // whitening smoother int(0.02*pitch*delta) (spectrum.cu:205)
+  p->whiten_win = std::max((int)(0.02 * c.seg * (double)c.delta), 1);
+  int rc = FXC_OK;
+  const size_t rbytes = (size_t)c.filter_count * (c.nfft_filter / 2 + 1) * sizeof(float);
+  if ((rc = dmalloc((void **)&p->d_resp, rbytes)) != FXC_OK) {
+    delete p;
+    return rc;
+  }
+  if (hipMemcpy(p->d_resp, c.responses, rbytes, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("fxc_s2s_plan_create: response upload failed");
+    dfree(p->d_resp);
+    delete p;
+    return FXC_ERR_HIP;
+  }
+  if ((rc = make_tables(c.nfft_filter, &p->tf)) != FXC_OK ||
+      (rc = make_tables(c.seg, &p->ts)) != FXC_OK ||
+      (rc = make_tables(c.nfft_out, &p->to)) != FXC_OK) {
+    dfree(p->d_resp);
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return FXC_OK;
+}
+
+extern "C" int fxc_s2s_plan_destroy(fxc_s2s_plan *p) {
+  if (!p) return FXC_OK;
+  for (void *q : {(void *)p->d_resp, (void *)p->d_x, (void *)p->d_w, (void *)p->d_xb,
+                  (void *)p->d_total, (void *)p->d_X, (void *)p->d_base, (void *)p->d_amp,
+                  (void *)p->d_ww, (void *)p->d_steps})
+    dfree(q);
+  delete p;
+  return FXC_OK;
+}
+
+extern "C" int fxc_s2s_plan_info(const fxc_s2s_plan *p, int *nspec, int *taper_size,
+                                 int *runabs_win, int *whiten_win) {
+  FXC_CHECK_ARG(p, "fxc_s2s_plan_info: null plan");
+  if (nspec) *nspec = p->nspec;
+  if (taper_size) *taper_size = p->taper_size;
+  if (runabs_win) *runabs_win = p->runabs_win;
+  if (whiten_win) *whiten_win = p->whiten_win;
+  return FXC_OK;
+}
+
+static int s2s_ensure(fxc_s2s_plan *p, long long rows, long long frames, int nsteps) {
+  const fxc_s2s_config &c = p->cfg;
+  if (rows > p->cap_rows) {
+    for (void *q : {(void *)p->d_x, (void *)p->d_xb, (void *)p->d_total, (void *)p->d_X,
+                    (void *)p->d_base, (void *)p->d_amp})
+      dfree(q);
+    p->d_x = p->d_xb = p->d_total = p->d_amp = nullptr;
+    p->d_X = p->d_base = nullptr;
+    p->cap_rows = 0;
+    FXC_TRY(dmalloc((void **)&p->d_x, (size_t)rows * c.seg * sizeof(float)));
+    if (c.whiten) {
+      FXC_TRY(dmalloc((void **)&p->d_X, (size_t)rows * p->fpitch * sizeof(float2)));
+      FXC_TRY(dmalloc((void **)&p->d_amp, (size_t)rows * p->fpitch * sizeof(float)));
+    }
+    if (c.normalize == 1) {
+      FXC_TRY(dmalloc((void **)&p->d_xb, (size_t)rows * c.seg * sizeof(float)));
+      FXC_TRY(dmalloc((void **)&p->d_total, (size_t)rows * c.seg * sizeof(float)));
+      FXC_TRY(dmalloc((void **)&p->d_base, (size_t)rows * p->bpitch * sizeof(float2)));
+    }
+    p->cap_rows = rows;
+  }
+  if (frames > p->cap_frames) {
+    dfree(p->d_w);
+    dfree(p->d_ww);
+    p->d_w = p->d_ww = nullptr;
+    p->cap_frames = 0;
+    FXC_TRY(dmalloc((void **)&p->d_w, (size_t)frames * c.seg * sizeof(float)));
+    if (c.whiten) FXC_TRY(dmalloc((void **)&p->d_ww, (size_t)frames * p->fpitch * sizeof(float)));
+    p->cap_frames = frames;
+  }
+  if (nsteps > p->cap_steps) {
+    dfree(p->d_steps);
+    p->d_steps = nullptr;
+    FXC_TRY(dmalloc((void **)&p->d_steps, (size_t)nsteps * sizeof(int)));
+    p->cap_steps = nsteps;
+  }
+  return FXC_OK;
+}
+
+static int launch_weight(bool cplx, const void *data, long long pitch, int width, int nch, int win,
+                         long long frames, float *w, long long wpitch, hipStream_t st) {
+  const int per_block = 256 * kWeightChunk;
+  const unsigned gx = (unsigned)ceil_div((size_t)width, (size_t)per_block);
+  const dim3 grid(gx, (unsigned)frames);
+  if (cplx)
+    hipLaunchKernelGGL(s2s_weight_kernel<true>, grid, dim3(256), 0, st, data, pitch, width, nch,
+                       win, w, wpitch);
+  else
+    hipLaunchKernelGGL(s2s_weight_kernel<false>, grid, dim3(256), 0, st, data, pitch, width, nch,
+                       win, w, wpitch);
+  FXC_HIP(hipGetLastError());
+  return FXC_OK;
+}
+
+// Whitening of the rows in d_x (mode/w: pending normalisation).  With
+// `out` set the inverse also produces the output spectrum.
+static int s2s_whiten(fxc_s2s_plan *p, long long rows, long long frames, int mode, const float *w,
+                      float2 *out, hipStream_t st) {
+  const fxc_s2s_config &c = p->cfg;
+  const int Ms = c.seg / 2;
+  size_t lds = lds_bytes(Ms + 1, Ms);
+  FXC_TRY(set_lds(s2s_whiten_fwd_kernel, lds));
+  hipLaunchKernelGGL(s2s_whiten_fwd_kernel, dim3((unsigned)rows), dim3(kS2sThreads), lds, st,
+                     p->d_x, c.seg, c.num_ch, mode, w, p->ts, c.delta, p->d_X, p->d_amp, p->fpitch);
+  FXC_HIP(hipGetLastError());
+  FXC_TRY(launch_weight(true, p->d_X, p->fpitch, p->fwidth, c.num_ch, p->whiten_win, frames,
+                        p->d_ww, p->fpitch, st));
+  const int Mo = c.nfft_out / 2;
+  const int data = std::max(Ms, out ? Mo : Ms) + 1;
+  lds = ((size_t)data + tables_count(Ms) + (out ? tables_count(Mo) : 0)) * sizeof(float2);
+  FXC_CHECK_ARG(lds <= (size_t)kLdsComplexCapacity * sizeof(float2),
+                "whitening + output transform exceed LDS");
+  FXC_TRY(set_lds(s2s_whiten_inv_kernel, lds));
+  const double inv_seg = 1.0 / (double)((float)c.seg * c.delta);
+  hipLaunchKernelGGL(s2s_whiten_inv_kernel, dim3((unsigned)rows), dim3(kS2sThreads), lds, st,
+                     p->d_X, p->fpitch, p->d_ww, c.num_ch, c.seg, c.idx[0], c.idx[1], c.idx[2],
+                     c.idx[3], p->ts, inv_seg, p->d_x, p->to, c.delta, c.phase_only, out,
+                     p->nspec);
+  FXC_HIP(hipGetLastError());
+  return FXC_OK;
+}
+
+// Processes nfiles traces [nfiles][npts] (nfiles = groups * num_ch) for the
+// windows listed in h_steps; writes [nsteps][nfiles][nspec] complex64.
+extern "C" int fxc_s2s_process(fxc_s2s_plan *p, const float *d_traces, int nfiles,
+                               const int *h_steps, int nsteps, float *d_out, void *stream) {
+  FXC_CHECK_ARG(p && d_traces && d_out && h_steps && nfiles >= 1 && nsteps >= 1,
+                "fxc_s2s_process: invalid arguments");
+  const fxc_s2s_config &c = p->cfg;
+  FXC_CHECK_ARG(nfiles % c.num_ch == 0, "fxc_s2s_process: nfiles must be a multiple of num_ch");
+  for (int i = 0; i < nsteps; ++i)
+    FXC_CHECK_ARG(h_steps[i] >= 0, "fxc_s2s_process: negative step index");
+  hipStream_t st = (hipStream_t)stream;
+  const long long rows = (long long)nfiles * nsteps;
+  const long long frames = rows / c.num_ch;
+  FXC_TRY(s2s_ensure(p, rows, frames, nsteps));
+  FXC_HIP(hipMemcpyAsync(p->d_steps, h_steps, nsteps * sizeof(int), hipMemcpyHostToDevice, st));
+  // K1 prefilter
+  const int Mf = c.nfft_filter / 2;
+  size_t lds = lds_bytes(Mf + 1, Mf);
+  FXC_TRY(set_lds(s2s_prefilter_kernel, lds));
+  const double inv_f = 1.0 / (double)((float)c.nfft_filter * c.delta);
+  hipLaunchKernelGGL(s2s_prefilter_kernel, dim3((unsigned)rows), dim3(kS2sThreads), lds, st,
+                     d_traces, c.npts, nfiles, p->d_steps, c.shift, c.seg, p->taper_size, p->tf,
+                     p->d_resp, c.delta, inv_f, p->d_x);
+  FXC_HIP(hipGetLastError());
+  int mode = kLoadPlain;
+  const float *w = nullptr;
+  if (c.whiten == 1 || c.whiten == 3) FXC_TRY(s2s_whiten(p, rows, frames, kLoadPlain, nullptr, nullptr, st));
+  if (c.normalize == 3) {
+    FXC_TRY(launch_weight(false, p->d_x, c.seg, c.seg, c.num_ch, p->runabs_win, frames, p->d_w,
+                          c.seg, st));
+    mode = kLoadRunabs;
+    w = p->d_w;
+  } else if (c.normalize == 2) {
+    mode = kLoadOnebit;
+  } else if (c.normalize == 1) {
+    const int nb = c.nfft_filter / 2 + 1;
+    lds = lds_bytes(Mf + 1, Mf);
+    FXC_TRY(set_lds(s2s_mf_base_kernel, lds));
+    hipLaunchKernelGGL(s2s_mf_base_kernel, dim3((unsigned)rows), dim3(kS2sThreads), lds, st,
+                       p->d_x, c.seg, p->tf, c.delta, p->d_base, p->bpitch);
+    FXC_HIP(hipGetLastError());
+    FXC_HIP(hipMemsetAsync(p->d_total, 0, (size_t)rows * c.seg * sizeof(float), st));
+    FXC_TRY(set_lds(s2s_mf_band_kernel, lds));
+    for (int i = 1; i < c.filter_count; ++i) {
+      hipLaunchKernelGGL(s2s_mf_band_kernel, dim3((unsigned)rows), dim3(kS2sThreads), lds, st,
+                         p->d_base, p->bpitch, p->tf, p->d_resp + (size_t)i * nb, inv_f, c.seg,
+                         p->d_xb);
+      FXC_HIP(hipGetLastError());
+      FXC_TRY(launch_weight(false, p->d_xb, c.seg, c.seg, c.num_ch, p->mf_win[i], frames, p->d_w,
+                            c.seg, st));
+      const size_t n = (size_t)rows * c.seg;
+      hipLaunchKernelGGL(s2s_mf_accum_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st,
+                         p->d_xb, p->d_w, c.seg, c.num_ch, rows, p->d_total);
+      FXC_HIP(hipGetLastError());
+    }
+    FXC_HIP(hipMemcpyAsync(p->d_x, p->d_total, (size_t)rows * c.seg * sizeof(float),
+                           hipMemcpyDeviceToDevice, st));
+  }
+  if (c.whiten == 2 || c.whiten == 3) {
+    FXC_TRY(s2s_whiten(p, rows, frames, mode, w, (float2 *)d_out, st));
+  } else {
+    const int Mo = c.nfft_out / 2;
+    lds = lds_bytes(Mo + 1, Mo);
+    FXC_TRY(set_lds(s2s_output_kernel, lds));
+    hipLaunchKernelGGL(s2s_output_kernel, dim3((unsigned)rows), dim3(kS2sThreads), lds, st,
+                       p->d_x, c.seg, c.num_ch, mode, w, p->to, c.delta, c.phase_only,
+                       (float2 *)d_out, p->nspec);
+    FXC_HIP(hipGetLastError());
+  }
+  return FXC_OK;
+}

@@ -138,6 +138,13 @@ struct TileDesc {
   int rec_base;
 };
 
+// Streaming store of one complex value: the cross-spectra are written once
+// and read back by another kernel, so they must not evict the station
+// spectra that the contraction re-reads from L2.
+__device__ __forceinline__ void store_stream(float2 *dst, float2 v) {
+  __builtin_nontemporal_store(__builtin_bit_cast(unsigned long long, v), (unsigned long long *)dst);
+}
+
 // Workgroup -> (tile, bin chunk).  Workgroups are dispatched round-robin over
 // the 8 XCDs, so 8 consecutive workgroups take 8 consecutive bin chunks of one
 // tile: each XCD's L2 then serves one chunk of every station's spectra, and
@@ -225,7 +232,7 @@ contract_fp32_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int n
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int row = rows[(mi * 8 + a) * kTile + mj * 8 + c];
-      if (row >= 0) xspec[(size_t)row * xpitch + k] = cscale(acc[a][c], scale);
+      if (row >= 0) store_stream(xspec + (size_t)row * xpitch + k, cscale(acc[a][c], scale));
     }
   }
 }
@@ -374,7 +381,7 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
   if (k < nspec) {
     for (int p = tid >> 4; p < kTilePairs; p += blockDim.x >> 4) {
       const int row = rows[p];
-      if (row >= 0) xspec[(size_t)row * xpitch + k] = stage[p * kBinChunk + b];
+      if (row >= 0) store_stream(xspec + (size_t)row * xpitch + k, stage[p * kBinChunk + b]);
     }
   }
 }
@@ -389,7 +396,11 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
 // [nfft/2 - H, nfft/2 + H].  The scale is evaluated as the reference does:
 // weight = 1.0/(double)((float)nfft * dt), product in double, rounded to float.
 // ---------------------------------------------------------------------------
+// 1024 threads with butterflies up to radix 8 measured faster than 512
+// threads with radix-21 butterflies (latency, not ALU, bounds this kernel).
 constexpr int kLagThreads = 1024;
+constexpr int kLagMaxRadix = 8;
+constexpr unsigned kRadixLag = (1u << 2) | (1u << 3) | (1u << 4) | (1u << 5) | (1u << 6) | (1u << 7) | (1u << 8);
 constexpr int kLagRegs = (kLdsComplexCapacity + kLagThreads - 1) / kLagThreads;
 
 __device__ __forceinline__ void lag_prefetch(float2 (&pre)[kLagRegs], const float2 *__restrict__ X,
@@ -442,7 +453,7 @@ lag_kernel(const float2 *__restrict__ xspec, long long xpitch, int nspec, long l
       }
     }
     __syncthreads();
-    lds_fft(lds, 1, plan, twl, 1.f);
+    lds_fft<kRadixLag>(lds, 1, plan, twl, 1.f);
     const int out_row = row_out ? row_out[row] : (int)row;
     float *y = cc + (size_t)out_row * (2 * half_cc + 1);
     for (int m = threadIdx.x; m <= 2 * half_cc; m += blockDim.x) {
@@ -552,7 +563,7 @@ extern "C" int fxc_xc_plan_create(fxc_xc_plan **out, int nspec, int nstep, float
   if (precision != FXC_PREC_FP32)
     FXC_CHECK_ARG(nstep <= 128, "fxc_xc_plan_create: MFMA forms support up to 128 windows");
   const RealFftPlan *rp = nullptr;
-  FXC_TRY(plan_cache_real(nfft, 1, &rp));
+  FXC_TRY(plan_cache_real(nfft, 1, &rp, kLagMaxRadix));
   FXC_CHECK_ARG(real_lds_count(nfft / 2) <= kLdsComplexCapacity,
                 "fxc_xc_plan_create: nfft %d does not fit one workgroup", nfft);
   fxc_xc_plan *p = new fxc_xc_plan();
@@ -838,7 +849,7 @@ static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long
                         const int *row_out, float *d_cc, hipStream_t st) {
   if (nrows == 0) return FXC_OK;
   const RealFftPlan *rp = nullptr;
-  FXC_TRY(plan_cache_real(p->nfft, 1, &rp));
+  FXC_TRY(plan_cache_real(p->nfft, 1, &rp, kLagMaxRadix));
   const size_t lds = (size_t)real_lds_count(p->nfft / 2) * sizeof(float2);
   FXC_HIP(hipFuncSetAttribute((const void *)lag_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));

@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import PRECISIONS, check, lib
+from ._lib import PRECISIONS, S2sConfig, check, lib
 
 
 def _stream_ptr(device) -> ctypes.c_void_p:
@@ -156,4 +156,67 @@ class XcPlan:
             raise ValueError("out must be [npairs, 2*half_cc+1]")
         check(lib().fxc_xc_correlate(self._h, spec.data_ptr(), out.data_ptr(),
                                      _stream_ptr(spec.device)), "fxc_xc_correlate")
+        return out
+
+
+class S2sPlan:
+    """sac2spec per-window pipeline (preprocess, prefilter, normalisation,
+    whitening, output spectrum).
+
+    responses: float32 [filter_count, nfft_filter//2 + 1] power responses
+    (band 0 = broad band); freq_lows: [filter_count]; idx: whitening taper
+    bins on the window grid.
+    """
+
+    def __init__(self, npts, seg, shift, nfft_filter, nfft_out, num_ch, delta, freq_low, idx,
+                 responses, freq_lows, whiten=2, normalize=3, phase_only=False):
+        resp = np.ascontiguousarray(responses, dtype=np.float32)
+        if resp.ndim != 2 or resp.shape[1] != nfft_filter // 2 + 1:
+            raise ValueError("responses must be [filter_count, nfft_filter//2+1]")
+        lows = np.ascontiguousarray(freq_lows, dtype=np.float32)
+        cfg = S2sConfig(npts, seg, shift, nfft_filter, nfft_out, num_ch, float(delta),
+                        float(freq_low), (ctypes.c_int * 4)(*[int(v) for v in idx]), int(whiten),
+                        int(normalize), int(bool(phase_only)), resp.shape[0], resp.ctypes.data,
+                        lows.ctypes.data)
+        self._h = ctypes.c_void_p()
+        check(lib().fxc_s2s_plan_create(ctypes.byref(self._h), ctypes.byref(cfg)),
+              "fxc_s2s_plan_create")
+        self.npts, self.seg, self.num_ch = npts, seg, num_ch
+        self.nspec = nfft_out // 2 + 1
+
+    def info(self):
+        a, b, c, d = (ctypes.c_int() for _ in range(4))
+        check(lib().fxc_s2s_plan_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                      ctypes.byref(d)), "fxc_s2s_plan_info")
+        return {"nspec": a.value, "taper_size": b.value, "runabs_win": c.value,
+                "whiten_win": d.value}
+
+    def close(self):
+        if self._h:
+            lib().fxc_s2s_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def process(self, traces: torch.Tensor, steps, out: torch.Tensor | None = None) -> torch.Tensor:
+        """traces: float32 [nfiles, npts] (device); steps: window indices.
+        Returns complex64 [len(steps), nfiles, nspec]."""
+        _require(traces, torch.float32, "traces")
+        if traces.ndim != 2 or traces.shape[1] != self.npts:
+            raise ValueError("traces must be [nfiles, npts]")
+        nfiles = traces.shape[0]
+        if nfiles % self.num_ch:
+            raise ValueError("nfiles must be a multiple of num_ch")
+        st = np.ascontiguousarray(steps, dtype=np.int32)
+        if out is None:
+            out = torch.empty((len(st), nfiles, self.nspec), dtype=torch.complex64,
+                              device=traces.device)
+        _require(out, torch.complex64, "out")
+        check(lib().fxc_s2s_process(self._h, traces.data_ptr(), nfiles, st.ctypes.data, len(st),
+                                    out.data_ptr(), _stream_ptr(traces.device)),
+              "fxc_s2s_process")
         return out

@@ -86,6 +86,41 @@ int fxc_xc_get_timing(fxc_xc_plan *plan, double *ms3, long long *launches3);
  * native/xc/worker/async_workers.cu:18-86, minus file I/O). */
 int fxc_xc_correlate(fxc_xc_plan *plan, const float *d_spec, float *d_cc, void *stream);
 
+/* ------------------------------------------------------------ sac2spec --
+ * Per station-window preprocessing, normalisation, whitening and output
+ * spectrum.  Replaces ComputeWorkerBatch (native/sac2spec/worker/compute.cu:8-272)
+ * with the plan values main() derives (native/sac2spec/cuda.main.cu:190-296). */
+typedef struct fxc_s2s_config {
+  int npts;         /* samples per trace (llround(sac_len/delta)) */
+  int seg;          /* window length (2,3,5,7-smooth, even) */
+  int shift;        /* window shift in samples */
+  int nfft_filter;  /* prefilter FFT length (seg + filter padding, even smooth) */
+  int nfft_out;     /* output FFT length; nspec = nfft_out/2 + 1 */
+  int num_ch;       /* channels per station group (rows of one frame) */
+  float delta;      /* sampling interval (s) */
+  float freq_low;   /* broad-band low corner (filter 0) */
+  int idx[4];       /* whitening taper bins on the seg grid */
+  int whiten;       /* 0 none, 1 before, 2 after, 3 both normalisation */
+  int normalize;    /* 0 none, 1 RUN-ABS-MF, 2 one-bit, 3 RUN-ABS */
+  int phase_only;   /* 1: phase-only output spectrum */
+  int filter_count; /* bands; band 0 is the broad band */
+  const float *responses; /* host [filter_count][nfft_filter/2+1] |H|^2 */
+  const float *freq_lows; /* host [filter_count] low corners */
+} fxc_s2s_config;
+
+typedef struct fxc_s2s_plan fxc_s2s_plan;
+
+int fxc_s2s_plan_create(fxc_s2s_plan **plan, const fxc_s2s_config *cfg);
+int fxc_s2s_plan_destroy(fxc_s2s_plan *plan);
+/* Derived sizes: nspec, time taper length, RUN-ABS and whitening windows. */
+int fxc_s2s_plan_info(const fxc_s2s_plan *plan, int *nspec, int *taper_size, int *runabs_win,
+                      int *whiten_win);
+/* d_traces: [nfiles][npts] float (nfiles = groups*num_ch, channels of a group
+ * adjacent); h_steps: window indices to process (skip list applied);
+ * d_out: [nsteps][nfiles][nspec] complex64, the StepPack payload order. */
+int fxc_s2s_process(fxc_s2s_plan *plan, const float *d_traces, int nfiles, const int *h_steps,
+                    int nsteps, float *d_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

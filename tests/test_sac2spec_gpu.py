@@ -1,0 +1,86 @@
+"""sac2spec stage on the GPU vs the FP64 oracle (oracle/sac2spec.py).
+
+Small configuration with the reference's own length rules: dt 0.1 s,
+24-minute records, 3-minute windows (seg 1800), max_lag 20 s
+(nfft_out 2000), butter(2) bands; 2 stations x 3 channels so frames share
+weights across channels.  Tolerance: relative L2 <= 1e-5 (FP32 vs FP64) on
+the output spectra, per row as well as overall.
+"""
+import numpy as np
+import pytest
+
+from oracle import sac2spec as s2s
+from oracle.xc import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(tmp_path, bands):
+    path = str(tmp_path / "filter.txt")
+    s2s.write_filter_file(path, 0.1, bands)
+    f = s2s.read_filters(path)
+    plan = s2s.plan_lengths(1440, 0.1, 180, 180, 20, f)
+    return plan
+
+
+def _gpu_plan(plan, nch, whiten, normalize, phase_only):
+    from fastxc_amd.ops import S2sPlan
+
+    resp = np.stack([r[: plan.nfft_filter // 2 + 1] for r in plan.responses])
+    return S2sPlan(plan.npts, plan.seg, plan.shift, plan.nfft_filter, plan.nfft_out, nch,
+                   plan.delta, plan.freq_low, plan.idx, resp, plan.freq_lows, whiten, normalize,
+                   phase_only)
+
+
+def _traces(rng, nfiles, npts):
+    t = rng.standard_normal((nfiles, npts)).astype(np.float32)
+    t += np.float32(0.3) * np.sin(np.arange(npts) * 0.05).astype(np.float32)  # shared component
+    return t
+
+
+@pytest.mark.parametrize("whiten,normalize,phase_only,bands", [
+    (2, 3, 0, "0.1/2"),          # defaults: RUN-ABS, whiten after
+    (0, 0, 0, "0.1/2"),          # prefilter only
+    (1, 2, 0, "0.1/2"),          # whiten before, one-bit
+    (3, 1, 1, "0.1/0.5 0.5/2"),  # RUN-ABS-MF, whiten both, phase-only output
+    (2, 1, 0, "0.1/0.5 0.5/2"),  # RUN-ABS-MF, whiten after
+])
+def test_process_matches_oracle(gpu, tmp_path, whiten, normalize, phase_only, bands):
+    import torch
+
+    plan = _setup(tmp_path, bands)
+    nch, nsta = 3, 2
+    rng = np.random.default_rng(whiten * 10 + normalize)
+    tr = _traces(rng, nch * nsta, plan.npts)
+    steps = [0, 2, 3, 7]
+    gp = _gpu_plan(plan, nch, whiten, normalize, phase_only)
+    info = gp.info()
+    assert info["nspec"] == plan.nspec
+    out = gp.process(torch.from_numpy(tr).to(gpu), steps).cpu().numpy()
+    win = s2s.cut_windows(tr.astype(np.float64), plan, steps)  # [nsteps, files, seg]
+    frames = win.reshape(len(steps) * nsta, nch, plan.seg)
+    want = s2s.process_batch(frames, plan, whiten, normalize, bool(phase_only))
+    want = want.reshape(len(steps), nch * nsta, plan.nspec)
+    err = rel_l2(out, want)
+    print(f"whiten={whiten} normalize={normalize} phase_only={phase_only} rel_l2={err:.3e}")
+    assert err < 1e-5
+    rows = np.linalg.norm((out - want).reshape(-1, plan.nspec), axis=1) / \
+        np.linalg.norm(want.reshape(-1, plan.nspec), axis=1)
+    assert np.all(rows < 1e-4)
+
+
+def test_production_lengths_one_station(gpu, tmp_path):
+    """seg 36000, nfft_filter 36288, nfft_out 37044 at dt = 0.1 (3 windows)."""
+    import torch
+
+    path = str(tmp_path / "filter.txt")
+    s2s.write_filter_file(path, 0.1, "0.1/2")
+    plan = s2s.plan_lengths(3 * 3600, 0.1, 3600, 3600, 100, s2s.read_filters(path))
+    assert (plan.seg, plan.nfft_filter, plan.nfft_out) == (36000, 36288, 37044)
+    rng = np.random.default_rng(9)
+    tr = _traces(rng, 1, plan.npts)
+    gp = _gpu_plan(plan, 1, 2, 3, 0)
+    out = gp.process(torch.from_numpy(tr).to(gpu), [0, 1, 2]).cpu().numpy()
+    win = s2s.cut_windows(tr.astype(np.float64), plan, [0, 1, 2])
+    want = s2s.process_batch(win.reshape(3, 1, plan.seg), plan, 2, 3, False).reshape(3, 1, -1)
+    assert rel_l2(out, want) < 1e-5
