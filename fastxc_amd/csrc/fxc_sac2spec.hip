@@ -564,12 +564,19 @@ extern "C" int fxc_s2s_plan_create(fxc_s2s_plan **out, const fxc_s2s_config *cfg
                 "fxc_s2s_plan_create: at least one filter response is required");
   FXC_CHECK_ARG(c.normalize != 1 || c.filter_count >= 2,
                 "fxc_s2s_plan_create: RUN-ABS-MF needs sub-band filters");
-  const int M = std::max(std::max(c.nfft_filter, c.seg), c.nfft_out) / 2;
-  FXC_CHECK_ARG(2 * (M + 1) + 2 * tables_count(M) <= kLdsComplexCapacity ||
-                    std::max(c.seg / 2, c.nfft_out / 2) + 1 + tables_count(c.seg / 2) +
-                            tables_count(c.nfft_out / 2) <=
-                        kLdsComplexCapacity,
-                "fxc_s2s_plan_create: window too long for the single-workgroup FFT");
+  {
+    // exact LDS needs of the per-row kernels (data slots + staged tables)
+    const int Mf = c.nfft_filter / 2, Ms = c.seg / 2, Mo = c.nfft_out / 2;
+    const int need_f = Mf + 1 + tables_count(Mf);
+    const int need_s = Ms + 1 + tables_count(Ms);
+    const int need_o = Mo + 1 + tables_count(Mo);
+    const int need_so = std::max(Ms, Mo) + 1 + tables_count(Ms) + tables_count(Mo);
+    const int need = std::max(std::max(need_f, need_s), std::max(need_o, need_so));
+    FXC_CHECK_ARG(need <= kLdsComplexCapacity,
+                  "fxc_s2s_plan_create: transforms (seg %d, filter %d, out %d) exceed the "
+                  "single-workgroup LDS FFT (%d > %d complex)", c.seg, c.nfft_filter, c.nfft_out,
+                  need, kLdsComplexCapacity);
+  }
   fxc_s2s_plan *p = new fxc_s2s_plan();
   p->cfg = c;
   p->cfg.responses = nullptr;

@@ -1,0 +1,377 @@
+// Phase-weighted stacks: ncf_pws and ncf_tfpws hot paths.
+//
+// PWS replaces compute_pws_host_batch (native/pws/worker/gpu.cu:87-282):
+// R2C + Hilbert mask -> C2C inverse -> a/N / max(|a/N|, 1e-7) -> W = sum_g
+// w_g phi_g -> pws = lin * |W| (kernels/cuda.pws.cu:3-95, power 1).
+// TF-PWS replaces compute_tfpws_from_prestack
+// (native/tfpws/core/tfpws_compute.cu:19-366): for every voice f of the
+// Stockwell transform the Gaussian-modulated, shifted analytic spectra of
+// the linear stack and of every group are inverse transformed, the groups'
+// unit phasors weight the linear stack's voice (|W|^1, the weight carrying
+// 1/N), the voice is summed over time, optionally blended with the linear
+// spectrum in a band, and the voices are inverse transformed
+// (kernels/cuda.stransform.cu:59-332).
+//
+// MI355X design: the correlation length (2001 = 3*23*29) is transformed in
+// LDS by 256-thread workgroups with register butterflies up to radix 29.
+// The analytic spectra of all groups are computed once; the TF-PWS voice
+// kernel keeps the modulated transforms, the weight W_f(tau) and the
+// Gaussian in LDS, so per voice only the (group-count + 1) spectra are read
+// (from L2) and one complex number is written: the S-transform is never
+// materialised in HBM.  W is accumulated in the FFT's digit-reversed order,
+// which is a consistent permutation of tau, so no reordering pass exists.
+#include "fxc_plan_cache.hpp"
+#include "fastxc_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace fxc {
+
+constexpr int kStkThreads = 256;
+constexpr int kStkBatch = 4;       // transforms held in LDS at once
+constexpr int kVoiceChunk = 32;    // TF-PWS voices per workgroup
+
+__device__ __forceinline__ void stage_tw(float2 *dst, const FftPlanDev &plan,
+                                         const float2 *__restrict__ tw) {
+  for (int i = threadIdx.x; i < kTwFine + plan.ncoarse; i += blockDim.x) dst[i] = tw[i];
+}
+
+// Analytic spectra of real rows: A[k] = 2 X[k] for 1 <= k < (n+1)/2, else 0
+// (hilbertTransformKernel after R2C).  Output natural order, [rows][n].
+__global__ void __launch_bounds__(kStkThreads)
+analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, FftPlanDev plan,
+                const float2 *__restrict__ tw, float2 *__restrict__ A) {
+  extern __shared__ float2 lds[];
+  const int n = plan.n;
+  float2 *twl = lds + (size_t)per_wg * n;
+  stage_tw(twl, plan, tw);
+  const long long r0 = (long long)blockIdx.x * per_wg;
+  const int nt = (int)std::min<long long>(per_wg, rows - r0);
+  for (int i = threadIdx.x; i < nt * n; i += blockDim.x) {
+    const int t = i / n, j = i - t * n;
+    lds[i] = make_float2(x[(r0 + t) * n + j], 0.f);
+  }
+  __syncthreads();
+  lds_fft<kRadixAll>(lds, nt, plan, twl, -1.f);
+  const int half = (n + 1) / 2;
+  for (int i = threadIdx.x; i < nt * n; i += blockDim.x) {
+    const int t = i / n, k = i - t * n;
+    float2 v = make_float2(0.f, 0.f);
+    if (k >= 1 && k < half) {
+      const float2 z = lds[t * n + fft_pos(plan, k)];
+      v = make_float2(2.f * z.x, 2.f * z.y);
+    }
+    A[(r0 + t) * n + k] = v;
+  }
+}
+
+// PWS: one workgroup per pair.
+__global__ void __launch_bounds__(kStkThreads)
+pws_kernel(const float2 *__restrict__ A, const float *__restrict__ weights,
+           const long long *__restrict__ offsets, const float *__restrict__ lin,
+           FftPlanDev plan, const float2 *__restrict__ tw_inv, float *__restrict__ out) {
+  extern __shared__ float2 lds[];
+  const int n = plan.n;
+  float2 *W = lds;                       // [n], digit-reversed positions
+  float2 *buf = lds + n;                 // [kStkBatch][n]
+  float2 *twl = buf + (size_t)kStkBatch * n;
+  stage_tw(twl, plan, tw_inv);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) W[i] = make_float2(0.f, 0.f);
+  const int pair = blockIdx.x;
+  const long long g0 = offsets[pair], g1 = offsets[pair + 1];
+  const float invn = 1.f / (float)n;
+  for (long long gb = g0; gb < g1; gb += kStkBatch) {
+    const int nb = (int)std::min<long long>(kStkBatch, g1 - gb);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb * n; i += blockDim.x) buf[i] = A[gb * n + i];
+    __syncthreads();
+    lds_fft<kRadixAll>(buf, nb, plan, twl, 1.f);
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+      float2 acc = W[p];
+      for (int b = 0; b < nb; ++b) {
+        const float2 z = buf[b * n + p];
+        const float re = z.x * invn, im = z.y * invn;
+        float mod = sqrtf(re * re + im * im);
+        mod = mod > 1e-7f ? mod : 1e-7f;
+        const float wg = weights[gb + b];
+        acc.x += (re / mod) * wg;
+        acc.y += (im / mod) * wg;
+      }
+      W[p] = acc;
+    }
+  }
+  __syncthreads();
+  const float *l = lin + (size_t)pair * n;
+  float *o = out + (size_t)pair * n;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const float2 w = W[fft_pos(plan, t)];
+    o[t] = l[t] * sqrtf(w.x * w.x + w.y * w.y);
+  }
+}
+
+// TF-PWS voices [f0, f0 + kVoiceChunk) of one pair: out_spec[pair][f] =
+// sum_tau ST_lin,f(tau) |W_f(tau)|.
+__global__ void __launch_bounds__(kStkThreads)
+tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Alin,
+                   const float *__restrict__ weights, const long long *__restrict__ offsets,
+                   FftPlanDev plan, const float2 *__restrict__ tw_inv, int nfreq,
+                   const unsigned char *__restrict__ chunk_active, float2 *__restrict__ out_spec) {
+  extern __shared__ float2 lds[];
+  const int n = plan.n;
+  float2 *W = lds;                                   // [n]
+  float *gauss = (float *)(lds + n);                 // [n] floats (n/2 complex slots)
+  float2 *buf = lds + n + (n + 1) / 2;               // [kStkBatch][n]
+  float2 *twl = buf + (size_t)kStkBatch * n;
+  __shared__ float red_x[kStkThreads / 64], red_y[kStkThreads / 64];
+  stage_tw(twl, plan, tw_inv);
+  const int pair = blockIdx.y;
+  const int chunk = blockIdx.x;
+  if (chunk_active && !chunk_active[chunk]) return;  // uniform per workgroup
+  const int f0 = chunk * kVoiceChunk;
+  const long long g0 = offsets[pair], g1 = offsets[pair + 1];
+  const float2 *Al = Alin + (size_t)pair * n;
+  const int half = (n + 1) / 2;
+  const float pi2 = -2.0f * (float)M_PI * (float)M_PI;
+  const float nf = (float)n;
+  for (int f = f0; f < std::min(nfreq, f0 + kVoiceChunk); ++f) {
+    __syncthreads();
+    // Gaussian window of voice f (gaussianModulateSub, scale 0.1)
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+      float g;
+      if (f == 0) {
+        g = 1.f;
+      } else if (c == 0) {
+        g = 0.f;
+      } else {
+        const int cm = c < half ? c : n - c;
+        g = expf(pi2 * (float)cm * (float)cm * 0.1f / ((float)f * (float)f));
+      }
+      gauss[c] = g;
+      W[c] = make_float2(0.f, 0.f);
+    }
+    for (long long gb = g0; gb < g1; gb += kStkBatch) {
+      const int nb = (int)std::min<long long>(kStkBatch, g1 - gb);
+      __syncthreads();
+      for (int i = threadIdx.x; i < nb * n; i += blockDim.x) {
+        const int b = i / n, c = i - b * n;
+        int src = f + c;
+        src = src >= n ? src - n : src;
+        buf[i] = cscale(Ag[(gb + b) * n + src], gauss[c]);
+      }
+      __syncthreads();
+      lds_fft<kRadixAll>(buf, nb, plan, twl, 1.f);
+      for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        float2 acc = W[p];
+        for (int b = 0; b < nb; ++b) {
+          const float2 z = buf[b * n + p];
+          float fa = sqrtf(z.x * z.x + z.y * z.y);
+          if (fa <= 1e-7f) fa = 1.f;
+          const float wg = weights[gb + b];
+          acc.x += wg * z.x / (fa * nf);
+          acc.y += wg * z.y / (fa * nf);
+        }
+        W[p] = acc;
+      }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+      int src = f + c;
+      src = src >= n ? src - n : src;
+      buf[c] = cscale(Al[src], gauss[c]);
+    }
+    __syncthreads();
+    lds_fft<kRadixAll>(buf, 1, plan, twl, 1.f);
+    float sx = 0.f, sy = 0.f;
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+      const float2 w = W[p];
+      const float mag = sqrtf(w.x * w.x + w.y * w.y);
+      sx += buf[p].x * mag;
+      sy += buf[p].y * mag;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      sx += __shfl_down(sx, off);
+      sy += __shfl_down(sy, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      red_x[threadIdx.x >> 6] = sx;
+      red_y[threadIdx.x >> 6] = sy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float tx = 0.f, ty = 0.f;
+      for (int i = 0; i < kStkThreads / 64; ++i) {
+        tx += red_x[i];
+        ty += red_y[i];
+      }
+      out_spec[(size_t)pair * n + f] = make_float2(tx, ty);
+    }
+  }
+}
+
+// Final TF-PWS step per pair: voices (blended with the linear analytic
+// spectrum in band mode), zero above nfreq, inverse transform, Re / n.
+__global__ void __launch_bounds__(kStkThreads)
+tfpws_final_kernel(const float2 *__restrict__ out_spec, const float2 *__restrict__ Alin, int nfreq,
+                   int band_limited, float df, float fmin, float fmax, float taper,
+                   FftPlanDev plan, const float2 *__restrict__ tw_inv, float *__restrict__ y) {
+  extern __shared__ float2 lds[];
+  const int n = plan.n;
+  float2 *twl = lds + n;
+  stage_tw(twl, plan, tw_inv);
+  const int pair = blockIdx.x;
+  const float2 *os = out_spec + (size_t)pair * n;
+  const float2 *Al = Alin + (size_t)pair * n;
+  for (int f = threadIdx.x; f < n; f += blockDim.x) {
+    float2 v = make_float2(0.f, 0.f);
+    if (band_limited) {
+      v = Al[f];
+      if (f < nfreq) {
+        const float freq = f * df;
+        float alpha = 0.f;
+        if (freq >= fmin && freq <= fmax) {
+          const float t = fminf(taper, 0.5f * (fmax - fmin));
+          if (t <= 0.f)
+            alpha = 1.f;
+          else if (freq < fmin + t)
+            alpha = 0.5f - 0.5f * cosf((float)M_PI * ((freq - fmin) / t));
+          else if (freq > fmax - t)
+            alpha = 0.5f - 0.5f * cosf((float)M_PI * ((fmax - freq) / t));
+          else
+            alpha = 1.f;
+        }
+        v = make_float2(Al[f].x * (1.f - alpha) + os[f].x * alpha,
+                        Al[f].y * (1.f - alpha) + os[f].y * alpha);
+      }
+    } else if (f < nfreq) {
+      v = os[f];
+    }
+    lds[f] = v;
+  }
+  __syncthreads();
+  lds_fft<kRadixAll>(lds, 1, plan, twl, 1.f);
+  float *o = y + (size_t)pair * n;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) o[t] = lds[fft_pos(plan, t)].x / (float)n;
+}
+
+struct StackWork {
+  float2 *A = nullptr;      // [groups + pairs][n]
+  long long *offsets = nullptr;
+  float2 *spec = nullptr;   // [pairs][n]
+  unsigned char *active = nullptr;
+  ~StackWork() {
+    dfree(A);
+    dfree(offsets);
+    dfree(spec);
+    dfree(active);
+  }
+};
+
+static int stack_common(const float *d_pre, const float *d_lin, const long long *h_off, int npairs,
+                        int n, StackWork &w, const FftPlan **pinv, hipStream_t st) {
+  const FftPlan *pf = nullptr;
+  FXC_TRY(plan_cache_complex(n, -1, &pf));
+  FXC_TRY(plan_cache_complex(n, 1, pinv));
+  const long long ng = h_off[npairs];
+  const long long rows = ng + npairs;
+  FXC_TRY(dmalloc((void **)&w.A, (size_t)rows * n * sizeof(float2)));
+  FXC_TRY(dmalloc((void **)&w.offsets, (size_t)(npairs + 1) * sizeof(long long)));
+  FXC_HIP(hipMemcpyAsync(w.offsets, h_off, (size_t)(npairs + 1) * sizeof(long long),
+                         hipMemcpyHostToDevice, st));
+  const int per = std::max(1, std::min(8, (kLdsComplexCapacity - tw_lds_count(n)) / n));
+  const size_t lds = ((size_t)per * n + tw_lds_count(n)) * sizeof(float2);
+  FXC_HIP(hipFuncSetAttribute((const void *)analytic_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (ng > 0) {
+    hipLaunchKernelGGL(analytic_kernel, dim3((unsigned)ceil_div((size_t)ng, (size_t)per)),
+                       dim3(kStkThreads), lds, st, d_pre, ng, per, pf->dev, pf->d_tw, w.A);
+    FXC_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(analytic_kernel, dim3((unsigned)ceil_div((size_t)npairs, (size_t)per)),
+                     dim3(kStkThreads), lds, st, d_lin, (long long)npairs, per, pf->dev, pf->d_tw,
+                     w.A + (size_t)ng * n);
+  FXC_HIP(hipGetLastError());
+  return FXC_OK;
+}
+
+static int check_stack_args(const float *pre, const float *wts, const long long *off,
+                            const float *lin, int npairs, int n, const float *out) {
+  FXC_CHECK_ARG(pre && wts && off && lin && out && npairs >= 1 && n >= 2,
+                "stack: invalid arguments");
+  FXC_CHECK_ARG(off[0] == 0, "stack: group offsets must start at 0");
+  for (int i = 0; i < npairs; ++i)
+    FXC_CHECK_ARG(off[i + 1] > off[i], "stack: pair %d has no groups", i);
+  int radix[kMaxPasses], np = 0;
+  FXC_CHECK_ARG(factor_radices(n, radix, &np), "stack: length %d has no supported factorisation", n);
+  FXC_CHECK_ARG((size_t)n * (1 + kStkBatch) + (n + 1) / 2 + tw_lds_count(n) <=
+                    (size_t)kLdsComplexCapacity,
+                "stack: length %d too long for the LDS transforms", n);
+  return FXC_OK;
+}
+
+}  // namespace fxc
+
+using namespace fxc;
+
+extern "C" int fxc_pws(const float *d_prestack, const float *d_weights, const long long *h_offsets,
+                       const float *d_linear, int npairs, int n, float *d_out, void *stream) {
+  FXC_TRY(check_stack_args(d_prestack, d_weights, h_offsets, d_linear, npairs, n, d_out));
+  hipStream_t st = (hipStream_t)stream;
+  StackWork w;
+  const FftPlan *pinv = nullptr;
+  FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, &pinv, st));
+  const size_t lds = ((size_t)n * (1 + kStkBatch) + tw_lds_count(n)) * sizeof(float2);
+  FXC_HIP(hipFuncSetAttribute((const void *)pws_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds));
+  hipLaunchKernelGGL(pws_kernel, dim3((unsigned)npairs), dim3(kStkThreads), lds, st, w.A, d_weights,
+                     w.offsets, d_linear, pinv->dev, pinv->d_tw, d_out);
+  FXC_HIP(hipGetLastError());
+  FXC_HIP(hipStreamSynchronize(st));  // workspace is released on return
+  return FXC_OK;
+}
+
+extern "C" int fxc_tfpws(const float *d_prestack, const float *d_weights,
+                         const long long *h_offsets, const float *d_linear, int npairs, int n,
+                         int band_limited, float delta, float fmin, float fmax, float taper_hz,
+                         float *d_out, void *stream) {
+  FXC_TRY(check_stack_args(d_prestack, d_weights, h_offsets, d_linear, npairs, n, d_out));
+  FXC_CHECK_ARG(!band_limited || (delta > 0.f && fmax > fmin), "fxc_tfpws: invalid band");
+  hipStream_t st = (hipStream_t)stream;
+  StackWork w;
+  const FftPlan *pinv = nullptr;
+  FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, &pinv, st));
+  const int nfreq = n / 2 + 1;
+  const int nchunks = (nfreq + kVoiceChunk - 1) / kVoiceChunk;
+  const float df = band_limited ? (float)(1.0 / ((double)n * delta)) : 0.f;
+  FXC_TRY(dmalloc((void **)&w.spec, (size_t)npairs * n * sizeof(float2)));
+  FXC_HIP(hipMemsetAsync(w.spec, 0, (size_t)npairs * n * sizeof(float2), st));
+  if (band_limited) {
+    // chunks with no voice inside [fmin, fmax] only carry the linear spectrum
+    std::vector<unsigned char> act(nchunks, 0);
+    for (int c = 0; c < nchunks; ++c) {
+      const int a = c * kVoiceChunk, b = std::min(nfreq, a + kVoiceChunk) - 1;
+      const double lo = a * (1.0 / ((double)n * delta)), hi = b * (1.0 / ((double)n * delta));
+      act[c] = !(hi < fmin || lo > fmax);
+    }
+    FXC_TRY(dmalloc((void **)&w.active, nchunks));
+    FXC_HIP(hipMemcpyAsync(w.active, act.data(), nchunks, hipMemcpyHostToDevice, st));
+  }
+  const long long ng = h_offsets[npairs];
+  size_t lds = ((size_t)n * (1 + kStkBatch) + (n + 1) / 2 + tw_lds_count(n)) * sizeof(float2);
+  FXC_HIP(hipFuncSetAttribute((const void *)tfpws_voice_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(tfpws_voice_kernel, dim3((unsigned)nchunks, (unsigned)npairs),
+                     dim3(kStkThreads), lds, st, w.A, w.A + (size_t)ng * n, d_weights, w.offsets,
+                     pinv->dev, pinv->d_tw, nfreq, (const unsigned char *)w.active, w.spec);
+  FXC_HIP(hipGetLastError());
+  lds = ((size_t)n + tw_lds_count(n)) * sizeof(float2);
+  FXC_HIP(hipFuncSetAttribute((const void *)tfpws_final_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(tfpws_final_kernel, dim3((unsigned)npairs), dim3(kStkThreads), lds, st, w.spec,
+                     w.A + (size_t)ng * n, nfreq, band_limited, df, fmin, fmax, taper_hz, pinv->dev,
+                     pinv->d_tw, d_out);
+  FXC_HIP(hipGetLastError());
+  FXC_HIP(hipStreamSynchronize(st));  // workspace is released on return
+  return FXC_OK;
+}

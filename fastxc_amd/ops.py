@@ -220,3 +220,41 @@ class S2sPlan:
                                     out.data_ptr(), _stream_ptr(traces.device)),
               "fxc_s2s_process")
         return out
+
+
+def _stack_args(prestack, weights, offsets, linear, out):
+    _require(prestack, torch.float32, "prestack")
+    _require(weights, torch.float32, "weights")
+    _require(linear, torch.float32, "linear")
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    npairs, n = linear.shape
+    if prestack.ndim != 2 or prestack.shape[1] != n or prestack.shape[0] != off[-1]:
+        raise ValueError("prestack must be [offsets[-1], n]")
+    if weights.shape != (prestack.shape[0],) or off.shape != (npairs + 1,):
+        raise ValueError("weights/offsets shape mismatch")
+    if out is None:
+        out = torch.empty_like(linear)
+    _require(out, torch.float32, "out")
+    return off, npairs, n, out
+
+
+def pws(prestack, weights, offsets, linear, out=None):
+    """Phase-weighted stack (power 1) of every pair; prestack rows are group means."""
+    off, npairs, n, out = _stack_args(prestack, weights, offsets, linear, out)
+    check(lib().fxc_pws(prestack.data_ptr(), weights.data_ptr(), off.ctypes.data,
+                        linear.data_ptr(), npairs, n, out.data_ptr(),
+                        _stream_ptr(linear.device)), "fxc_pws")
+    return out
+
+
+def tfpws(prestack, weights, offsets, linear, band=None, delta=1.0, out=None):
+    """Time-frequency PWS (S-transform, scale 0.1, power 1); prestack rows are group sums.
+    band: None or (fmin, fmax, taper_hz)."""
+    off, npairs, n, out = _stack_args(prestack, weights, offsets, linear, out)
+    bl = band is not None
+    fmin, fmax, taper = band if bl else (0.0, 0.0, 0.0)
+    check(lib().fxc_tfpws(prestack.data_ptr(), weights.data_ptr(), off.ctypes.data,
+                          linear.data_ptr(), npairs, n, int(bl), float(delta), float(fmin),
+                          float(fmax), float(taper), out.data_ptr(), _stream_ptr(linear.device)),
+          "fxc_tfpws")
+    return out

@@ -121,6 +121,21 @@ int fxc_s2s_plan_info(const fxc_s2s_plan *plan, int *nspec, int *taper_size, int
 int fxc_s2s_process(fxc_s2s_plan *plan, const float *d_traces, int nfiles, const int *h_steps,
                     int nsteps, float *d_out, void *stream);
 
+/* --------------------------------------------------------------- stacks --
+ * Phase-weighted stacks of cross-correlations of length n (2H+1).  Groups of
+ * pair p are rows h_offsets[p] .. h_offsets[p+1]-1 of d_prestack [groups][n];
+ * d_weights [groups] = n_g/T; d_linear [npairs][n]; d_out [npairs][n].
+ * fxc_pws: prestack rows are group MEANS; out = lin * |sum_g w_g phi_g|
+ * (compute_pws_host_batch, native/pws/worker/gpu.cu:87-282).
+ * fxc_tfpws: prestack rows are group SUMS; Stockwell-domain PWS with
+ * Gaussian scale 0.1, power 1 and the optional band blend
+ * (compute_tfpws_from_prestack, native/tfpws/core/tfpws_compute.cu:19-366). */
+int fxc_pws(const float *d_prestack, const float *d_weights, const long long *h_offsets,
+            const float *d_linear, int npairs, int n, float *d_out, void *stream);
+int fxc_tfpws(const float *d_prestack, const float *d_weights, const long long *h_offsets,
+              const float *d_linear, int npairs, int n, int band_limited, float delta, float fmin,
+              float fmax, float taper_hz, float *d_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

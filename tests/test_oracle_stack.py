@@ -1,0 +1,70 @@
+"""Pins the stack oracle (CPU only) with closed forms and brute force."""
+import numpy as np
+
+from oracle import stack
+
+
+def test_linear_stack_is_float64_mean():
+    r = np.random.default_rng(0).standard_normal((7, 33)).astype(np.float32)
+    assert np.array_equal(stack.linear_stack(r), r.astype(np.float64).mean(0).astype(np.float32))
+
+
+def test_analytic_signal_of_cosine():
+    N, k = 101, 7
+    t = np.arange(N)
+    x = np.cos(2 * np.pi * k * t / N)
+    a = np.fft.ifft(stack.analytic_spectrum(x))
+    assert np.allclose(a, np.exp(2j * np.pi * k * t / N), atol=1e-12)
+
+
+def test_pws_identical_traces_equal_linear():
+    rng = np.random.default_rng(1)
+    tr = rng.standard_normal(61)
+    recs = np.tile(tr, (9, 1))
+    g, w, lin = stack.pws_prepare(recs, 4)
+    assert np.allclose(w, [4 / 9, 4 / 9, 1 / 9])
+    assert np.allclose(stack.pws(g, w, lin), lin)
+
+
+def test_pws_group_means_and_weights():
+    rng = np.random.default_rng(2)
+    recs = rng.standard_normal((5, 20))
+    g, w, lin = stack.pws_prepare(recs, 2)
+    assert np.allclose(g[0], recs[:2].mean(0)) and np.allclose(g[2], recs[4])
+    assert np.allclose(lin, recs.mean(0))
+    s, w2, lin2 = stack.tfpws_prepare(recs, 2)
+    assert np.allclose(s[1], recs[2:4].sum(0)) and np.allclose(lin2, recs.mean(0))
+    assert np.allclose(w, w2)
+
+
+def test_s_transform_matches_bruteforce():
+    rng = np.random.default_rng(3)
+    N = 15
+    A = stack.analytic_spectrum(rng.standard_normal(N))
+    for f in (0, 1, 4, 7):
+        got = stack.s_transform_voice(A[None], f)[0]
+        want = np.zeros(N, complex)
+        for tau in range(N):
+            for c in range(N):
+                cm = c if c < (N + 1) // 2 else N - c
+                if f == 0:
+                    gc = 1.0
+                elif c == 0:
+                    gc = 0.0
+                else:
+                    gc = np.exp(-2 * np.pi ** 2 * cm * cm * 0.1 / f ** 2)
+                want[tau] += A[(f + c) % N] * gc * np.exp(2j * np.pi * c * tau / N)
+        assert np.allclose(got, want, atol=1e-10)
+
+
+def test_tfpws_identical_groups_vanish():
+    # identical groups give |W| = 1/N at every tau; with G(0, f != 0) = 0 each
+    # voice's time sum is N * X_f[0] = 0, and the DC voice carries A[0] = 0.
+    rng = np.random.default_rng(4)
+    tr = rng.standard_normal(41)
+    s, w, lin = stack.tfpws_prepare(np.tile(tr, (6, 1)), 2)
+    out = stack.tfpws(s / 2, w, lin)  # scaled sums equal the linear stack
+    assert np.max(np.abs(out)) < 1e-9 * np.max(np.abs(lin))
+    # the band-limited blend keeps the analytic linear stack outside the band
+    outb = stack.tfpws(s / 2, w, lin, band=(0.2, 0.3, 0.01), delta=1.0)
+    assert np.max(np.abs(outb)) > 0.1 * np.max(np.abs(lin))

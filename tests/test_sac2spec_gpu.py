@@ -61,6 +61,20 @@ def test_process_matches_oracle(gpu, tmp_path, whiten, normalize, phase_only, ba
     frames = win.reshape(len(steps) * nsta, nch, plan.seg)
     want = s2s.process_batch(frames, plan, whiten, normalize, bool(phase_only))
     want = want.reshape(len(steps), nch * nsta, plan.nspec)
+    if phase_only:
+        # Phase-only output maps every bin to unit magnitude, so bins at the
+        # FP32 noise floor (out of band) carry O(1) phase differences against
+        # FP64.  Compare bins whose FP64 amplitude is above 1e-3 of the row
+        # maximum; every other bin must still be unit or zero magnitude.
+        amp = np.abs(s2s.process_batch(frames, plan, whiten, normalize, False)).reshape(want.shape)
+        keep = amp > 1e-3 * amp.max(axis=-1, keepdims=True)
+        mag = np.abs(out[~keep])
+        assert np.all((np.abs(mag - 1) < 1e-3) | (mag == 0))
+        out, want = out[keep], want[keep]
+        err = rel_l2(out, want)
+        print(f"phase-only compared bins: {keep.mean():.3f} rel_l2={err:.3e}")
+        assert err < 1e-4
+        return
     err = rel_l2(out, want)
     print(f"whiten={whiten} normalize={normalize} phase_only={phase_only} rel_l2={err:.3e}")
     assert err < 1e-5

@@ -1,0 +1,103 @@
+"""PWS and TF-PWS on the GPU vs the FP64 oracle (oracle/stack.py).
+
+Inputs follow the reference host preparation in float32
+(native/pws/io/sourcepack_io.cu:385-478, native/tfpws/io/sourcepack_io.cu:403-490);
+the oracle prepares the same records in FP64.  Tolerance: relative L2
+1e-5 for PWS, 1e-4 for TF-PWS (a sum over n/2+1 voices of FP32 inverse
+transforms).
+"""
+import numpy as np
+import pytest
+
+from oracle import stack
+from oracle.xc import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _records(rng, T, N):
+    t = np.arange(N)
+    sig = np.sin(2 * np.pi * 0.05 * (t - N // 2)) * np.exp(-((t - N // 2) / (0.1 * N)) ** 2)
+    return (sig[None] + 0.7 * rng.standard_normal((T, N))).astype(np.float32)
+
+
+def _prep32(recs, B, kind):
+    T, N = recs.shape
+    g = 1 if B < 2 else B
+    pre, w = [], []
+    lin = np.zeros(N, np.float32)
+    inv_t = np.float32(1.0) / np.float32(T)
+    for a in range(0, T, g):
+        b = min(a + g, T)
+        s = np.zeros(N, np.float32)
+        for i in range(a, b):
+            s += recs[i]
+        if kind == "pws":
+            s = s * (np.float32(1.0) / np.float32(b - a))
+            wg = np.float32(b - a) / np.float32(T)
+            lin += s * wg
+        else:
+            wg = np.float32(b - a) * inv_t
+            lin += s * inv_t
+        pre.append(s)
+        w.append(wg)
+    return np.stack(pre).astype(np.float32), np.array(w, np.float32), lin
+
+
+def _batch(rng, npairs, T, N, B, kind):
+    recs = [_records(rng, T + p, N) for p in range(npairs)]  # ragged group counts
+    pre, w, lin, off = [], [], [], [0]
+    for r in recs:
+        a, b, c = _prep32(r, B, kind)
+        pre.append(a)
+        w.append(b)
+        lin.append(c)
+        off.append(off[-1] + len(b))
+    return recs, np.concatenate(pre), np.concatenate(w), np.stack(lin), np.array(off, np.int64)
+
+
+@pytest.mark.parametrize("N", [87, 345, 2001])
+def test_pws_matches_oracle(gpu, N):
+    import torch
+    from fastxc_amd import ops
+
+    rng = np.random.default_rng(N)
+    recs, pre, w, lin, off = _batch(rng, 3, 10, N, 4, "pws")
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(gpu)
+    out = ops.pws(d(pre), d(w), off, d(lin)).cpu().numpy()
+    for p, r in enumerate(recs):
+        g, wg, l = stack.pws_prepare(r, 4)
+        want = stack.pws(g, wg, l)
+        err = rel_l2(out[p], want)
+        print(f"pws N={N} pair={p} rel_l2={err:.3e}")
+        assert err < 1e-5
+
+
+@pytest.mark.parametrize("N,band", [(87, None), (345, None), (2001, None), (345, (0.1, 0.3, 0.02))])
+def test_tfpws_matches_oracle(gpu, N, band):
+    import torch
+    from fastxc_amd import ops
+
+    rng = np.random.default_rng(N + 1)
+    recs, pre, w, lin, off = _batch(rng, 2, 8, N, 3, "tfpws")
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(gpu)
+    out = ops.tfpws(d(pre), d(w), off, d(lin), band=band, delta=1.0).cpu().numpy()
+    for p, r in enumerate(recs):
+        s, wg, l = stack.tfpws_prepare(r, 3)
+        want = stack.tfpws(s, wg, l, band=band, delta=1.0)
+        err = rel_l2(out[p], want)
+        print(f"tfpws N={N} band={band} pair={p} rel_l2={err:.3e}")
+        assert err < 1e-4
+
+
+def test_tfpws_identical_groups_vanish(gpu):
+    import torch
+    from fastxc_amd import ops
+
+    rng = np.random.default_rng(5)
+    tr = rng.standard_normal(345).astype(np.float32)
+    recs = np.tile(tr, (6, 1))
+    pre, w, lin = _prep32(recs, 2, "tfpws")
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(gpu)
+    out = ops.tfpws(d(pre / 2), d(w), np.array([0, 3]), d(lin[None])).cpu().numpy()
+    assert np.max(np.abs(out)) < 1e-4 * np.max(np.abs(lin))
