@@ -16,7 +16,9 @@ LIB_PATH = os.path.join(_HERE, "libfastxc_amd.so")
 FXC_PREC_FP32 = 0
 FXC_PREC_BF16X3 = 1
 FXC_PREC_FP16 = 2
-PRECISIONS = {"fp32": FXC_PREC_FP32, "bf16x3": FXC_PREC_BF16X3, "fp16": FXC_PREC_FP16}
+FXC_PREC_PLAIN = 3
+PRECISIONS = {"fp32": FXC_PREC_FP32, "bf16x3": FXC_PREC_BF16X3, "fp16": FXC_PREC_FP16,
+              "plain": FXC_PREC_PLAIN}
 
 _lib = None
 

@@ -93,9 +93,12 @@ struct FftPlan {
 // a prime factor with no butterfly (then a Bluestein plan is needed).
 // Radices above max_radix are not used (kernels with tight register budgets
 // ask for small butterflies).
-bool factor_radices(int n, int *radix, int *npass, int max_radix = 32);
+// largest_last: the radices in ascending order, so the last pass (stride 1)
+// has the largest butterfly (used when a consumer folds the last pass into
+// its output step).
+bool factor_radices(int n, int *radix, int *npass, int max_radix = 32, bool largest_last = false);
 // Builds the plan and uploads its FP64-computed twiddle table.
-int fft_plan_create(FftPlan *plan, int n, int sign, int max_radix = 32);
+int fft_plan_create(FftPlan *plan, int n, int sign, int max_radix = 32, bool largest_last = false);
 void fft_plan_destroy(FftPlan *plan);
 
 // Device memory helpers that record an error on failure.

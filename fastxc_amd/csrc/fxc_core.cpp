@@ -1,6 +1,7 @@
 // Host-side core: error strings, device allocation, FFT plan construction.
 #include "fxc_common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -38,7 +39,7 @@ void dfree(void *ptr) {
 // Radix preference: large register-resident butterflies first keeps the LDS
 // pass count low (18522 = 21*21*21*2, 18000 = 16*15*15*5, 18144 = 21*16*9*6);
 // the odd primes 23 and 29 cover the correlation length 2001 = 3*23*29.
-bool factor_radices(int n, int *radix, int *npass, int max_radix) {
+bool factor_radices(int n, int *radix, int *npass, int max_radix, bool largest_last) {
   static const int prefs[] = {21, 16, 15, 14, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2, 23, 29};
   int m = n;
   int cnt = 0;
@@ -56,14 +57,15 @@ bool factor_radices(int n, int *radix, int *npass, int max_radix) {
     *npass = 0;
     return true;
   }
+  if (largest_last) std::sort(radix, radix + cnt);
   *npass = cnt;
   return true;
 }
 
-int fft_plan_create(FftPlan *plan, int n, int sign, int max_radix) {
+int fft_plan_create(FftPlan *plan, int n, int sign, int max_radix, bool largest_last) {
   FftPlanDev d{};
   d.n = n;
-  if (!factor_radices(n, d.radix, &d.npass, max_radix)) {
+  if (!factor_radices(n, d.radix, &d.npass, max_radix, largest_last)) {
     set_error("fft length %d has a prime factor without a butterfly", n);
     return FXC_ERR_UNSUPPORTED;
   }

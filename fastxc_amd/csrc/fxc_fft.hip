@@ -16,13 +16,13 @@
 
 namespace fxc {
 
-int real_fft_plan_create(RealFftPlan *plan, int n, int sign, int max_radix) {
+int real_fft_plan_create(RealFftPlan *plan, int n, int sign, int max_radix, bool largest_last) {
   if (n < 2 || (n & 1)) {
     set_error("real fft length %d must be even and >= 2", n);
     return FXC_ERR_ARG;
   }
   plan->n = n;
-  FXC_TRY(fft_plan_create(&plan->half, n / 2, sign, max_radix));
+  FXC_TRY(fft_plan_create(&plan->half, n / 2, sign, max_radix, largest_last));
   const int M = n / 2;
   std::vector<float2> fold(fold_lds_count(M));
   for (int u = 0; u < kTwFine; ++u) {

@@ -208,10 +208,12 @@ __device__ __forceinline__ float2 tw_lookup(const float2 *tw, int e) {
   return cmul(tw[kTwFine + (e >> 7)], tw[e & (kTwFine - 1)]);
 }
 
-// Multiplies v[1..R-1] by w^(q*e), q = 1..R-1 (exponents modulo n).
+// Multiplies v[1..R-1] by w^(q*e), q = 1..R-1 (exponents modulo n).  Up to
+// radix 16 the powers come from one table read and a chain of products
+// (at most 14 roundings, ~1e-6 relative); larger radices read each power.
 template <int R>
 __device__ __forceinline__ void apply_twiddles(float2 *v, const float2 *tw, int e, int n) {
-  if constexpr (R <= 8) {
+  if constexpr (R <= 16) {
     const float2 w = tw_lookup(tw, e);
     float2 wq = w;
     v[1] = cmul(v[1], w);
@@ -347,6 +349,54 @@ __device__ inline void lds_fft(float2 *buf, int ntrans, const FftPlanDev &plan, 
   for (int p = 0; p < plan.npass; ++p) {
     FXC_RADIX_SWITCH(dif_pass)
   }
+}
+
+// The first npass_run DIF passes only.  After passes 0..P-2 every block of
+// span R = radix[P-1] contiguous values holds the input of an R-point DFT
+// whose output t is bin k = lower + (n/R) t, where the block starts at the
+// digit-reversed position of lower = k mod (n/R) (see last_pass_bin).
+template <unsigned kMask = kRadixSmooth>
+__device__ inline void lds_fft_head(float2 *buf, int ntrans, const FftPlanDev &plan,
+                                    const float2 *tw, float sgn, int npass_run) {
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  for (int p = 0; p < npass_run; ++p) {
+    FXC_RADIX_SWITCH(dif_pass)
+  }
+}
+
+// Bin k of a transform whose last pass was left out (lds_fft_head with
+// npass_run = P-1): the direct R-point DFT of its block,
+//   Z[k] = sum_{m<R} buf[base + m] w_R^(m t),  w_R = w_n^(n/R),
+// with the powers w_R^(m t) re-read from the table every 8 terms.
+__device__ inline float2 last_pass_bin(const float2 *buf, const FftPlanDev &plan,
+                                       const float2 *tw, int k) {
+  const int P = plan.npass;
+  if (P == 0) return buf[0];
+  const int R = plan.radix[P - 1];
+  const int g = plan.n / R;
+  const int t = (int)udiv_magic((unsigned)k, (unsigned)g, plan.bmagic[P - 1]);
+  int lower = k - t * g;
+  int base = 0;
+  for (int p = 0; p < P - 1; ++p) {
+    const int r = plan.radix[p];
+    const int q = (int)udiv_magic((unsigned)lower, (unsigned)r, plan.rmagic[p]);
+    base += (lower - q * r) * plan.stride[p];
+    lower = q;
+  }
+  const float2 wt = tw_lookup(tw, t * g);
+  float2 z = buf[base];
+  float2 wm = wt;
+  int e = t;  // (m t) mod R for m = 1
+  for (int m = 1; m < R; ++m) {
+    const float2 v = buf[base + m];
+    z.x += v.x * wm.x - v.y * wm.y;
+    z.y += v.x * wm.y + v.y * wm.x;
+    e += t;
+    if (e >= R) e -= R;
+    wm = ((m + 1) & 7) == 0 ? tw_lookup(tw, e * g) : cmul(wm, wt);
+  }
+  return z;
 }
 
 // Digit-reversed input -> natural-order output (the inverse ordering of

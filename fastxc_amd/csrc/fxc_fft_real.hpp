@@ -42,7 +42,8 @@ __device__ inline void stage_real_tables(float2 *lds, const FftPlanDev &plan,
   __syncthreads();
 }
 
-int real_fft_plan_create(RealFftPlan *plan, int n, int sign, int max_radix = 32);
+int real_fft_plan_create(RealFftPlan *plan, int n, int sign, int max_radix = 32,
+                         bool largest_last = false);
 void real_fft_plan_destroy(RealFftPlan *plan);
 // Largest number of transforms one 1024-thread workgroup can hold.
 int max_transforms_per_wg(const FftPlanDev &plan, int nthreads, int lds_complex_capacity);

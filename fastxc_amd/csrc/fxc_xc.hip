@@ -16,6 +16,9 @@
 //   BF16X3  spectra split into bf16 hi+lo, three MFMA products
 //           (hi*hi + hi*lo + lo*hi) with fp32 accumulation;
 //   FP16    one fp16 MFMA product with a power-of-two scale per station.
+// A fourth form, PLAIN, keeps the reference's own step order on the GPU (the
+// stack row of every pair read, updated and written once per window, fp32);
+// it exists to time the reference's method beside the tiled forms.
 // The cross-spectra go to an HBM workspace in [pair][bin] rows; the lag
 // kernel folds each row into an 18522-point complex inverse FFT held wholly
 // in LDS and writes only the 2H+1 lags.
@@ -24,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -237,11 +241,10 @@ contract_fp32_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int n
   }
 }
 
-// MFMA contraction.  8 waves: wave = (subtile 0..3 of 16x16 stations, bin
-// half).  Per bin: Cr = A.B1, Ci = A.B2 with a_s = [Re | Im] and
+// MFMA contraction.  Per bin: Cr = A.B1, Ci = A.B2 with a_s = [Re | Im] and
 // b2_s = [Im | -Re], so conj(a)*b = Cr + i Ci.  Results are staged in LDS as
 // [pair][bin] and written as 128-byte rows.
-// Operand fragments of one lane for one bin and one split part (hi or lo).
+// Operand fragments of one lane for one bin and one split part (hi or lo):
 // K' = 2*kh = 32*NK32 with kh a multiple of 16, so every K step is one
 // 16x16x32 MFMA: a dependent accumulate chain never mixes MFMA shapes.
 // Source side: A; receiver side: B1 = [Re|Im] and B2 = [Im|-Re].
@@ -249,10 +252,20 @@ template <int NK32>
 struct SrcFrag {
   uint4 a32[NK32];
 };
+// B2 = [Im | -Re] is B1 with its halves swapped and one half negated.  When
+// kh is a multiple of 32 (NK32 even) the swap moves whole K steps, so B2 is
+// formed in registers from B1; otherwise it is loaded separately.
 template <int NK32>
 struct RecFrag {
-  uint4 b32[NK32], c32[NK32];
+  static constexpr bool kDerived = (NK32 % 2) == 0;
+  uint4 b32[NK32];
+  uint4 c32[kDerived ? 1 : NK32];
 };
+
+__device__ __forceinline__ uint4 neg_bits16(uint4 v) {
+  v.x ^= 0x80008000u; v.y ^= 0x80008000u; v.z ^= 0x80008000u; v.w ^= 0x80008000u;
+  return v;
+}
 
 template <int NK32>
 __device__ __forceinline__ void load_src(const unsigned short *a, int lane_g, SrcFrag<NK32> &o) {
@@ -266,13 +279,23 @@ __device__ __forceinline__ void load_rec(const unsigned short *b, int lane_g, Re
 #pragma unroll
   for (int s = 0; s < NK32; ++s) {
     const int ea = 32 * s + 8 * lane_g;
-    const bool hi_half = ea >= kh;
     o.b32[s] = *(const uint4 *)(b + ea);
-    uint4 c = *(const uint4 *)(b + (hi_half ? ea - kh : ea + kh));
-    if (hi_half) {
-      c.x ^= 0x80008000u; c.y ^= 0x80008000u; c.z ^= 0x80008000u; c.w ^= 0x80008000u;
+    if constexpr (!RecFrag<NK32>::kDerived) {
+      const bool hi_half = ea >= kh;
+      const uint4 c = *(const uint4 *)(b + (hi_half ? ea - kh : ea + kh));
+      o.c32[s] = hi_half ? neg_bits16(c) : c;
     }
-    o.c32[s] = c;
+  }
+}
+
+// K step s of B2.
+template <int NK32>
+__device__ __forceinline__ uint4 rec_b2(const RecFrag<NK32> &y, int s) {
+  if constexpr (RecFrag<NK32>::kDerived) {
+    constexpr int h = NK32 / 2;
+    return s < h ? y.b32[s + h] : neg_bits16(y.b32[s - h]);
+  } else {
+    return y.c32[s];
   }
 }
 
@@ -294,17 +317,36 @@ __device__ __forceinline__ void mfma_bin(const SrcFrag<NK32> &x, const RecFrag<N
 #pragma unroll
   for (int s = 0; s < NK32; ++s) {
     mfma32<kMode>(x.a32[s], y.b32[s], cr);
-    mfma32<kMode>(x.a32[s], y.c32[s], ci);
+    mfma32<kMode>(x.a32[s], rec_b2(y, s), ci);
   }
 }
 
+// LDS slot of (pair p, bin b) in the [pair][16 bins] staging tile.  The bin
+// is XOR-swizzled with the pair's low four bits: an MFMA result register
+// holds one bin for 16 consecutive pairs, which would otherwise sit 128 bytes
+// apart in one bank group; swizzled, the 16 writes land in 16 distinct bank
+// pairs, and a 16-thread group still reads one pair's 16 bins as a
+// permutation of one 128-byte row.
+__device__ __forceinline__ int stage_index(int p, int b) { return p * kBinChunk + (b ^ (p & 15)); }
+
+// Operands of one bin for the whole 32x32 tile: two 16-station source blocks
+// and two 16-station receiver blocks, hi (and lo for the split form).
+template <int NK32, bool kSplit>
+struct TileOps {
+  SrcFrag<NK32> sh[2], sl[kSplit ? 2 : 1];
+  RecFrag<NK32> rh[2], rl[kSplit ? 2 : 1];
+};
+
+// One workgroup = one 32x32 station tile x 16 bins; 8 waves, each computing
+// the whole tile (four 16x16 MFMA blocks) for two of the 16 bins, so every
+// station fragment is read once per workgroup.
 template <int kMode, int NK32>
 __global__ void __launch_bounds__(512)
 contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short *__restrict__ lo,
                      int nsta, int nspec, const float *__restrict__ sta_scale,
                      const TileDesc *__restrict__ tiles, const int *__restrict__ tile_rows,
                      float2 *__restrict__ xspec, long long xpitch, int ntiles, float scale) {
-  extern __shared__ float2 stage[];  // [kTilePairs][kBinChunk]
+  extern __shared__ float2 stage[];  // [kTilePairs][kBinChunk], swizzled
   __shared__ int rows[kTilePairs];
   int tile, chunk;
   if (!tile_chunk(ntiles, (nspec + kBinChunk - 1) / kBinChunk, tile, chunk)) return;
@@ -314,65 +356,78 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
   const TileDesc td = tiles[tile];
   const int wave = tid >> 6;
   const int lane = tid & 63;
-  const int sub = wave & 3;
-  const int half = wave >> 2;
-  const int si = sub >> 1, sj = sub & 1;
   const int lane_r = lane & 15, lane_g = lane >> 4;
-  const int s_sta = td.src_base + si * 16 + lane_r;
-  const int r_sta = td.rec_base + sj * 16 + lane_r;
-  const bool r_ok = r_sta < nsta;
-  // Stations past the array end load a valid station; their rows/columns
-  // belong to pairs that are never stored.
-  const int s_ld = s_sta < nsta ? s_sta : nsta - 1;
-  const int r_ld = r_ok ? r_sta : nsta - 1;
   constexpr int kp = 32 * NK32;
   constexpr bool kSplit = kMode == FXC_PREC_BF16X3;
+  // Stations past the array end load a valid station; their rows/columns
+  // belong to pairs that are never stored.
+  size_t s_off[2], r_off[2];
+  float col_scale[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int s_sta = td.src_base + h * 16 + lane_r;
+    const int r_sta = td.rec_base + h * 16 + lane_r;
+    s_off[h] = (size_t)(s_sta < nsta ? s_sta : nsta - 1) * nspec * kp;
+    r_off[h] = (size_t)(r_sta < nsta ? r_sta : nsta - 1) * nspec * kp;
+    col_scale[h] = scale;
+    if constexpr (kMode == FXC_PREC_FP16) col_scale[h] *= (r_sta < nsta ? sta_scale[r_sta] : 0.f);
+  }
+  float row_scale[2][4];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      row_scale[h][r] = 1.f;
+      if constexpr (kMode == FXC_PREC_FP16) {
+        const int sa = td.src_base + h * 16 + 4 * lane_g + r;
+        row_scale[h][r] = sa < nsta ? sta_scale[sa] : 0.f;
+      }
+    }
   const int k0 = chunk * kBinChunk;
-  // bins whose operands are loaded ahead together, sized to ~160 VGPRs of fragments
-  constexpr int kFragRegs = 12 * NK32 * (kSplit ? 2 : 1);
-  constexpr int kGroupRaw = 160 / kFragRegs;
-  constexpr int kGroup = kGroupRaw >= 4 ? 4 : (kGroupRaw >= 2 ? 2 : 1);
-  float col_scale = scale;
-  if constexpr (kMode == FXC_PREC_FP16) col_scale *= (r_ok ? sta_scale[r_sta] : 0.f);
-  for (int g0 = 0; g0 < kBinChunk / 2; g0 += kGroup) {
-    SrcFrag<NK32> sh[kGroup], sl[kSplit ? kGroup : 1];
-    RecFrag<NK32> rh[kGroup], rl[kSplit ? kGroup : 1];
+  auto one_bin = [&](int b) {
+    int k = k0 + b;
+    k = k < nspec ? k : nspec - 1;  // tail bins compute a valid bin, never stored
+    TileOps<NK32, kSplit> o;
 #pragma unroll
-    for (int i = 0; i < kGroup; ++i) {
-      int k = k0 + half * (kBinChunk / 2) + g0 + i;
-      k = k < nspec ? k : nspec - 1;  // tail bins compute a valid bin, never stored
-      const size_t ao = ((size_t)s_ld * nspec + k) * kp;
-      const size_t bo = ((size_t)r_ld * nspec + k) * kp;
-      load_src(hi + ao, lane_g, sh[i]);
-      load_rec(hi + bo, lane_g, rh[i]);
+    for (int h = 0; h < 2; ++h) {
+      const size_t ao = s_off[h] + (size_t)k * kp;
+      const size_t bo = r_off[h] + (size_t)k * kp;
+      load_src(hi + ao, lane_g, o.sh[h]);
+      load_rec(hi + bo, lane_g, o.rh[h]);
       if constexpr (kSplit) {
-        load_src(lo + ao, lane_g, sl[i]);
-        load_rec(lo + bo, lane_g, rl[i]);
+        load_src(lo + ao, lane_g, o.sl[h]);
+        load_rec(lo + bo, lane_g, o.rl[h]);
       }
     }
 #pragma unroll
-    for (int i = 0; i < kGroup; ++i) {
-      const int b = half * (kBinChunk / 2) + g0 + i;
-      f32x4_t cr = {0.f, 0.f, 0.f, 0.f};
-      f32x4_t ci = {0.f, 0.f, 0.f, 0.f};
-      mfma_bin<kMode>(sh[i], rh[i], cr, ci);  // hi.hi
-      if constexpr (kSplit) {
-        mfma_bin<kMode>(sh[i], rl[i], cr, ci);  // hi.lo
-        mfma_bin<kMode>(sl[i], rh[i], cr, ci);  // lo.hi
-      }
-      // D layout: lane holds rows 4*lane_g + r, column lane_r of the 16x16 block.
+    for (int si = 0; si < 2; ++si)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int a = si * 16 + 4 * lane_g + r;
-        const int c = sj * 16 + lane_r;
-        float sc = col_scale;
-        if constexpr (kMode == FXC_PREC_FP16) {
-          const int sa = td.src_base + a;
-          sc *= (sa < nsta) ? sta_scale[sa] : 0.f;
+      for (int sj = 0; sj < 2; ++sj) {
+        f32x4_t cr = {0.f, 0.f, 0.f, 0.f};
+        f32x4_t ci = {0.f, 0.f, 0.f, 0.f};
+        mfma_bin<kMode>(o.sh[si], o.rh[sj], cr, ci);  // hi.hi
+        if constexpr (kSplit) {
+          mfma_bin<kMode>(o.sl[si], o.rh[sj], cr, ci);  // lo.hi
+          mfma_bin<kMode>(o.sh[si], o.rl[sj], cr, ci);  // hi.lo
         }
-        stage[(a * kTile + c) * kBinChunk + b] = make_float2(cr[r] * sc, ci[r] * sc);
+        // D layout: lane holds rows 4*lane_g + r, column lane_r of the 16x16 block.
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int a = si * 16 + 4 * lane_g + r;
+          const int c = sj * 16 + lane_r;
+          const float sc = col_scale[sj] * row_scale[si][r];
+          stage[stage_index(a * kTile + c, b)] = make_float2(cr[r] * sc, ci[r] * sc);
+        }
       }
-    }
+  };
+  // Two bins per wave.  Small K: both bins' loads may be in flight at once;
+  // large K: one bin at a time keeps the operands within the register file.
+  if constexpr (NK32 * (kSplit ? 2 : 1) <= 4) {
+#pragma unroll
+    for (int bi = 0; bi < kBinChunk / 8; ++bi) one_bin(wave + 8 * bi);
+  } else {
+#pragma unroll 1
+    for (int bi = 0; bi < kBinChunk / 8; ++bi) one_bin(wave + 8 * bi);
   }
   __syncthreads();
   // 16 consecutive threads write one pair's 16 bins (128 bytes).
@@ -381,7 +436,7 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
   if (k < nspec) {
     for (int p = tid >> 4; p < kTilePairs; p += blockDim.x >> 4) {
       const int row = rows[p];
-      if (row >= 0) store_stream(xspec + (size_t)row * xpitch + k, stage[p * kBinChunk + b]);
+      if (row >= 0) store_stream(xspec + (size_t)row * xpitch + k, stage[stage_index(p, b)]);
     }
   }
 }
@@ -396,11 +451,14 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
 // [nfft/2 - H, nfft/2 + H].  The scale is evaluated as the reference does:
 // weight = 1.0/(double)((float)nfft * dt), product in double, rounded to float.
 // ---------------------------------------------------------------------------
-// 1024 threads with butterflies up to radix 8 measured faster than 512
-// threads with radix-21 butterflies (latency, not ALU, bounds this kernel).
+// 1024 threads (16 waves, 128 VGPRs each, 40 of them holding the prefetched
+// next row).  The butterfly set is a template parameter: radices up to 8, or
+// additionally the register-resident composite 9, which takes one LDS pass
+// fewer on 2*3^3*7^3 (radix 14 no longer fits the register budget).
 constexpr int kLagThreads = 1024;
-constexpr int kLagMaxRadix = 8;
-constexpr unsigned kRadixLag = (1u << 2) | (1u << 3) | (1u << 4) | (1u << 5) | (1u << 6) | (1u << 7) | (1u << 8);
+constexpr int kLagMaxRadixDefault = 9;
+constexpr unsigned kRadixLag8 = (1u << 2) | (1u << 3) | (1u << 4) | (1u << 5) | (1u << 6) | (1u << 7) | (1u << 8);
+constexpr unsigned kRadixLag9 = kRadixLag8 | (1u << 9);
 constexpr int kLagRegs = (kLdsComplexCapacity + kLagThreads - 1) / kLagThreads;
 
 __device__ __forceinline__ void lag_prefetch(float2 (&pre)[kLagRegs], const float2 *__restrict__ X,
@@ -412,6 +470,7 @@ __device__ __forceinline__ void lag_prefetch(float2 (&pre)[kLagRegs], const floa
   }
 }
 
+template <unsigned kRadixLag>
 __global__ void __launch_bounds__(kLagThreads)
 lag_kernel(const float2 *__restrict__ xspec, long long xpitch, int nspec, long long nrows,
            const int *__restrict__ row_out, float *__restrict__ cc, int half_cc, double weight,
@@ -453,19 +512,47 @@ lag_kernel(const float2 *__restrict__ xspec, long long xpitch, int nspec, long l
       }
     }
     __syncthreads();
-    lds_fft<kRadixLag>(lds, 1, plan, twl, 1.f);
+    // all passes but the last; the last (largest radix, stride 1) is
+    // evaluated only for the bins that hold kept lags
+    lds_fft_head<kRadixLag>(lds, 1, plan, twl, 1.f, plan.npass - 1);
     const int out_row = row_out ? row_out[row] : (int)row;
     float *y = cc + (size_t)out_row * (2 * half_cc + 1);
-    for (int m = threadIdx.x; m <= 2 * half_cc; m += blockDim.x) {
-      const int l = m - half_cc;
-      const int par = l & 1;
-      int n = (l - par) / 2;
-      n = n < 0 ? n + M : n;
-      const float2 z = lds[fft_pos(plan, n)];
-      const float v = par ? z.y : z.x;
-      y[m] = (float)((double)v * weight);
+    // lag l = 2n + par reads Re (par 0) or Im (par 1) of Z[n mod M]
+    const int n_lo = -((half_cc + 1) / 2), n_hi = half_cc / 2;
+    for (int i = threadIdx.x; i <= n_hi - n_lo; i += blockDim.x) {
+      const int n = n_lo + i;
+      const float2 z = last_pass_bin(lds, plan, twl, n < 0 ? n + M : n);
+      const int m0 = 2 * n + half_cc;
+      if (m0 >= 0) y[m0] = (float)((double)z.x * weight);
+      if (m0 + 1 <= 2 * half_cc) y[m0 + 1] = (float)((double)z.y * weight);
     }
     __syncthreads();
+  }
+}
+
+// PLAIN form, one launch per window w (the reference's accumulation order):
+//   stack[r][k] += (conj(S[w][src_r][k]) * S[w][rec_r][k]) * scale
+// Work items are (row, 256-bin block) pairs, nrows * nbk of them, visited
+// with a grid-stride loop so the launch stays within the 2^32 work-item
+// limit of one dispatch.
+__global__ void __launch_bounds__(256)
+plain_accum_kernel(const float2 *__restrict__ spec_w, int nspec, const int *__restrict__ row_src,
+                   const int *__restrict__ row_rec, long long nrows, float2 *__restrict__ stack,
+                   long long xpitch, int nbk, float scale) {
+  const long long nitems = nrows * nbk;
+  for (long long it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const long long row = it / nbk;
+    const int k = (int)(it % nbk) * 256 + threadIdx.x;
+    if (k >= nspec) continue;
+    const float2 a = spec_w[(size_t)row_src[row] * nspec + k];
+    const float2 b = spec_w[(size_t)row_rec[row] * nspec + k];
+    const float pr = a.x * b.x + a.y * b.y;
+    const float pi = a.x * b.y - a.y * b.x;
+    float2 *d = stack + (size_t)row * xpitch + k;
+    float2 v = *d;
+    v.x += pr * scale;
+    v.y += pi * scale;
+    *d = v;
   }
 }
 
@@ -492,6 +579,7 @@ struct fxc_xc_plan {
   float dt = 0.f;
   int precision = FXC_PREC_FP32;
   int kh = 0;
+  int lag_max_radix = 8;           // largest butterfly of the lag FFT plan
   long long chunk_rows_max = 0;  // cross-spectrum rows per workspace fill
   // pairs
   int nsta = 0;
@@ -502,6 +590,8 @@ struct fxc_xc_plan {
   TileDesc *d_tiles = nullptr;
   int *d_tile_rows = nullptr;     // chunk-relative rows
   int *d_row_task = nullptr;      // global row -> task index
+  int *d_row_src = nullptr;       // global row -> src station (PLAIN form)
+  int *d_row_rec = nullptr;       // global row -> rec station (PLAIN form)
   // workspaces
   float2 *d_xspec = nullptr;
   size_t xspec_rows = 0;
@@ -537,9 +627,13 @@ static void xc_free_pairs(fxc_xc_plan *p) {
   dfree(p->d_tiles);
   dfree(p->d_tile_rows);
   dfree(p->d_row_task);
+  dfree(p->d_row_src);
+  dfree(p->d_row_rec);
   p->d_tiles = nullptr;
   p->d_tile_rows = nullptr;
   p->d_row_task = nullptr;
+  p->d_row_src = nullptr;
+  p->d_row_rec = nullptr;
   p->h_tiles.clear();
   p->chunk_tile_begin.clear();
   p->chunk_row_begin.clear();
@@ -554,16 +648,19 @@ extern "C" int fxc_xc_plan_create(fxc_xc_plan **out, int nspec, int nstep, float
                 "fxc_xc_plan_create: invalid shape nspec=%d nstep=%d dt=%g half_cc=%d", nspec,
                 nstep, (double)dt, half_cc);
   FXC_CHECK_ARG(precision == FXC_PREC_FP32 || precision == FXC_PREC_BF16X3 ||
-                    precision == FXC_PREC_FP16,
+                    precision == FXC_PREC_FP16 || precision == FXC_PREC_PLAIN,
                 "fxc_xc_plan_create: unknown precision %d", precision);
   const int nfft = 2 * (nspec - 1);
   FXC_CHECK_ARG(2 * half_cc + 1 <= nfft, "fxc_xc_plan_create: 2*half_cc+1 exceeds nfft");
   FXC_CHECK_ARG(nfft / 2 <= kLdsComplexCapacity,
                 "fxc_xc_plan_create: nfft %d exceeds the single-workgroup FFT size", nfft);
-  if (precision != FXC_PREC_FP32)
+  if (precision == FXC_PREC_BF16X3 || precision == FXC_PREC_FP16)
     FXC_CHECK_ARG(nstep <= 128, "fxc_xc_plan_create: MFMA forms support up to 128 windows");
+  // lag butterfly set: radices up to 9 (default) or 8; FXC_XC_LAG_MAX_RADIX overrides
+  int lag_max_radix = kLagMaxRadixDefault;
+  if (const char *env = getenv("FXC_XC_LAG_MAX_RADIX")) lag_max_radix = atoi(env) >= 9 ? 9 : 8;
   const RealFftPlan *rp = nullptr;
-  FXC_TRY(plan_cache_real(nfft, 1, &rp, kLagMaxRadix));
+  FXC_TRY(plan_cache_real(nfft, 1, &rp, lag_max_radix, true));
   FXC_CHECK_ARG(real_lds_count(nfft / 2) <= kLdsComplexCapacity,
                 "fxc_xc_plan_create: nfft %d does not fit one workgroup", nfft);
   fxc_xc_plan *p = new fxc_xc_plan();
@@ -575,8 +672,11 @@ extern "C" int fxc_xc_plan_create(fxc_xc_plan **out, int nspec, int nstep, float
   p->dt = dt;
   p->precision = precision;
   p->xpitch = (nspec + 15) / 16 * 16;
-  p->kh = (nstep + 15) / 16 * 16;  // K' = 2*kh is a whole number of 16x16x32 steps
+  // K' = 2*kh is a whole number of 16x16x32 steps; beyond 16 windows kh is a
+  // multiple of 32 so the receiver's [Im|-Re] operand is formed in registers
+  p->kh = nstep <= 16 ? 16 : (nstep + 31) / 32 * 32;
   p->chunk_rows_max = 0;
+  p->lag_max_radix = lag_max_radix;
   *out = p;
   return FXC_OK;
 }
@@ -707,6 +807,17 @@ extern "C" int fxc_xc_set_pairs(fxc_xc_plan *p, int nsta, const int *h_src, cons
   FXC_HIP(hipMemcpy(p->d_tile_rows, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice));
   FXC_HIP(hipMemcpy(p->d_row_task, row_task.data(), row_task.size() * sizeof(int),
                     hipMemcpyHostToDevice));
+  if (p->precision == FXC_PREC_PLAIN) {
+    std::vector<int> rs(row_task.size()), rr(row_task.size());
+    for (size_t i = 0; i < row_task.size(); ++i) {
+      rs[i] = h_src[row_task[i]];
+      rr[i] = h_rec[row_task[i]];
+    }
+    FXC_TRY(dmalloc((void **)&p->d_row_src, rs.size() * sizeof(int)));
+    FXC_TRY(dmalloc((void **)&p->d_row_rec, rr.size() * sizeof(int)));
+    FXC_HIP(hipMemcpy(p->d_row_src, rs.data(), rs.size() * sizeof(int), hipMemcpyHostToDevice));
+    FXC_HIP(hipMemcpy(p->d_row_rec, rr.data(), rr.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   return FXC_OK;
 }
 
@@ -731,7 +842,7 @@ static int xc_ensure_workspace(fxc_xc_plan *p, size_t rows) {
     FXC_TRY(dmalloc((void **)&p->d_xspec, rows * (size_t)p->xpitch * sizeof(float2)));
     p->xspec_rows = rows;
   }
-  if (p->precision != FXC_PREC_FP32) {
+  if (p->precision == FXC_PREC_BF16X3 || p->precision == FXC_PREC_FP16) {
     const size_t elems = (size_t)p->nsta * p->nspec * (size_t)(2 * p->kh);
     if (elems > p->split_elems) {
       dfree(p->d_hi);
@@ -758,7 +869,7 @@ static int xc_ensure_workspace(fxc_xc_plan *p, size_t rows) {
 
 // Split pass for the MFMA forms (no-op for FP32).
 static int xc_prepare_spectra(fxc_xc_plan *p, const float2 *d_spec, hipStream_t st) {
-  if (p->precision == FXC_PREC_FP32) return FXC_OK;
+  if (p->precision == FXC_PREC_FP32 || p->precision == FXC_PREC_PLAIN) return FXC_OK;
   FXC_TRY(xc_mark(p, kTimeSplit, true, st));
   const dim3 grid((unsigned)ceil_div(p->nspec, 256), (unsigned)p->nsta);
   if (p->precision == FXC_PREC_FP16) {
@@ -800,7 +911,7 @@ static int launch_mfma_mode(int kh, dim3 grid, size_t lds, hipStream_t st,
 #define FXC_KH(V) \
   case V: return launch_mfma_kh<kMode, V>(grid, lds, st, hi, lo, nsta, nspec, sc, tiles, trows, dst, xpitch, scale);
   switch (kh) {
-    FXC_KH(16) FXC_KH(32) FXC_KH(48) FXC_KH(64) FXC_KH(80) FXC_KH(96) FXC_KH(112) FXC_KH(128)
+    FXC_KH(16) FXC_KH(32) FXC_KH(64) FXC_KH(96) FXC_KH(128)
     default: break;
   }
 #undef FXC_KH
@@ -830,7 +941,17 @@ static int xc_contract_chunk(fxc_xc_plan *p, const float2 *d_spec, long long c, 
   const TileDesc *tiles = p->d_tiles + t0;
   const int *trows = p->d_tile_rows + t0 * kTilePairs;
   FXC_TRY(xc_mark(p, kTimeContract, true, st));
-  if (p->precision == FXC_PREC_FP32) {
+  if (p->precision == FXC_PREC_PLAIN) {
+    const long long r0 = p->chunk_row_begin[c], nrows = p->chunk_row_begin[c + 1] - r0;
+    const int nbk = ceil_div(p->nspec, 256);
+    FXC_HIP(hipMemsetAsync(dst, 0, (size_t)nrows * p->xpitch * sizeof(float2), st));
+    const size_t win = (size_t)p->nsta * p->nspec;
+    const long long grid = std::min<long long>(nrows * nbk, 1ll << 20);
+    for (int w = 0; w < p->nstep; ++w)
+      hipLaunchKernelGGL(plain_accum_kernel, dim3((unsigned)grid), dim3(256), 0, st,
+                         d_spec + w * win, p->nspec, p->d_row_src + r0, p->d_row_rec + r0, nrows,
+                         dst, p->xpitch, nbk, scale);
+  } else if (p->precision == FXC_PREC_FP32) {
     hipLaunchKernelGGL(contract_fp32_kernel, dim3(nwg), dim3(256), 0, st, d_spec, p->nstep,
                        p->nsta, p->nspec, tiles, trows, dst, p->xpitch, (int)ntl, scale);
   } else {
@@ -849,9 +970,10 @@ static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long
                         const int *row_out, float *d_cc, hipStream_t st) {
   if (nrows == 0) return FXC_OK;
   const RealFftPlan *rp = nullptr;
-  FXC_TRY(plan_cache_real(p->nfft, 1, &rp, kLagMaxRadix));
+  FXC_TRY(plan_cache_real(p->nfft, 1, &rp, p->lag_max_radix, true));
   const size_t lds = (size_t)real_lds_count(p->nfft / 2) * sizeof(float2);
-  FXC_HIP(hipFuncSetAttribute((const void *)lag_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+  auto kern = p->lag_max_radix >= 9 ? lag_kernel<kRadixLag9> : lag_kernel<kRadixLag8>;
+  FXC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
   int dev = 0, ncu = 0;
   FXC_HIP(hipGetDevice(&dev));
@@ -860,7 +982,7 @@ static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long
   // reference weight: 1.0 / (width * dt) with width converted to float
   const double weight = 1.0 / (double)((float)p->nfft * p->dt);
   FXC_TRY(xc_mark(p, kTimeLag, true, st));
-  hipLaunchKernelGGL(lag_kernel, dim3((unsigned)grid), dim3(kLagThreads), lds, st, xs, xpitch,
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kLagThreads), lds, st, xs, xpitch,
                      p->nspec, nrows, row_out, d_cc, p->half_cc, weight, rp->half.dev,
                      rp->half.d_tw, rp->d_fold);
   FXC_HIP(hipGetLastError());

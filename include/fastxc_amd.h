@@ -50,6 +50,7 @@ int fxc_fft_c2r(const float *d_in, long long in_dist, float *d_out, long long ou
 #define FXC_PREC_FP32 0   /* fp32 vector FMA (reference arithmetic) */
 #define FXC_PREC_BF16X3 1 /* split-bf16 hi+lo MFMA, fp32 accumulation */
 #define FXC_PREC_FP16 2   /* fp16 MFMA with per-station power-of-2 scale */
+#define FXC_PREC_PLAIN 3  /* the reference's step order: per-window fp32 stack RMW */
 
 typedef struct fxc_xc_plan fxc_xc_plan;
 

@@ -11,7 +11,7 @@ from oracle import xc
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp32": 1e-5, "bf16x3": 1e-3, "fp16": 1e-3}
+TOL = {"fp32": 1e-5, "bf16x3": 1e-3, "fp16": 1e-3, "plain": 1e-5}
 
 
 def _spectra(rng, nstep, nsta, seg, nfft, dt, common=0.5):
@@ -26,7 +26,7 @@ def _all_pairs(nsta):
     return src.astype(np.int32), rec.astype(np.int32)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16", "plain"])
 @pytest.mark.parametrize("nsta,nstep,seg,nfft,H", [(40, 5, 800, 882 * 2, 100), (9, 24, 1500, 1764, 300),
                                                     (33, 47, 600, 1764, 200)])
 def test_correlate_matches_oracle(gpu, precision, nsta, nstep, seg, nfft, H):
