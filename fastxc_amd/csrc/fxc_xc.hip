@@ -89,45 +89,77 @@ __global__ void station_scale_kernel(const unsigned int *__restrict__ amax_bits,
   scale[s] = ldexpf(1.f, e);
 }
 
+// Split pass of the MFMA forms.  Operand layout [bin][K group][station][8]:
+// K' = 2*kh values per (station, bin) -- window w's real part at K index w,
+// its imaginary part at kh + w, zero for w >= nstep -- cut into groups of 8
+// consecutive K indices (16 bytes), with the stations of one group adjacent,
+// so the 16 lanes that hold one K group of 16 stations in an MFMA operand
+// read 256 contiguous bytes.  One workgroup converts 64 stations x 16 bins:
+// each block of 8 windows is staged through LDS so the reads run along the
+// bins and the writes along the stations.
+constexpr int kSplitSta = 64;
+constexpr int kSplitBins = 16;
+
 template <int kMode>
-__global__ void split_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec,
-                             int kh, const float *__restrict__ inv_scale,
-                             unsigned short *__restrict__ hi, unsigned short *__restrict__ lo) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = blockIdx.y;
-  if (k >= nspec) return;
-  const int kp = 2 * kh;
-  const size_t obase = ((size_t)s * nspec + k) * kp;
-  const float sc = (kMode == FXC_PREC_FP16) ? inv_scale[s] : 1.f;
-  for (int c = 0; c < kp; c += 8) {
-    unsigned short h[8], l[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = c + j;
-      const int w = e < kh ? e : e - kh;
-      float v = 0.f;
-      if (w < nstep) {
-        const float2 z = spec[((size_t)w * nsta + s) * nspec + k];
-        v = (e < kh ? z.x : z.y) * sc;
-      }
-      if (kMode == FXC_PREC_FP16) {
-        const _Float16 f = (_Float16)v;
-        h[j] = __builtin_bit_cast(unsigned short, f);
-        l[j] = 0;
-      } else {
-        const unsigned short hb = bf16_bits_rne(v);
-        h[j] = hb;
-        l[j] = bf16_bits_rne(v - bf16_bits_to_float(hb));
-      }
+__global__ void __launch_bounds__(256)
+split_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec, int kh,
+             const float *__restrict__ inv_scale, unsigned short *__restrict__ hi,
+             unsigned short *__restrict__ lo) {
+  __shared__ float2 tile[8][kSplitSta][kSplitBins + 1];
+  const int tid = threadIdx.x;
+  const int s0 = blockIdx.y * kSplitSta;
+  const int k0 = blockIdx.x * kSplitBins;
+  const int ngrp = kh / 4;  // K groups of 8 values: kh/8 real + kh/8 imaginary
+  for (int wb = 0; wb < kh; wb += 8) {
+    __syncthreads();
+    // load windows wb..wb+7: 16 lanes read one station's 16 bins (128 bytes)
+    for (int i = tid; i < 8 * kSplitSta * kSplitBins; i += blockDim.x) {
+      const int kk = i % kSplitBins;
+      const int ss = (i / kSplitBins) % kSplitSta;
+      const int ww = i / (kSplitBins * kSplitSta);
+      const int w = wb + ww, sidx = s0 + ss, k = k0 + kk;
+      float2 z = make_float2(0.f, 0.f);
+      if (w < nstep && sidx < nsta && k < nspec) z = spec[((size_t)w * nsta + sidx) * nspec + k];
+      tile[ww][ss][kk] = z;
     }
-    uint4 hv, lv;
-    hv.x = h[0] | ((unsigned)h[1] << 16); hv.y = h[2] | ((unsigned)h[3] << 16);
-    hv.z = h[4] | ((unsigned)h[5] << 16); hv.w = h[6] | ((unsigned)h[7] << 16);
-    *(uint4 *)(hi + obase + c) = hv;
-    if (kMode == FXC_PREC_BF16X3) {
-      lv.x = l[0] | ((unsigned)l[1] << 16); lv.y = l[2] | ((unsigned)l[3] << 16);
-      lv.z = l[4] | ((unsigned)l[5] << 16); lv.w = l[6] | ((unsigned)l[7] << 16);
-      *(uint4 *)(lo + obase + c) = lv;
+    __syncthreads();
+    // convert: thread = (station, bin); 64 consecutive threads write 64
+    // consecutive stations' 16-byte groups
+    for (int i = tid; i < kSplitSta * kSplitBins; i += blockDim.x) {
+      const int ss = i % kSplitSta;
+      const int kk = i / kSplitSta;
+      const int sidx = s0 + ss, k = k0 + kk;
+      if (sidx >= nsta || k >= nspec) continue;
+      const float sc = (kMode == FXC_PREC_FP16) ? inv_scale[sidx] : 1.f;
+#pragma unroll
+      for (int part = 0; part < 2; ++part) {  // 0: real parts, 1: imaginary parts
+        unsigned short h[8], l[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const float2 z = tile[j][ss][kk];
+          const float v = (part == 0 ? z.x : z.y) * sc;
+          if (kMode == FXC_PREC_FP16) {
+            h[j] = __builtin_bit_cast(unsigned short, (_Float16)v);
+            l[j] = 0;
+          } else {
+            const unsigned short hb = bf16_bits_rne(v);
+            h[j] = hb;
+            l[j] = bf16_bits_rne(v - bf16_bits_to_float(hb));
+          }
+        }
+        const int g = part * (kh / 8) + wb / 8;
+        const size_t o = (((size_t)k * ngrp + g) * nsta + sidx) * 8;
+        uint4 hv;
+        hv.x = h[0] | ((unsigned)h[1] << 16); hv.y = h[2] | ((unsigned)h[3] << 16);
+        hv.z = h[4] | ((unsigned)h[5] << 16); hv.w = h[6] | ((unsigned)h[7] << 16);
+        *(uint4 *)(hi + o) = hv;
+        if (kMode == FXC_PREC_BF16X3) {
+          uint4 lv;
+          lv.x = l[0] | ((unsigned)l[1] << 16); lv.y = l[2] | ((unsigned)l[3] << 16);
+          lv.z = l[4] | ((unsigned)l[5] << 16); lv.w = l[6] | ((unsigned)l[7] << 16);
+          *(uint4 *)(lo + o) = lv;
+        }
+      }
     }
   }
 }
@@ -149,19 +181,24 @@ __device__ __forceinline__ void store_stream(float2 *dst, float2 v) {
   __builtin_nontemporal_store(__builtin_bit_cast(unsigned long long, v), (unsigned long long *)dst);
 }
 
-// Workgroup -> (tile, bin chunk).  Workgroups are dispatched round-robin over
-// the 8 XCDs, so 8 consecutive workgroups take 8 consecutive bin chunks of one
+// Work item -> (tile, bin chunk).  Workgroups are dispatched round-robin over
+// the 8 XCDs, so 8 consecutive items take 8 consecutive bin chunks of one
 // tile: each XCD's L2 then serves one chunk of every station's spectra, and
-// each pair row receives 8 x 128 contiguous bytes at once.
+// each pair row receives 8 x 128 contiguous bytes at once.  A persistent
+// grid whose size is a multiple of 8 keeps item i on XCD i mod 8 as well.
 constexpr int kChunkGroup = 8;
 
-__device__ __forceinline__ bool tile_chunk(int ntiles, int nchunks, int &tile, int &chunk) {
-  const int per_group = ntiles * kChunkGroup;
-  const int g = blockIdx.x / per_group;
-  const int r = blockIdx.x - g * per_group;
+__device__ __forceinline__ bool tile_chunk_of(long long item, int ntiles, int nchunks, int &tile,
+                                              int &chunk) {
+  const long long per_group = (long long)ntiles * kChunkGroup;
+  const int g = (int)(item / per_group);
+  const int r = (int)(item - g * per_group);
   tile = r / kChunkGroup;
   chunk = g * kChunkGroup + (r - tile * kChunkGroup);
   return chunk < nchunks;
+}
+__device__ __forceinline__ bool tile_chunk(int ntiles, int nchunks, int &tile, int &chunk) {
+  return tile_chunk_of(blockIdx.x, ntiles, nchunks, tile, chunk);
 }
 
 // FP32 contraction: thread = (bin, 8x8 micro-tile of pairs).
@@ -267,22 +304,27 @@ __device__ __forceinline__ uint4 neg_bits16(uint4 v) {
   return v;
 }
 
+// a / b point at K group 0 of one (bin, station) in the split layout; K
+// group g sits gstride = 8*nsta elements further.  K step s of lane group
+// lane_g is K group 4s + lane_g.
 template <int NK32>
-__device__ __forceinline__ void load_src(const unsigned short *a, int lane_g, SrcFrag<NK32> &o) {
+__device__ __forceinline__ void load_src(const unsigned short *a, size_t gstride, int lane_g,
+                                         SrcFrag<NK32> &o) {
 #pragma unroll
-  for (int s = 0; s < NK32; ++s) o.a32[s] = *(const uint4 *)(a + 32 * s + 8 * lane_g);
+  for (int s = 0; s < NK32; ++s) o.a32[s] = *(const uint4 *)(a + (4 * s + lane_g) * gstride);
 }
 
 template <int NK32>
-__device__ __forceinline__ void load_rec(const unsigned short *b, int lane_g, RecFrag<NK32> &o) {
-  constexpr int kh = 16 * NK32;
+__device__ __forceinline__ void load_rec(const unsigned short *b, size_t gstride, int lane_g,
+                                         RecFrag<NK32> &o) {
+  constexpr int kgh = 2 * NK32;  // K groups per half (kh / 8)
 #pragma unroll
   for (int s = 0; s < NK32; ++s) {
-    const int ea = 32 * s + 8 * lane_g;
-    o.b32[s] = *(const uint4 *)(b + ea);
+    const int g = 4 * s + lane_g;
+    o.b32[s] = *(const uint4 *)(b + g * gstride);
     if constexpr (!RecFrag<NK32>::kDerived) {
-      const bool hi_half = ea >= kh;
-      const uint4 c = *(const uint4 *)(b + (hi_half ? ea - kh : ea + kh));
+      const bool hi_half = g >= kgh;
+      const uint4 c = *(const uint4 *)(b + (hi_half ? g - kgh : g + kgh) * gstride);
       o.c32[s] = hi_half ? neg_bits16(c) : c;
     }
   }
@@ -337,9 +379,93 @@ struct TileOps {
   RecFrag<NK32> rh[2], rl[kSplit ? 2 : 1];
 };
 
-// One workgroup = one 32x32 station tile x 16 bins; 8 waves, each computing
+// Per-lane addressing of one work item (tile x 16-bin chunk).
+struct ItemAddr {
+  size_t s_off[2], r_off[2];  // station offsets (elements) of the lane's rows/columns
+  float col_scale[2];
+  float row_scale[2][4];
+  int k0;
+};
+
+template <int kMode>
+__device__ __forceinline__ void item_addr(const TileDesc &td, int chunk, int nsta, int lane_r,
+                                          int lane_g, float scale, const float *sta_scale,
+                                          ItemAddr &ia) {
+  // Stations past the array end load a valid station; their rows/columns
+  // belong to pairs that are never stored.
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int s_sta = td.src_base + h * 16 + lane_r;
+    const int r_sta = td.rec_base + h * 16 + lane_r;
+    ia.s_off[h] = (size_t)(s_sta < nsta ? s_sta : nsta - 1) * 8;
+    ia.r_off[h] = (size_t)(r_sta < nsta ? r_sta : nsta - 1) * 8;
+    ia.col_scale[h] = scale;
+    if constexpr (kMode == FXC_PREC_FP16) ia.col_scale[h] *= (r_sta < nsta ? sta_scale[r_sta] : 0.f);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      ia.row_scale[h][r] = 1.f;
+      if constexpr (kMode == FXC_PREC_FP16) {
+        const int sa = td.src_base + h * 16 + 4 * lane_g + r;
+        ia.row_scale[h][r] = sa < nsta ? sta_scale[sa] : 0.f;
+      }
+    }
+  }
+  ia.k0 = chunk * kBinChunk;
+}
+
+template <int kMode, int NK32, bool kSplit>
+__device__ __forceinline__ void load_tile_ops(const unsigned short *__restrict__ hi,
+                                              const unsigned short *__restrict__ lo,
+                                              const ItemAddr &ia, int b, int nsta, int nspec,
+                                              int lane_g, TileOps<NK32, kSplit> &o) {
+  constexpr int kp = 32 * NK32;
+  const size_t gstride = (size_t)nsta * 8;
+  int k = ia.k0 + b;
+  k = k < nspec ? k : nspec - 1;  // tail bins compute a valid bin, never stored
+  const size_t kb = (size_t)k * kp * nsta;  // bin k: K' x nsta elements
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    load_src(hi + kb + ia.s_off[h], gstride, lane_g, o.sh[h]);
+    load_rec(hi + kb + ia.r_off[h], gstride, lane_g, o.rh[h]);
+    if constexpr (kSplit) {
+      load_src(lo + kb + ia.s_off[h], gstride, lane_g, o.sl[h]);
+      load_rec(lo + kb + ia.r_off[h], gstride, lane_g, o.rl[h]);
+    }
+  }
+}
+
+template <int kMode, int NK32, bool kSplit>
+__device__ __forceinline__ void compute_tile_bin(const TileOps<NK32, kSplit> &o, const ItemAddr &ia,
+                                                 int b, int lane_r, int lane_g, float2 *stage) {
+#pragma unroll
+  for (int si = 0; si < 2; ++si)
+#pragma unroll
+    for (int sj = 0; sj < 2; ++sj) {
+      f32x4_t cr = {0.f, 0.f, 0.f, 0.f};
+      f32x4_t ci = {0.f, 0.f, 0.f, 0.f};
+      mfma_bin<kMode>(o.sh[si], o.rh[sj], cr, ci);  // hi.hi
+      if constexpr (kSplit) {
+        mfma_bin<kMode>(o.sl[si], o.rh[sj], cr, ci);  // lo.hi
+        mfma_bin<kMode>(o.sh[si], o.rl[sj], cr, ci);  // hi.lo
+      }
+      // D layout: lane holds rows 4*lane_g + r, column lane_r of the 16x16 block.
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = si * 16 + 4 * lane_g + r;
+        const int c = sj * 16 + lane_r;
+        const float sc = ia.col_scale[sj] * ia.row_scale[si][r];
+        stage[stage_index(a * kTile + c, b)] = make_float2(cr[r] * sc, ci[r] * sc);
+      }
+    }
+}
+
+// One work item = one 32x32 station tile x 16 bins; 8 waves, each computing
 // the whole tile (four 16x16 MFMA blocks) for two of the 16 bins, so every
-// station fragment is read once per workgroup.
+// station fragment is read once per workgroup.  Persistent: a workgroup walks
+// items blockIdx.x, +gridDim.x, ...  With small K each wave software-
+// pipelines its bins: the operands of the next bin -- of this item or of the
+// next one -- are loaded before the current bin's MFMAs, so load latency
+// overlaps the MFMAs, the LDS staging and the previous item's stores.
 template <int kMode, int NK32>
 __global__ void __launch_bounds__(512)
 contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short *__restrict__ lo,
@@ -348,96 +474,80 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
                      float2 *__restrict__ xspec, long long xpitch, int ntiles, float scale) {
   extern __shared__ float2 stage[];  // [kTilePairs][kBinChunk], swizzled
   __shared__ int rows[kTilePairs];
-  int tile, chunk;
-  if (!tile_chunk(ntiles, (nspec + kBinChunk - 1) / kBinChunk, tile, chunk)) return;
   const int tid = threadIdx.x;
-  const int *trow = tile_rows + (size_t)tile * kTilePairs;
-  for (int i = tid; i < kTilePairs; i += blockDim.x) rows[i] = trow[i];
-  const TileDesc td = tiles[tile];
   const int wave = tid >> 6;
   const int lane = tid & 63;
   const int lane_r = lane & 15, lane_g = lane >> 4;
-  constexpr int kp = 32 * NK32;
   constexpr bool kSplit = kMode == FXC_PREC_BF16X3;
-  // Stations past the array end load a valid station; their rows/columns
-  // belong to pairs that are never stored.
-  size_t s_off[2], r_off[2];
-  float col_scale[2];
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int s_sta = td.src_base + h * 16 + lane_r;
-    const int r_sta = td.rec_base + h * 16 + lane_r;
-    s_off[h] = (size_t)(s_sta < nsta ? s_sta : nsta - 1) * nspec * kp;
-    r_off[h] = (size_t)(r_sta < nsta ? r_sta : nsta - 1) * nspec * kp;
-    col_scale[h] = scale;
-    if constexpr (kMode == FXC_PREC_FP16) col_scale[h] *= (r_sta < nsta ? sta_scale[r_sta] : 0.f);
-  }
-  float row_scale[2][4];
-#pragma unroll
-  for (int h = 0; h < 2; ++h)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      row_scale[h][r] = 1.f;
-      if constexpr (kMode == FXC_PREC_FP16) {
-        const int sa = td.src_base + h * 16 + 4 * lane_g + r;
-        row_scale[h][r] = sa < nsta ? sta_scale[sa] : 0.f;
-      }
-    }
-  const int k0 = chunk * kBinChunk;
-  auto one_bin = [&](int b) {
-    int k = k0 + b;
-    k = k < nspec ? k : nspec - 1;  // tail bins compute a valid bin, never stored
-    TileOps<NK32, kSplit> o;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const size_t ao = s_off[h] + (size_t)k * kp;
-      const size_t bo = r_off[h] + (size_t)k * kp;
-      load_src(hi + ao, lane_g, o.sh[h]);
-      load_rec(hi + bo, lane_g, o.rh[h]);
-      if constexpr (kSplit) {
-        load_src(lo + ao, lane_g, o.sl[h]);
-        load_rec(lo + bo, lane_g, o.rl[h]);
-      }
-    }
-#pragma unroll
-    for (int si = 0; si < 2; ++si)
-#pragma unroll
-      for (int sj = 0; sj < 2; ++sj) {
-        f32x4_t cr = {0.f, 0.f, 0.f, 0.f};
-        f32x4_t ci = {0.f, 0.f, 0.f, 0.f};
-        mfma_bin<kMode>(o.sh[si], o.rh[sj], cr, ci);  // hi.hi
-        if constexpr (kSplit) {
-          mfma_bin<kMode>(o.sl[si], o.rh[sj], cr, ci);  // lo.hi
-          mfma_bin<kMode>(o.sh[si], o.rl[sj], cr, ci);  // hi.lo
-        }
-        // D layout: lane holds rows 4*lane_g + r, column lane_r of the 16x16 block.
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int a = si * 16 + 4 * lane_g + r;
-          const int c = sj * 16 + lane_r;
-          const float sc = col_scale[sj] * row_scale[si][r];
-          stage[stage_index(a * kTile + c, b)] = make_float2(cr[r] * sc, ci[r] * sc);
-        }
-      }
+  constexpr bool kPipe = NK32 * (kSplit ? 2 : 1) <= 4;
+  constexpr int kBins = kBinChunk / 8;  // bins per wave and item
+  const int nchunks = (nspec + kBinChunk - 1) / kBinChunk;
+  const long long nitems = (long long)ntiles * ((nchunks + kChunkGroup - 1) / kChunkGroup * kChunkGroup);
+  // next valid item at or after `it` (items whose chunk is past the end are skipped)
+  auto next_item = [&](long long it, int &tile, int &chunk) -> long long {
+    for (; it < nitems; it += gridDim.x)
+      if (tile_chunk_of(it, ntiles, nchunks, tile, chunk)) return it;
+    return nitems;
   };
-  // Two bins per wave.  Small K: both bins' loads may be in flight at once;
-  // large K: one bin at a time keeps the operands within the register file.
-  if constexpr (NK32 * (kSplit ? 2 : 1) <= 4) {
-#pragma unroll
-    for (int bi = 0; bi < kBinChunk / 8; ++bi) one_bin(wave + 8 * bi);
-  } else {
-#pragma unroll 1
-    for (int bi = 0; bi < kBinChunk / 8; ++bi) one_bin(wave + 8 * bi);
-  }
-  __syncthreads();
-  // 16 consecutive threads write one pair's 16 bins (128 bytes).
-  const int b = tid & 15;
-  const int k = k0 + b;
-  if (k < nspec) {
-    for (int p = tid >> 4; p < kTilePairs; p += blockDim.x >> 4) {
-      const int row = rows[p];
-      if (row >= 0) store_stream(xspec + (size_t)row * xpitch + k, stage[stage_index(p, b)]);
+  int tile = 0, chunk = 0;
+  long long item = next_item(blockIdx.x, tile, chunk);
+  if (item >= nitems) return;
+  ItemAddr ia;
+  item_addr<kMode>(tiles[tile], chunk, nsta, lane_r, lane_g, scale, sta_scale, ia);
+  TileOps<NK32, kSplit> cur;
+  if constexpr (kPipe) load_tile_ops<kMode>(hi, lo, ia, wave, nsta, nspec, lane_g, cur);
+  // workspace rows of the tile's 1024 pairs, two per thread, loaded one item
+  // ahead so the LDS fill after the barrier does not wait on memory
+  static_assert(kTilePairs == 2 * 512, "two row entries per thread");
+  int rreg0 = tile_rows[(size_t)tile * kTilePairs + tid];
+  int rreg1 = tile_rows[(size_t)tile * kTilePairs + tid + 512];
+  while (item < nitems) {
+    __syncthreads();  // the previous item's store phase is done with rows/stage
+    rows[tid] = rreg0;
+    rows[tid + 512] = rreg1;
+    int ntile = 0, nchunk = 0;
+    const long long nitem = next_item(item + gridDim.x, ntile, nchunk);
+    ItemAddr na = ia;
+    if (nitem < nitems) {
+      item_addr<kMode>(tiles[ntile], nchunk, nsta, lane_r, lane_g, scale, sta_scale, na);
+      rreg0 = tile_rows[(size_t)ntile * kTilePairs + tid];
+      rreg1 = tile_rows[(size_t)ntile * kTilePairs + tid + 512];
     }
+    if constexpr (kPipe) {
+#pragma unroll
+      for (int bi = 0; bi < kBins; ++bi) {
+        TileOps<NK32, kSplit> nxt;
+        if (bi + 1 < kBins)
+          load_tile_ops<kMode>(hi, lo, ia, wave + 8 * (bi + 1), nsta, nspec, lane_g, nxt);
+        else if (nitem < nitems)
+          load_tile_ops<kMode>(hi, lo, na, wave, nsta, nspec, lane_g, nxt);
+        // keep the scheduler from sinking the loads below the MFMAs
+        __builtin_amdgcn_sched_barrier(0);
+        compute_tile_bin<kMode>(cur, ia, wave + 8 * bi, lane_r, lane_g, stage);
+        cur = nxt;
+      }
+    } else {
+#pragma unroll 1
+      for (int bi = 0; bi < kBins; ++bi) {
+        TileOps<NK32, kSplit> o;
+        load_tile_ops<kMode>(hi, lo, ia, wave + 8 * bi, nsta, nspec, lane_g, o);
+        compute_tile_bin<kMode>(o, ia, wave + 8 * bi, lane_r, lane_g, stage);
+      }
+    }
+    __syncthreads();
+    // 16 consecutive threads write one pair's 16 bins (128 bytes).
+    const int b = tid & 15;
+    const int k = ia.k0 + b;
+    if (k < nspec) {
+      for (int p = tid >> 4; p < kTilePairs; p += blockDim.x >> 4) {
+        const int row = rows[p];
+        if (row >= 0) store_stream(xspec + (size_t)row * xpitch + k, stage[stage_index(p, b)]);
+      }
+    }
+    item = nitem;
+    tile = ntile;
+    chunk = nchunk;
+    ia = na;
   }
 }
 
@@ -871,7 +981,7 @@ static int xc_ensure_workspace(fxc_xc_plan *p, size_t rows) {
 static int xc_prepare_spectra(fxc_xc_plan *p, const float2 *d_spec, hipStream_t st) {
   if (p->precision == FXC_PREC_FP32 || p->precision == FXC_PREC_PLAIN) return FXC_OK;
   FXC_TRY(xc_mark(p, kTimeSplit, true, st));
-  const dim3 grid((unsigned)ceil_div(p->nspec, 256), (unsigned)p->nsta);
+  const dim3 grid((unsigned)ceil_div(p->nspec, kSplitBins), (unsigned)ceil_div(p->nsta, kSplitSta));
   if (p->precision == FXC_PREC_FP16) {
     FXC_HIP(hipMemsetAsync(p->d_amax, 0, p->nsta * sizeof(unsigned int), st));
     hipLaunchKernelGGL(station_absmax_kernel, dim3(8, p->nsta), dim3(256), 0, st, d_spec,
@@ -895,7 +1005,14 @@ static int launch_mfma_kh(dim3 grid, size_t lds, hipStream_t st, const unsigned 
                           const TileDesc *tiles, const int *trows, float2 *dst, long long xpitch,
                           float scale) {
   auto kern = contract_mfma_kernel<kMode, KH / 16>;
-  const dim3 grid1(grid.x);
+  // persistent grid: a multiple of 8 workgroups (one per CU, LDS-bound),
+  // capped by the number of work items
+  int dev = 0, ncu = 0;
+  FXC_HIP(hipGetDevice(&dev));
+  FXC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  unsigned g = (unsigned)std::max(8, ncu / 8 * 8);
+  if (const char *env = getenv("FXC_XC_MFMA_GRID")) g = (unsigned)std::max(8, atoi(env) / 8 * 8);
+  const dim3 grid1(std::min(g, grid.x));
   FXC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
   hipLaunchKernelGGL(kern, grid1, dim3(512), lds, st, hi, lo, nsta, nspec, sc, tiles, trows, dst,
