@@ -1,0 +1,174 @@
+// Workgroup FFT with a compile-time plan (device code + host table builder).
+//
+// The runtime-plan passes of fxc_fft_device.hpp spend a large share of their
+// VALU work on index arithmetic (magic-number divisions by runtime strides)
+// and on the two-level twiddle lookup.  For the one length the correlation
+// hot path runs at production size, the plan is a template argument instead:
+// strides, divisions and LDS offsets fold into constants.  The first pass
+// (the one with the most distinct twiddles) reads w^(j q), q = 1..R-1,
+// ready-made from an L2-resident global table; later passes read w^j from a
+// small LDS table and chain its powers.
+#pragma once
+
+#include "fxc_fft_device.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace fxc {
+
+// DIF plan: radices in pass order; pass p works on spans of span(p) values
+// with stride(p) = span(p) / R_p between the legs of a butterfly.
+template <int M, int... Rs>
+struct FixedPlan {
+  static constexpr int kM = M;
+  static constexpr int kPasses = (int)sizeof...(Rs);
+  static constexpr int kRadix[kPasses] = {Rs...};
+  static constexpr int span(int p) {
+    int s = M;
+    for (int i = 0; i < p; ++i) s /= kRadix[i];
+    return s;
+  }
+  static constexpr int stride(int p) { return span(p) / kRadix[p]; }
+  // twiddle row of one butterfly: R-1 values padded to an even count so
+  // every row starts on a 16-byte boundary
+  static constexpr int tw_pitch(int p) { return (kRadix[p] - 1 + 1) / 2 * 2; }
+  static constexpr int tw_offset(int p) {
+    int o = 0;
+    for (int i = 0; i < p; ++i) o += stride(i) * tw_pitch(i);
+    return o;
+  }
+  static constexpr bool valid() {
+    int prod = 1;
+    for (int i = 0; i < kPasses; ++i) prod *= kRadix[i];
+    return prod == M;
+  }
+};
+
+// Host: the twiddle table of passes 0..npass-1 for transform sign `sign`:
+// row j of pass p holds exp(sign*2*pi*i*j*q/span(p)), q = 1..R-1.
+template <class FP>
+std::vector<float2> fixed_twiddles(int npass, int sign) {
+  std::vector<float2> t((size_t)FP::tw_offset(npass), make_float2(0.f, 0.f));
+  for (int p = 0; p < npass; ++p) {
+    const int R = FP::kRadix[p], S = FP::span(p), s = FP::stride(p);
+    for (int j = 0; j < s; ++j)
+      for (int q = 1; q < R; ++q) {
+        const long long e = ((long long)j * q) % S;
+        const double a = sign * 2.0 * 3.14159265358979323846 * (double)e / (double)S;
+        t[(size_t)FP::tw_offset(p) + (size_t)j * FP::tw_pitch(p) + (q - 1)] =
+            make_float2((float)std::cos(a), (float)std::sin(a));
+      }
+  }
+  return t;
+}
+
+// Base twiddles of passes 1..P-2 for LDS: w_S^j = exp(sign*2*pi*i*j/S),
+// j < stride(p), S = span(p), pass after pass.
+template <class FP>
+constexpr int fixed_lds_tw_offset(int p) {
+  int o = 0;
+  for (int i = 1; i < p; ++i) o += FP::stride(i);
+  return o;
+}
+template <class FP>
+constexpr int fixed_lds_tw_count() {
+  return fixed_lds_tw_offset<FP>(FP::kPasses - 1);
+}
+template <class FP>
+std::vector<float2> fixed_lds_twiddles(int sign) {
+  std::vector<float2> t((size_t)fixed_lds_tw_count<FP>());
+  for (int p = 1; p < FP::kPasses - 1; ++p)
+    for (int j = 0; j < FP::stride(p); ++j) {
+      const double a = sign * 2.0 * 3.14159265358979323846 * (double)j / (double)FP::span(p);
+      t[(size_t)fixed_lds_tw_offset<FP>(p) + j] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+  return t;
+}
+
+// One in-place DIF pass of a single transform held in LDS by kThreads threads.
+// kLdsTw = false: the R-1 twiddles of a butterfly come ready-made from the
+// global table twg; true: w^j is read from the LDS base table twl (see
+// fixed_lds_twiddles) and its powers are chained.
+template <class FP, int p, int kThreads, bool kLdsTw>
+__device__ __forceinline__ void fixed_dif_pass(float2 *buf, const float2 *__restrict__ twg,
+                                               const float2 *twl, float sgn) {
+  constexpr int R = FP::kRadix[p];
+  constexpr int S = FP::span(p);
+  constexpr int s = FP::stride(p);
+  constexpr int nbt = FP::kM / R;
+  constexpr int pitch = FP::tw_pitch(p);
+  const float2 *__restrict__ tp = twg + FP::tw_offset(p);
+  const float2 *tl = twl + fixed_lds_tw_offset<FP>(p);
+  for (int b = threadIdx.x; b < nbt; b += kThreads) {
+    const int blk = b / s;
+    const int j = b - blk * s;
+    float2 *base = buf + blk * S + j;
+    float2 w[R - 1];
+    if constexpr (kLdsTw) {
+      w[0] = tl[j];
+#pragma unroll
+      for (int q = 1; q < R - 1; ++q) w[q] = cmul(w[q - 1], w[0]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < R - 1; ++q) w[q] = tp[j * pitch + q];
+    }
+    float2 v[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) v[m] = base[m * s];
+    dft<R>(v, sgn);
+#pragma unroll
+    for (int q = 1; q < R; ++q) v[q] = cmul(v[q], w[q - 1]);
+#pragma unroll
+    for (int q = 0; q < R; ++q) base[q * s] = v[q];
+  }
+  __syncthreads();
+}
+
+// Passes p..P-2 with LDS twiddles (the last pass is left to
+// fixed_last_pass_bin).
+template <class FP, int p, int kThreads>
+__device__ __forceinline__ void fixed_dif_passes_lds(float2 *buf, const float2 *twl, float sgn) {
+  if constexpr (p < FP::kPasses - 1) {
+    fixed_dif_pass<FP, p, kThreads, true>(buf, nullptr, twl, sgn);
+    fixed_dif_passes_lds<FP, p + 1, kThreads>(buf, twl, sgn);
+  }
+}
+
+// Bin k after fixed_fft_head: the direct R-point DFT (R = last radix) of the
+// block that holds it, Z[k] = sum_m buf[base + m] w_R^(m t) with
+// t = k div (M/R) and base the digit-reversed position of k mod (M/R).
+// roots[u] = exp(sgn*2*pi*i*u/R); the powers are chained and re-read from
+// roots every 8 terms.
+template <class FP>
+__device__ __forceinline__ float2 fixed_last_pass_bin(const float2 *buf,
+                                                      const float2 *__restrict__ roots, int k) {
+  constexpr int P = FP::kPasses;
+  constexpr int R = FP::kRadix[P - 1];
+  constexpr int g = FP::kM / R;
+  const int t = k / g;
+  int lower = k - t * g;
+  int base = 0;
+#pragma unroll
+  for (int p = 0; p < P - 1; ++p) {
+    const int q = lower / FP::kRadix[p];
+    base += (lower - q * FP::kRadix[p]) * FP::stride(p);
+    lower = q;
+  }
+  const float2 wt = roots[t];
+  float2 z = buf[base];
+  float2 wm = wt;
+  int e = t;
+#pragma unroll
+  for (int m = 1; m < R; ++m) {
+    const float2 v = buf[base + m];
+    z.x += v.x * wm.x - v.y * wm.y;
+    z.y += v.x * wm.y + v.y * wm.x;
+    e += t;
+    if (e >= R) e -= R;
+    if (m + 1 < R) wm = ((m + 1) & 7) == 0 ? roots[e] : cmul(wm, wt);
+  }
+  return z;
+}
+
+}  // namespace fxc

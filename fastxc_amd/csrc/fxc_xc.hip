@@ -22,6 +22,7 @@
 // The cross-spectra go to an HBM workspace in [pair][bin] rows; the lag
 // kernel folds each row into an 18522-point complex inverse FFT held wholly
 // in LDS and writes only the 2H+1 lags.
+#include "fxc_fft_fixed.hpp"
 #include "fxc_plan_cache.hpp"
 #include "fastxc_amd.h"
 
@@ -640,6 +641,80 @@ lag_kernel(const float2 *__restrict__ xspec, long long xpitch, int nspec, long l
   }
 }
 
+// Lag synthesis at the production length nfft = 37044 (3600 s windows at
+// 0.1 s): the same steps as lag_kernel on a compile-time plan of the
+// 18522-point inverse FFT, 6 x 7 x 7 x 7 x 9, whose last (radix-9, stride-1)
+// pass is evaluated only for the kept lags.  Pass 0 reads its twiddles from
+// a global table; the next row is prefetched only after that pass, because
+// vector-memory counters retire in order and an earlier prefetch would make
+// every twiddle read wait for it.  Passes 1-3 take their twiddles from LDS.
+// The fold table exp(+2*pi*i*k/nfft), k <= M/2, and the 9th roots of unity
+// are global.
+using LagPlan18522 = FixedPlan<18522, 6, 7, 7, 7, 9>;
+static_assert(LagPlan18522::valid(), "18522 = 6*7*7*7*9");
+
+template <class FP>
+__global__ void __launch_bounds__(kLagThreads)
+lag_kernel_fixed(const float2 *__restrict__ xspec, long long xpitch, long long nrows,
+                 const int *__restrict__ row_out, float *__restrict__ cc, int half_cc,
+                 double weight, const float2 *__restrict__ twg, const float2 *__restrict__ twl_g,
+                 const float2 *__restrict__ foldg, const float2 *__restrict__ roots) {
+  extern __shared__ float2 lds[];  // [row: M+1][base twiddles of passes 1..P-2]
+  constexpr int M = FP::kM;
+  constexpr int nspec = M + 1;
+  static_assert(nspec <= kLagRegs * kLagThreads, "prefetch registers cover one row");
+  float2 *twl = lds + nspec;
+  for (int i = threadIdx.x; i < fixed_lds_tw_count<FP>(); i += kLagThreads) twl[i] = twl_g[i];
+  float2 pre[kLagRegs];
+  long long row = blockIdx.x;
+  if (row < nrows) lag_prefetch(pre, xspec + row * xpitch, nspec);
+  for (; row < nrows; row += gridDim.x) {
+#pragma unroll
+    for (int i = 0; i < kLagRegs; ++i) {
+      const int k = threadIdx.x + i * kLagThreads;
+      if (k < nspec) lds[k] = pre[i];
+    }
+    __syncthreads();
+    // fold pairs (k, M-k): Z[k] = (X[k] + conj X[M-k]) + i w_k (X[k] - conj X[M-k]),
+    // w_{M-k} = -conj(w_k)
+    for (int k = threadIdx.x; 2 * k <= M; k += kLagThreads) {
+      if (k == 0) {
+        const float a = lds[0].x, b = lds[M].x;  // imaginary parts of DC/Nyquist ignored
+        lds[0] = make_float2(a + b, a - b);
+      } else {
+        const float2 w = foldg[k];
+        const float2 a = lds[k];
+        const float2 b = lds[M - k];
+        const float2 s1 = cadd(a, cconj(b));
+        const float2 d1 = cmul(w, csub(a, cconj(b)));
+        const float2 zk = make_float2(s1.x - d1.y, s1.y + d1.x);
+        if (2 * k != M) {
+          const float2 s2 = cadd(b, cconj(a));
+          const float2 d2 = cmul(make_float2(-w.x, w.y), csub(b, cconj(a)));
+          lds[M - k] = make_float2(s2.x - d2.y, s2.y + d2.x);
+        }
+        lds[k] = zk;
+      }
+    }
+    __syncthreads();
+    fixed_dif_pass<FP, 0, kLagThreads, false>(lds, twg, nullptr, 1.f);
+    const long long nxt = row + gridDim.x;
+    if (nxt < nrows) lag_prefetch(pre, xspec + nxt * xpitch, nspec);
+    fixed_dif_passes_lds<FP, 1, kLagThreads>(lds, twl, 1.f);
+    const int out_row = row_out ? row_out[row] : (int)row;
+    float *y = cc + (size_t)out_row * (2 * half_cc + 1);
+    const int n_lo = -((half_cc + 1) / 2), n_hi = half_cc / 2;
+    for (int i = threadIdx.x; i <= n_hi - n_lo; i += kLagThreads) {
+      const int n = n_lo + i;
+      const float2 z = fixed_last_pass_bin<FP>(lds, roots, n < 0 ? n + M : n);
+      const int m0 = 2 * n + half_cc;
+      if (m0 >= 0) y[m0] = (float)((double)z.x * weight);
+      if (m0 + 1 <= 2 * half_cc) y[m0 + 1] = (float)((double)z.y * weight);
+    }
+    __syncthreads();
+  }
+}
+
 // PLAIN form, one launch per window w (the reference's accumulation order):
 //   stack[r][k] += (conj(S[w][src_r][k]) * S[w][rec_r][k]) * scale
 // Work items are (row, 256-bin block) pairs, nrows * nbk of them, visited
@@ -690,6 +765,8 @@ struct fxc_xc_plan {
   int precision = FXC_PREC_FP32;
   int kh = 0;
   int lag_max_radix = 8;           // largest butterfly of the lag FFT plan
+  // tables of the compile-time lag plan (nfft/2 == LagPlan18522::kM only)
+  float2 *d_fix_tw = nullptr, *d_fix_twl = nullptr, *d_fix_fold = nullptr, *d_fix_roots = nullptr;
   long long chunk_rows_max = 0;  // cross-spectrum rows per workspace fill
   // pairs
   int nsta = 0;
@@ -787,6 +864,34 @@ extern "C" int fxc_xc_plan_create(fxc_xc_plan **out, int nspec, int nstep, float
   p->kh = nstep <= 16 ? 16 : (nstep + 31) / 32 * 32;
   p->chunk_rows_max = 0;
   p->lag_max_radix = lag_max_radix;
+  const char *fixed_env = getenv("FXC_XC_LAG_FIXED");
+  if (nfft / 2 == LagPlan18522::kM && !(fixed_env && atoi(fixed_env) == 0)) {
+    using FP = LagPlan18522;
+    const std::vector<float2> tw = fixed_twiddles<FP>(1, 1);
+    const std::vector<float2> twl = fixed_lds_twiddles<FP>(1);
+    std::vector<float2> fold(FP::kM / 2 + 1), roots(FP::kRadix[FP::kPasses - 1]);
+    for (size_t k = 0; k < fold.size(); ++k) {
+      const double a = 2.0 * M_PI * (double)k / (double)nfft;
+      fold[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    for (size_t u = 0; u < roots.size(); ++u) {
+      const double a = 2.0 * M_PI * (double)u / (double)roots.size();
+      roots[u] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    auto upload = [](float2 **dst, const std::vector<float2> &v) -> int {
+      FXC_TRY(dmalloc((void **)dst, v.size() * sizeof(float2)));
+      FXC_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(float2), hipMemcpyHostToDevice));
+      return FXC_OK;
+    };
+    int rc = upload(&p->d_fix_tw, tw);
+    if (rc == FXC_OK) rc = upload(&p->d_fix_twl, twl);
+    if (rc == FXC_OK) rc = upload(&p->d_fix_fold, fold);
+    if (rc == FXC_OK) rc = upload(&p->d_fix_roots, roots);
+    if (rc != FXC_OK) {
+      fxc_xc_plan_destroy(p);
+      return rc;
+    }
+  }
   *out = p;
   return FXC_OK;
 }
@@ -840,6 +945,10 @@ extern "C" int fxc_xc_plan_destroy(fxc_xc_plan *p) {
   dfree(p->d_amax);
   dfree(p->d_inv_scale);
   dfree(p->d_scale);
+  dfree(p->d_fix_tw);
+  dfree(p->d_fix_twl);
+  dfree(p->d_fix_fold);
+  dfree(p->d_fix_roots);
   delete p;
   return FXC_OK;
 }
@@ -1099,9 +1208,20 @@ static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long
   // reference weight: 1.0 / (width * dt) with width converted to float
   const double weight = 1.0 / (double)((float)p->nfft * p->dt);
   FXC_TRY(xc_mark(p, kTimeLag, true, st));
-  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kLagThreads), lds, st, xs, xpitch,
-                     p->nspec, nrows, row_out, d_cc, p->half_cc, weight, rp->half.dev,
-                     rp->half.d_tw, rp->d_fold);
+  if (p->d_fix_tw) {
+    using FP = LagPlan18522;
+    const size_t flds = (size_t)(FP::kM + 1 + fixed_lds_tw_count<FP>()) * sizeof(float2);
+    auto fk = lag_kernel_fixed<FP>;
+    FXC_HIP(hipFuncSetAttribute((const void *)fk, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)flds));
+    hipLaunchKernelGGL(fk, dim3((unsigned)grid), dim3(kLagThreads), flds, st, xs, xpitch, nrows,
+                       row_out, d_cc, p->half_cc, weight, p->d_fix_tw, p->d_fix_twl,
+                       p->d_fix_fold, p->d_fix_roots);
+  } else {
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kLagThreads), lds, st, xs, xpitch,
+                       p->nspec, nrows, row_out, d_cc, p->half_cc, weight, rp->half.dev,
+                       rp->half.d_tw, rp->d_fold);
+  }
   FXC_HIP(hipGetLastError());
   FXC_TRY(xc_mark(p, kTimeLag, false, st));
   return FXC_OK;
