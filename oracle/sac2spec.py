@@ -28,8 +28,13 @@ Pinning (tests/test_oracle_sac2spec.py):
                      built, and against the closed-form config-0/2 numbers;
   |H|^2              against scipy.signal.freqz of the same coefficients;
   preprocess         a ramp plus offset detrends to zero; taper closed form;
+  prefilter          unit response is the identity on the window; a
+                     1 + cos(2 pi k d / n) response equals the brute-force
+                     circular convolution with delta[0] + (delta[+-d])/2;
   smooth_rows        brute force on small rows (edges, even/odd windows);
   runabs / onebit    constant input normalises to 1 (x/(W*|x|/W)) and sign;
+  runabs_mf          one unit-response band reduces to runabs at that band's
+                     corner; two identical bands give twice that;
   freq_whiten        flat |X| whitens to the taper; brute-force smooth;
   output_spectrum    numpy.fft.rfft * dt.
 """

@@ -17,7 +17,10 @@ Pinning (tests/test_oracle_stack.py): identical traces give |W| = 1 so
 pws = lin; identical groups give tfpws = 0 (the reference's zero Gaussian at
 c = 0 removes every voice's time sum); a cosine's analytic signal is
 exp(i w t); the FFT S-transform equals a brute-force DFT evaluation on small
-N; linear_stack equals numpy's float64 mean.
+N; the band blend with a band covering every voice equals plain tfpws and
+with a band above Nyquist returns the analytic linear stack's real part;
+group means/sums and weights against hand-computed values; linear_stack
+equals numpy's float64 mean.
 """
 from __future__ import annotations
 

@@ -128,3 +128,48 @@ def test_cpu_xc_restatement_matches_oracle():
     got = cbind.cpu_xc_correlate(spec, src, rec, H, dt, nthreads=2)
     want = xc.correlate(spec, src, rec, nfft, H, dt)
     assert xc.rel_l2(got, want) < 1e-5
+
+
+def _unit_plan(seg=600, nfft_filter=1200, nbands=2, delta=0.1):
+    """A plan whose every power response is 1 (identity filters)."""
+    return s2s.Plan(npts=seg, delta=delta, seg=seg, shift=seg, nstep=1, nfft_out=seg,
+                    nspec=seg // 2 + 1, nfft_filter=nfft_filter, freq_low=0.2, freq_high=2.0,
+                    idx=(0, 1, 2, 3), responses=[np.ones(nfft_filter)] * nbands,
+                    freq_lows=[0.2] + [0.25] * (nbands - 1))
+
+
+def test_prefilter_with_unit_response_is_identity():
+    # zero padding to nfft_filter, x dt, |H|^2 = 1, inverse / (nfft dt), keep
+    # the first seg samples: the identity on the window
+    p = _unit_plan()
+    x = np.random.default_rng(9).standard_normal((2, 1, p.seg))
+    assert np.allclose(s2s.prefilter(x, p), x, atol=1e-12)
+
+
+def test_prefilter_is_circular_convolution_with_response_kernel():
+    # |H|^2[k] = 1 + cos(2 pi k d / n) is the spectrum of the kernel
+    # delta[0] + (delta[d] + delta[-d]) / 2, so on the zero-padded window the
+    # prefilter is y[t] = x[t] + (xp[t-d] + xp[t+d]) / 2 (indices mod n)
+    p = _unit_plan()
+    n, d = p.nfft_filter, 37
+    k = np.arange(n)
+    resp = 1.0 + np.cos(2 * np.pi * k * d / n)
+    p.responses = [resp, resp]
+    x = np.random.default_rng(11).standard_normal(p.seg)
+    xp = np.zeros(n)
+    xp[: p.seg] = x
+    t = np.arange(p.seg)
+    want = x + 0.5 * (xp[(t - d) % n] + xp[(t + d) % n])
+    got = s2s.prefilter(x[None, None], p)[0, 0]
+    assert np.allclose(got, want, atol=1e-12)
+
+
+def test_runabs_mf_single_unit_band_equals_runabs():
+    # one extra band with |H|^2 = 1: the band signal is the window itself, so
+    # RUN-ABS-MF reduces to RUN-ABS at that band's low corner (no 0.667 factor)
+    p = _unit_plan(nbands=2)
+    x = np.random.default_rng(10).standard_normal((3, 2, p.seg))
+    assert np.allclose(s2s.runabs_mf(x, p), s2s.runabs(x, 0.25, p.delta), atol=1e-12)
+    # two identical bands add up to twice that
+    p3 = _unit_plan(nbands=3)
+    assert np.allclose(s2s.runabs_mf(x, p3), 2 * s2s.runabs(x, 0.25, p.delta), atol=1e-12)

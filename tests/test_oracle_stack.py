@@ -68,3 +68,22 @@ def test_tfpws_identical_groups_vanish():
     # the band-limited blend keeps the analytic linear stack outside the band
     outb = stack.tfpws(s / 2, w, lin, band=(0.2, 0.3, 0.01), delta=1.0)
     assert np.max(np.abs(outb)) > 0.1 * np.max(np.abs(lin))
+
+
+def test_tfpws_band_blend_limits():
+    # a band that covers every voice with no taper blends with alpha = 1
+    # (plain TF-PWS); a band above Nyquist keeps the analytic linear stack,
+    # whose inverse is the linear stack's analytic signal real part: lin
+    # minus its mean (the DC bin is dropped by the analytic mask)
+    rng = np.random.default_rng(12)
+    recs = rng.standard_normal((8, 45))
+    s, w, lin = stack.tfpws_prepare(recs, 2)
+    plain = stack.tfpws(s, w, lin)
+    full = stack.tfpws(s, w, lin, band=(0.0, 0.5, 0.0), delta=1.0)
+    assert np.allclose(full, plain, atol=1e-12)
+    none = stack.tfpws(s, w, lin, band=(10.0, 20.0, 0.1), delta=1.0)
+    N = lin.shape[-1]
+    A = stack.analytic_spectrum(lin[None])[0]
+    want = (np.fft.ifft(A) * N).real / N
+    assert np.allclose(none, want, atol=1e-12)
+    assert np.allclose(want, lin - lin.mean(), atol=1e-9) or N % 2 == 0
