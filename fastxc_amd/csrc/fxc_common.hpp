@@ -11,11 +11,7 @@
 #include <cstdint>
 #include <string>
 
-#define FXC_OK 0
-#define FXC_ERR_ARG 1
-#define FXC_ERR_HIP 2
-#define FXC_ERR_ALLOC 3
-#define FXC_ERR_UNSUPPORTED 4
+#include "fastxc_amd.h"  // status codes FXC_OK, FXC_ERR_*
 
 namespace fxc {
 

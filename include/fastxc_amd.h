@@ -15,6 +15,13 @@
 #ifndef FASTXC_AMD_H
 #define FASTXC_AMD_H
 
+/* Status codes returned by every entry point. */
+#define FXC_OK 0
+#define FXC_ERR_ARG 1         /* invalid argument or shape */
+#define FXC_ERR_HIP 2         /* a HIP runtime call failed */
+#define FXC_ERR_ALLOC 3       /* device allocation failed */
+#define FXC_ERR_UNSUPPORTED 4 /* valid but not supported (length, window count) */
+
 #ifdef __cplusplus
 extern "C" {
 #endif
