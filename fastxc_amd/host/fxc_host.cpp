@@ -260,6 +260,20 @@ std::vector<std::string> list_files_with_suffix(const std::string &dir, const ch
   return out;
 }
 
+bool dir_has_entries(const std::string &dir) {
+  DIR *d = opendir(dir.c_str());
+  if (!d) return false;
+  bool any = false;
+  while (struct dirent *e = readdir(d)) {
+    if (strcmp(e->d_name, ".") && strcmp(e->d_name, "..")) {
+      any = true;
+      break;
+    }
+  }
+  closedir(d);
+  return any;
+}
+
 int write_marker(const std::string &path) {
   FILE *fp = fopen(path.c_str(), "w");
   if (!fp) return -1;

@@ -72,6 +72,7 @@ std::string path_join(const std::string &a, const std::string &b);
 std::string absolute_path(const std::string &p);
 std::vector<std::string> list_files_with_suffix(const std::string &dir, const char *suffix);
 int write_marker(const std::string &path);  // "DONE\n"
+bool dir_has_entries(const std::string &dir);
 int pread_exact(int fd, size_t offset, void *dst, size_t bytes);
 
 // ------------------------------------------------------------------- SAC --

@@ -176,3 +176,35 @@ def read_xcpack_dir(xcpack_dir):
             data = np.frombuffer(blob, dtype=np.float32, offset=632)
             out.append((row, hd, data))
     return out
+
+
+def sac_null_bytes() -> bytearray:
+    """The reference's null SAC header: -12345 everywhere, nvhdr 6, text "-12345"."""
+    b = bytearray(632)
+    struct.pack_into("<70f", b, 0, *([-12345.0] * 70))
+    ints = [-12345] * 40
+    ints[6] = 6
+    struct.pack_into("<40i", b, 280, *ints)
+    off = 440
+    for _name, size in SAC_TEXT:
+        b[off:off + size] = (b"-12345" + b" " * (size - 6))
+        off += size
+    return b
+
+
+def write_sac(path, data, delta, stla=0.0, stlo=0.0, kstnm="", knetwk="", kcmpnm=""):
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    h = sac_null_bytes()
+    struct.pack_into("<f", h, 0, delta)                                   # delta
+    struct.pack_into("<ff", h, 5 * 4, 0.0, (len(data) - 1) * delta)      # b, e
+    struct.pack_into("<ff", h, SAC_FLOATS.index("stla") * 4, stla, stlo)
+    struct.pack_into("<i", h, 280 + 4 * SAC_INTS.index("npts"), len(data))
+    struct.pack_into("<i", h, 280 + 4 * SAC_INTS.index("iftype"), 1)
+    struct.pack_into("<i", h, 280 + 4 * SAC_INTS.index("leven"), 1)
+    for name, value in (("kstnm", kstnm), ("knetwk", knetwk), ("kcmpnm", kcmpnm)):
+        off = 440 + sum(s for n, s in SAC_TEXT[:[n for n, _ in SAC_TEXT].index(name)])
+        size = dict(SAC_TEXT)[name]
+        h[off:off + size] = value.encode()[:size].ljust(size, b" ")
+    with open(path, "wb") as f:
+        f.write(bytes(h))
+        f.write(data.tobytes())
