@@ -13,7 +13,10 @@
 // (kernels/cuda.stransform.cu:59-332).
 //
 // MI355X design: the correlation length (2001 = 3*23*29) is transformed in
-// LDS by 256-thread workgroups with register butterflies up to radix 29.
+// LDS by 256-thread workgroups with register butterflies up to radix 29;
+// lengths with other prime factors (2H+1 is arbitrary) use Bluestein's
+// convolution of a factorable length m >= 2n-1, also in LDS, so every
+// length up to about 5800 (direct: about 7000) is served.
 // The analytic spectra of all groups are computed once; the TF-PWS voice
 // kernel keeps the modulated transforms, the weight W_f(tau) and the
 // Gaussian in LDS, so per voice only the (group-count + 1) spectra are read
@@ -30,37 +33,106 @@
 namespace fxc {
 
 constexpr int kStkThreads = 256;
-constexpr int kStkBatch = 4;       // transforms held in LDS at once
+constexpr int kStkBatch = 4;       // transforms held in LDS at once (at most)
 constexpr int kVoiceChunk = 32;    // TF-PWS voices per workgroup
+constexpr int kLdsReserve = 16;    // float2 slots left for static LDS
 
-__device__ __forceinline__ void stage_tw(float2 *dst, const FftPlanDev &plan,
-                                         const float2 *__restrict__ tw) {
-  for (int i = threadIdx.x; i < kTwFine + plan.ncoarse; i += blockDim.x) dst[i] = tw[i];
+// Length-n transform of the stacks: the direct mixed-radix plan when n
+// factors into the implemented radices (rows of n values, digit-reversed
+// output), else Bluestein's convolution of length m (rows of m values,
+// natural-order output).
+struct StackFftDev {
+  FftPlanDev fwd;              // length ld, sign -1
+  FftPlanDev inv;              // length ld, sign +1
+  int n, ld, blue;
+  const float2 *chirp;         // [n] exp(-i pi j^2 / n) (Bluestein only)
+  const float2 *bhat_f;        // [m] forward-sign convolution kernel spectrum
+  const float2 *bhat_i;        // [m] inverse-sign
+};
+
+struct StackFft {
+  StackFftDev dev{};
+  const float2 *tw_f = nullptr, *tw_i = nullptr;
+  int twc = 0;                 // LDS twiddle slots of both tables
+};
+
+__device__ __forceinline__ int stack_pos(const StackFftDev &F, int k) {
+  return F.blue ? k : fft_pos(F.fwd, k);
+}
+
+// twl: [fwd table | inv table]
+__device__ __forceinline__ void stage_stack_tw(float2 *twl, const StackFftDev &F,
+                                               const float2 *__restrict__ tw_f,
+                                               const float2 *__restrict__ tw_i) {
+  const int cf = kTwFine + F.fwd.ncoarse, ci = kTwFine + F.inv.ncoarse;
+  for (int i = threadIdx.x; i < cf + ci; i += blockDim.x) twl[i] = i < cf ? tw_f[i] : tw_i[i - cf];
+}
+
+// In-place transforms (sgn -1 forward, +1 inverse) of nt rows at stride
+// F.ld whose input is natural order in [0, n).  The caller synchronises
+// before the call; the call ends with a barrier.  Output position of bin k:
+// stack_pos(F, k).
+__device__ void stack_fft(float2 *buf, int nt, const StackFftDev &F, const float2 *twl, float sgn) {
+  const float2 *twf = twl;
+  const float2 *twi = twl + kTwFine + F.fwd.ncoarse;
+  if (!F.blue) {
+    if (sgn < 0.f)
+      lds_fft<kRadixAll>(buf, nt, F.fwd, twf, -1.f);
+    else
+      lds_fft<kRadixAll>(buf, nt, F.inv, twi, 1.f);
+    return;
+  }
+  const int n = F.n, m = F.ld, total = nt * m;
+  const float cs = sgn < 0.f ? 1.f : -1.f;  // c_s = c_{-1} or its conjugate
+  const float2 *bh = sgn < 0.f ? F.bhat_f : F.bhat_i;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const int j = i % m;
+    float2 v = make_float2(0.f, 0.f);
+    if (j < n) {
+      const float2 c = F.chirp[j];
+      v = cmul(buf[i], make_float2(c.x, cs * c.y));
+    }
+    buf[i] = v;
+  }
+  __syncthreads();
+  lds_fft<kRadixAll>(buf, nt, F.fwd, twf, -1.f);
+  for (int i = threadIdx.x; i < total; i += blockDim.x) buf[i] = cmul(buf[i], bh[i % m]);
+  __syncthreads();
+  lds_fft_dit<kRadixAll>(buf, nt, F.inv, twi, 1.f);
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const int j = i % m;
+    if (j < n) {
+      const float2 c = F.chirp[j];
+      buf[i] = cmul(buf[i], make_float2(c.x, cs * c.y));
+    }
+  }
+  __syncthreads();
 }
 
 // Analytic spectra of real rows: A[k] = 2 X[k] for 1 <= k < (n+1)/2, else 0
 // (hilbertTransformKernel after R2C).  Output natural order, [rows][n].
 __global__ void __launch_bounds__(kStkThreads)
-analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, FftPlanDev plan,
-                const float2 *__restrict__ tw, float2 *__restrict__ A) {
+analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, StackFftDev F,
+                const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i,
+                float2 *__restrict__ A) {
   extern __shared__ float2 lds[];
-  const int n = plan.n;
-  float2 *twl = lds + (size_t)per_wg * n;
-  stage_tw(twl, plan, tw);
+  const int n = F.n, ld = F.ld;
+  float2 *twl = lds + (size_t)per_wg * ld;
+  stage_stack_tw(twl, F, tw_f, tw_i);
   const long long r0 = (long long)blockIdx.x * per_wg;
   const int nt = (int)std::min<long long>(per_wg, rows - r0);
   for (int i = threadIdx.x; i < nt * n; i += blockDim.x) {
     const int t = i / n, j = i - t * n;
-    lds[i] = make_float2(x[(r0 + t) * n + j], 0.f);
+    lds[t * ld + j] = make_float2(x[(r0 + t) * n + j], 0.f);
   }
   __syncthreads();
-  lds_fft<kRadixAll>(lds, nt, plan, twl, -1.f);
+  stack_fft(lds, nt, F, twl, -1.f);
   const int half = (n + 1) / 2;
   for (int i = threadIdx.x; i < nt * n; i += blockDim.x) {
     const int t = i / n, k = i - t * n;
     float2 v = make_float2(0.f, 0.f);
     if (k >= 1 && k < half) {
-      const float2 z = lds[t * n + fft_pos(plan, k)];
+      const float2 z = lds[t * ld + stack_pos(F, k)];
       v = make_float2(2.f * z.x, 2.f * z.y);
     }
     A[(r0 + t) * n + k] = v;
@@ -70,28 +142,32 @@ analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, FftPlan
 // PWS: one workgroup per pair.
 __global__ void __launch_bounds__(kStkThreads)
 pws_kernel(const float2 *__restrict__ A, const float *__restrict__ weights,
-           const long long *__restrict__ offsets, const float *__restrict__ lin,
-           FftPlanDev plan, const float2 *__restrict__ tw_inv, float *__restrict__ out) {
+           const long long *__restrict__ offsets, const float *__restrict__ lin, StackFftDev F,
+           const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i, int nbatch,
+           float *__restrict__ out) {
   extern __shared__ float2 lds[];
-  const int n = plan.n;
-  float2 *W = lds;                       // [n], digit-reversed positions
-  float2 *buf = lds + n;                 // [kStkBatch][n]
-  float2 *twl = buf + (size_t)kStkBatch * n;
-  stage_tw(twl, plan, tw_inv);
+  const int n = F.n, ld = F.ld;
+  float2 *W = lds;                       // [n], transform output positions
+  float2 *buf = lds + n;                 // [nbatch][ld]
+  float2 *twl = buf + (size_t)nbatch * ld;
+  stage_stack_tw(twl, F, tw_f, tw_i);
   for (int i = threadIdx.x; i < n; i += blockDim.x) W[i] = make_float2(0.f, 0.f);
   const int pair = blockIdx.x;
   const long long g0 = offsets[pair], g1 = offsets[pair + 1];
   const float invn = 1.f / (float)n;
-  for (long long gb = g0; gb < g1; gb += kStkBatch) {
-    const int nb = (int)std::min<long long>(kStkBatch, g1 - gb);
+  for (long long gb = g0; gb < g1; gb += nbatch) {
+    const int nb = (int)std::min<long long>(nbatch, g1 - gb);
     __syncthreads();
-    for (int i = threadIdx.x; i < nb * n; i += blockDim.x) buf[i] = A[gb * n + i];
+    for (int i = threadIdx.x; i < nb * n; i += blockDim.x) {
+      const int b = i / n, c = i - b * n;
+      buf[b * ld + c] = A[(gb + b) * n + c];
+    }
     __syncthreads();
-    lds_fft<kRadixAll>(buf, nb, plan, twl, 1.f);
+    stack_fft(buf, nb, F, twl, 1.f);
     for (int p = threadIdx.x; p < n; p += blockDim.x) {
       float2 acc = W[p];
       for (int b = 0; b < nb; ++b) {
-        const float2 z = buf[b * n + p];
+        const float2 z = buf[b * ld + p];
         const float re = z.x * invn, im = z.y * invn;
         float mod = sqrtf(re * re + im * im);
         mod = mod > 1e-7f ? mod : 1e-7f;
@@ -106,7 +182,7 @@ pws_kernel(const float2 *__restrict__ A, const float *__restrict__ weights,
   const float *l = lin + (size_t)pair * n;
   float *o = out + (size_t)pair * n;
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
-    const float2 w = W[fft_pos(plan, t)];
+    const float2 w = W[stack_pos(F, t)];
     o[t] = l[t] * sqrtf(w.x * w.x + w.y * w.y);
   }
 }
@@ -116,19 +192,20 @@ pws_kernel(const float2 *__restrict__ A, const float *__restrict__ weights,
 __global__ void __launch_bounds__(kStkThreads)
 tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Alin,
                    const float *__restrict__ weights, const long long *__restrict__ offsets,
-                   FftPlanDev plan, const float2 *__restrict__ tw_inv, int nfreq,
-                   const unsigned char *__restrict__ chunk_active, float2 *__restrict__ out_spec) {
+                   StackFftDev F, const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i,
+                   int nbatch, int nfreq, const unsigned char *__restrict__ chunk_active,
+                   float2 *__restrict__ out_spec) {
   extern __shared__ float2 lds[];
-  const int n = plan.n;
+  const int n = F.n, ld = F.ld;
   float2 *W = lds;                                   // [n]
   float *gauss = (float *)(lds + n);                 // [n] floats (n/2 complex slots)
-  float2 *buf = lds + n + (n + 1) / 2;               // [kStkBatch][n]
-  float2 *twl = buf + (size_t)kStkBatch * n;
+  float2 *buf = lds + n + (n + 1) / 2;               // [nbatch][ld]
+  float2 *twl = buf + (size_t)nbatch * ld;
   __shared__ float red_x[kStkThreads / 64], red_y[kStkThreads / 64];
-  stage_tw(twl, plan, tw_inv);
   const int pair = blockIdx.y;
   const int chunk = blockIdx.x;
   if (chunk_active && !chunk_active[chunk]) return;  // uniform per workgroup
+  stage_stack_tw(twl, F, tw_f, tw_i);
   const int f0 = chunk * kVoiceChunk;
   const long long g0 = offsets[pair], g1 = offsets[pair + 1];
   const float2 *Al = Alin + (size_t)pair * n;
@@ -151,21 +228,21 @@ tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Ali
       gauss[c] = g;
       W[c] = make_float2(0.f, 0.f);
     }
-    for (long long gb = g0; gb < g1; gb += kStkBatch) {
-      const int nb = (int)std::min<long long>(kStkBatch, g1 - gb);
+    for (long long gb = g0; gb < g1; gb += nbatch) {
+      const int nb = (int)std::min<long long>(nbatch, g1 - gb);
       __syncthreads();
       for (int i = threadIdx.x; i < nb * n; i += blockDim.x) {
         const int b = i / n, c = i - b * n;
         int src = f + c;
         src = src >= n ? src - n : src;
-        buf[i] = cscale(Ag[(gb + b) * n + src], gauss[c]);
+        buf[b * ld + c] = cscale(Ag[(gb + b) * n + src], gauss[c]);
       }
       __syncthreads();
-      lds_fft<kRadixAll>(buf, nb, plan, twl, 1.f);
+      stack_fft(buf, nb, F, twl, 1.f);
       for (int p = threadIdx.x; p < n; p += blockDim.x) {
         float2 acc = W[p];
         for (int b = 0; b < nb; ++b) {
-          const float2 z = buf[b * n + p];
+          const float2 z = buf[b * ld + p];
           float fa = sqrtf(z.x * z.x + z.y * z.y);
           if (fa <= 1e-7f) fa = 1.f;
           const float wg = weights[gb + b];
@@ -182,7 +259,7 @@ tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Ali
       buf[c] = cscale(Al[src], gauss[c]);
     }
     __syncthreads();
-    lds_fft<kRadixAll>(buf, 1, plan, twl, 1.f);
+    stack_fft(buf, 1, F, twl, 1.f);
     float sx = 0.f, sy = 0.f;
     for (int p = threadIdx.x; p < n; p += blockDim.x) {
       const float2 w = W[p];
@@ -214,12 +291,13 @@ tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Ali
 // spectrum in band mode), zero above nfreq, inverse transform, Re / n.
 __global__ void __launch_bounds__(kStkThreads)
 tfpws_final_kernel(const float2 *__restrict__ out_spec, const float2 *__restrict__ Alin, int nfreq,
-                   int band_limited, float df, float fmin, float fmax, float taper,
-                   FftPlanDev plan, const float2 *__restrict__ tw_inv, float *__restrict__ y) {
+                   int band_limited, float df, float fmin, float fmax, float taper, StackFftDev F,
+                   const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i,
+                   float *__restrict__ y) {
   extern __shared__ float2 lds[];
-  const int n = plan.n;
-  float2 *twl = lds + n;
-  stage_tw(twl, plan, tw_inv);
+  const int n = F.n;
+  float2 *twl = lds + F.ld;
+  stage_stack_tw(twl, F, tw_f, tw_i);
   const int pair = blockIdx.x;
   const float2 *os = out_spec + (size_t)pair * n;
   const float2 *Al = Alin + (size_t)pair * n;
@@ -250,9 +328,9 @@ tfpws_final_kernel(const float2 *__restrict__ out_spec, const float2 *__restrict
     lds[f] = v;
   }
   __syncthreads();
-  lds_fft<kRadixAll>(lds, 1, plan, twl, 1.f);
+  stack_fft(lds, 1, F, twl, 1.f);
   float *o = y + (size_t)pair * n;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) o[t] = lds[fft_pos(plan, t)].x / (float)n;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) o[t] = lds[stack_pos(F, t)].x / (float)n;
 }
 
 struct StackWork {
@@ -268,29 +346,63 @@ struct StackWork {
   }
 };
 
+// Direct plan when n factors, else the Bluestein plan.
+static int stack_fft_plan(int n, StackFft *S) {
+  int radix[kMaxPasses], np = 0;
+  const FftPlan *pf = nullptr, *pi = nullptr;
+  StackFftDev &d = S->dev;
+  if (factor_radices(n, radix, &np)) {
+    FXC_TRY(plan_cache_complex(n, -1, &pf));
+    FXC_TRY(plan_cache_complex(n, 1, &pi));
+    d.blue = 0;
+    d.ld = n;
+    d.chirp = d.bhat_f = d.bhat_i = nullptr;
+  } else {
+    const BluesteinPlan *bp = nullptr;
+    FXC_TRY(plan_cache_bluestein(n, &bp));
+    pf = bp->fwd;
+    pi = bp->inv;
+    d.blue = 1;
+    d.ld = bp->m;
+    d.chirp = bp->d_chirp;
+    d.bhat_f = bp->d_bhat_f;
+    d.bhat_i = bp->d_bhat_i;
+  }
+  d.n = n;
+  d.fwd = pf->dev;
+  d.inv = pi->dev;
+  S->tw_f = pf->d_tw;
+  S->tw_i = pi->d_tw;
+  S->twc = 2 * tw_lds_count(d.ld);
+  return FXC_OK;
+}
+
+// Rows of `ld` the LDS holds beside `fixed` other slots (0 when none fit).
+static int stack_batch(const StackFft &S, size_t fixed) {
+  const long long room = (long long)kLdsComplexCapacity - kLdsReserve - (long long)fixed - S.twc;
+  return room < S.dev.ld ? 0 : (int)std::min<long long>(kStkBatch, room / S.dev.ld);
+}
+
 static int stack_common(const float *d_pre, const float *d_lin, const long long *h_off, int npairs,
-                        int n, StackWork &w, const FftPlan **pinv, hipStream_t st) {
-  const FftPlan *pf = nullptr;
-  FXC_TRY(plan_cache_complex(n, -1, &pf));
-  FXC_TRY(plan_cache_complex(n, 1, pinv));
+                        int n, StackWork &w, const StackFft &S, hipStream_t st) {
   const long long ng = h_off[npairs];
   const long long rows = ng + npairs;
   FXC_TRY(dmalloc((void **)&w.A, (size_t)rows * n * sizeof(float2)));
   FXC_TRY(dmalloc((void **)&w.offsets, (size_t)(npairs + 1) * sizeof(long long)));
   FXC_HIP(hipMemcpyAsync(w.offsets, h_off, (size_t)(npairs + 1) * sizeof(long long),
                          hipMemcpyHostToDevice, st));
-  const int per = std::max(1, std::min(8, (kLdsComplexCapacity - tw_lds_count(n)) / n));
-  const size_t lds = ((size_t)per * n + tw_lds_count(n)) * sizeof(float2);
+  const int per = std::max(1, std::min(8, (kLdsComplexCapacity - S.twc) / S.dev.ld));
+  const size_t lds = ((size_t)per * S.dev.ld + S.twc) * sizeof(float2);
   FXC_HIP(hipFuncSetAttribute((const void *)analytic_kernel,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (ng > 0) {
     hipLaunchKernelGGL(analytic_kernel, dim3((unsigned)ceil_div((size_t)ng, (size_t)per)),
-                       dim3(kStkThreads), lds, st, d_pre, ng, per, pf->dev, pf->d_tw, w.A);
+                       dim3(kStkThreads), lds, st, d_pre, ng, per, S.dev, S.tw_f, S.tw_i, w.A);
     FXC_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(analytic_kernel, dim3((unsigned)ceil_div((size_t)npairs, (size_t)per)),
-                     dim3(kStkThreads), lds, st, d_lin, (long long)npairs, per, pf->dev, pf->d_tw,
-                     w.A + (size_t)ng * n);
+                     dim3(kStkThreads), lds, st, d_lin, (long long)npairs, per, S.dev, S.tw_f,
+                     S.tw_i, w.A + (size_t)ng * n);
   FXC_HIP(hipGetLastError());
   return FXC_OK;
 }
@@ -302,11 +414,6 @@ static int check_stack_args(const float *pre, const float *wts, const long long 
   FXC_CHECK_ARG(off[0] == 0, "stack: group offsets must start at 0");
   for (int i = 0; i < npairs; ++i)
     FXC_CHECK_ARG(off[i + 1] > off[i], "stack: pair %d has no groups", i);
-  int radix[kMaxPasses], np = 0;
-  FXC_CHECK_ARG(factor_radices(n, radix, &np), "stack: length %d has no supported factorisation", n);
-  FXC_CHECK_ARG((size_t)n * (1 + kStkBatch) + (n + 1) / 2 + tw_lds_count(n) <=
-                    (size_t)kLdsComplexCapacity,
-                "stack: length %d too long for the LDS transforms", n);
   return FXC_OK;
 }
 
@@ -317,15 +424,18 @@ using namespace fxc;
 extern "C" int fxc_pws(const float *d_prestack, const float *d_weights, const long long *h_offsets,
                        const float *d_linear, int npairs, int n, float *d_out, void *stream) {
   FXC_TRY(check_stack_args(d_prestack, d_weights, h_offsets, d_linear, npairs, n, d_out));
+  StackFft S;
+  FXC_TRY(stack_fft_plan(n, &S));
+  const int nbatch = stack_batch(S, (size_t)n);
+  FXC_CHECK_ARG(nbatch >= 1, "fxc_pws: length %d too long for the LDS transforms", n);
   hipStream_t st = (hipStream_t)stream;
   StackWork w;
-  const FftPlan *pinv = nullptr;
-  FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, &pinv, st));
-  const size_t lds = ((size_t)n * (1 + kStkBatch) + tw_lds_count(n)) * sizeof(float2);
+  FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, S, st));
+  const size_t lds = ((size_t)n + (size_t)nbatch * S.dev.ld + S.twc) * sizeof(float2);
   FXC_HIP(hipFuncSetAttribute((const void *)pws_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
   hipLaunchKernelGGL(pws_kernel, dim3((unsigned)npairs), dim3(kStkThreads), lds, st, w.A, d_weights,
-                     w.offsets, d_linear, pinv->dev, pinv->d_tw, d_out);
+                     w.offsets, d_linear, S.dev, S.tw_f, S.tw_i, nbatch, d_out);
   FXC_HIP(hipGetLastError());
   FXC_HIP(hipStreamSynchronize(st));  // workspace is released on return
   return FXC_OK;
@@ -337,10 +447,13 @@ extern "C" int fxc_tfpws(const float *d_prestack, const float *d_weights,
                          float *d_out, void *stream) {
   FXC_TRY(check_stack_args(d_prestack, d_weights, h_offsets, d_linear, npairs, n, d_out));
   FXC_CHECK_ARG(!band_limited || (delta > 0.f && fmax > fmin), "fxc_tfpws: invalid band");
+  StackFft S;
+  FXC_TRY(stack_fft_plan(n, &S));
+  const int nbatch = stack_batch(S, (size_t)n + (size_t)(n + 1) / 2);
+  FXC_CHECK_ARG(nbatch >= 1, "fxc_tfpws: length %d too long for the LDS transforms", n);
   hipStream_t st = (hipStream_t)stream;
   StackWork w;
-  const FftPlan *pinv = nullptr;
-  FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, &pinv, st));
+  FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, S, st));
   const int nfreq = n / 2 + 1;
   const int nchunks = (nfreq + kVoiceChunk - 1) / kVoiceChunk;
   const float df = band_limited ? (float)(1.0 / ((double)n * delta)) : 0.f;
@@ -358,19 +471,19 @@ extern "C" int fxc_tfpws(const float *d_prestack, const float *d_weights,
     FXC_HIP(hipMemcpyAsync(w.active, act.data(), nchunks, hipMemcpyHostToDevice, st));
   }
   const long long ng = h_offsets[npairs];
-  size_t lds = ((size_t)n * (1 + kStkBatch) + (n + 1) / 2 + tw_lds_count(n)) * sizeof(float2);
+  size_t lds = ((size_t)n + (n + 1) / 2 + (size_t)nbatch * S.dev.ld + S.twc) * sizeof(float2);
   FXC_HIP(hipFuncSetAttribute((const void *)tfpws_voice_kernel,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(tfpws_voice_kernel, dim3((unsigned)nchunks, (unsigned)npairs),
                      dim3(kStkThreads), lds, st, w.A, w.A + (size_t)ng * n, d_weights, w.offsets,
-                     pinv->dev, pinv->d_tw, nfreq, (const unsigned char *)w.active, w.spec);
+                     S.dev, S.tw_f, S.tw_i, nbatch, nfreq, (const unsigned char *)w.active, w.spec);
   FXC_HIP(hipGetLastError());
-  lds = ((size_t)n + tw_lds_count(n)) * sizeof(float2);
+  lds = ((size_t)S.dev.ld + S.twc) * sizeof(float2);
   FXC_HIP(hipFuncSetAttribute((const void *)tfpws_final_kernel,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(tfpws_final_kernel, dim3((unsigned)npairs), dim3(kStkThreads), lds, st, w.spec,
-                     w.A + (size_t)ng * n, nfreq, band_limited, df, fmin, fmax, taper_hz, pinv->dev,
-                     pinv->d_tw, d_out);
+                     w.A + (size_t)ng * n, nfreq, band_limited, df, fmin, fmax, taper_hz, S.dev,
+                     S.tw_f, S.tw_i, d_out);
   FXC_HIP(hipGetLastError());
   FXC_HIP(hipStreamSynchronize(st));  // workspace is released on return
   return FXC_OK;

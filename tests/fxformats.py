@@ -7,6 +7,8 @@ inputs, readers of outputs).  Test infrastructure only.
                    :275-330 (TSV columns and row format); records
                    [SAC header 632 B][npts float32] (native/xc/io/sac_record.cu)
   allowed_paths    fastxc/inventory/planner.py:368-397 (9 columns)
+  SourcePack       native/pws/io/sourcepack_io.cu (25-column index, records
+                   [SAC header][npts float32] at record_offset of record_path)
   SAC header       70 float32, 40 int32, 192 bytes of text
 """
 from __future__ import annotations
@@ -192,11 +194,12 @@ def sac_null_bytes() -> bytearray:
     return b
 
 
-def write_sac(path, data, delta, stla=0.0, stlo=0.0, kstnm="", knetwk="", kcmpnm=""):
+def sac_bytes(data, delta, stla=0.0, stlo=0.0, kstnm="", knetwk="", kcmpnm="", b=0.0):
+    """One SAC record: 632-byte header then float32 samples."""
     data = np.ascontiguousarray(data, dtype=np.float32)
     h = sac_null_bytes()
     struct.pack_into("<f", h, 0, delta)                                   # delta
-    struct.pack_into("<ff", h, 5 * 4, 0.0, (len(data) - 1) * delta)      # b, e
+    struct.pack_into("<ff", h, 5 * 4, b, b + (len(data) - 1) * delta)    # b, e
     struct.pack_into("<ff", h, SAC_FLOATS.index("stla") * 4, stla, stlo)
     struct.pack_into("<i", h, 280 + 4 * SAC_INTS.index("npts"), len(data))
     struct.pack_into("<i", h, 280 + 4 * SAC_INTS.index("iftype"), 1)
@@ -205,6 +208,31 @@ def write_sac(path, data, delta, stla=0.0, stlo=0.0, kstnm="", knetwk="", kcmpnm
         off = 440 + sum(s for n, s in SAC_TEXT[:[n for n, _ in SAC_TEXT].index(name)])
         size = dict(SAC_TEXT)[name]
         h[off:off + size] = value.encode()[:size].ljust(size, b" ")
+    return bytes(h) + data.tobytes()
+
+
+def write_sac(path, data, delta, stla=0.0, stlo=0.0, kstnm="", knetwk="", kcmpnm=""):
     with open(path, "wb") as f:
-        f.write(bytes(h))
-        f.write(data.tobytes())
+        f.write(sac_bytes(data, delta, stla, stlo, kstnm, knetwk, kcmpnm))
+
+
+SOURCEPACK_COLUMNS = [
+    "timestamp", "path_id", "component_slot", "source_key", "receiver_key", "src_id", "rec_id",
+    "src_network", "src_station", "src_location", "src_component", "rec_network", "rec_station",
+    "rec_location", "rec_component", "npts", "dt", "dist", "az", "baz", "record_path",
+    "record_offset", "bytes", "storage_kind", "final_pair_path"]
+
+
+def read_sourcepack_index(path):
+    """Rows of a sourcepack_index.tsv as dicts, with each record's header and samples."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    assert lines[0].split("\t") == SOURCEPACK_COLUMNS, lines[0]
+    out = []
+    for line in lines[1:]:
+        row = dict(zip(SOURCEPACK_COLUMNS, line.split("\t")))
+        with open(row["record_path"], "rb") as pf:
+            pf.seek(int(row["record_offset"]))
+            blob = pf.read(int(row["bytes"]))
+        out.append((row, parse_sac_header(blob), np.frombuffer(blob, dtype=np.float32, offset=632)))
+    return out

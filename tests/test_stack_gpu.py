@@ -1,6 +1,9 @@
 """PWS and TF-PWS on the GPU vs the FP64 oracle (oracle/stack.py).
 
-Inputs follow the reference host preparation in float32
+Lengths cover the direct mixed-radix transform (87 = 3*29, 345, 2001 =
+3*23*29) and Bluestein's convolution for lengths with other prime factors
+(201 = 3*67, 401 prime, 1001 = 7*11*13), as the correlation length 2H+1 is
+arbitrary.  Inputs follow the reference host preparation in float32
 (native/pws/io/sourcepack_io.cu:385-478, native/tfpws/io/sourcepack_io.cu:403-490);
 the oracle prepares the same records in FP64.  Tolerance: relative L2
 1e-5 for PWS, 1e-4 for TF-PWS (a sum over n/2+1 voices of FP32 inverse
@@ -56,7 +59,7 @@ def _batch(rng, npairs, T, N, B, kind):
     return recs, np.concatenate(pre), np.concatenate(w), np.stack(lin), np.array(off, np.int64)
 
 
-@pytest.mark.parametrize("N", [87, 345, 2001])
+@pytest.mark.parametrize("N", [87, 345, 2001, 201, 401, 1001])
 def test_pws_matches_oracle(gpu, N):
     import torch
     from fastxc_amd import ops
@@ -73,7 +76,8 @@ def test_pws_matches_oracle(gpu, N):
         assert err < 1e-5
 
 
-@pytest.mark.parametrize("N,band", [(87, None), (345, None), (2001, None), (345, (0.1, 0.3, 0.02))])
+@pytest.mark.parametrize("N,band", [(87, None), (345, None), (2001, None), (345, (0.1, 0.3, 0.02)),
+                                    (201, None), (1001, None), (401, (0.1, 0.3, 0.02))])
 def test_tfpws_matches_oracle(gpu, N, band):
     import torch
     from fastxc_amd import ops
@@ -101,3 +105,18 @@ def test_tfpws_identical_groups_vanish(gpu):
     d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(gpu)
     out = ops.tfpws(d(pre / 2), d(w), np.array([0, 3]), d(lin[None])).cpu().numpy()
     assert np.max(np.abs(out)) < 1e-4 * np.max(np.abs(lin))
+
+
+@pytest.mark.parametrize("kind", ["pws", "tfpws"])
+def test_stack_rejects_length_beyond_lds(gpu, kind):
+    """A length whose transforms do not fit one workgroup's LDS is an argument
+    error before any launch (no truncated or faulting run)."""
+    import torch
+    from fastxc_amd import ops
+    from fastxc_amd._lib import FxcError
+
+    n = 9001 if kind == "tfpws" else 15013
+    z = torch.zeros((2, n), dtype=torch.float32, device=gpu)
+    w = torch.full((2,), 0.5, dtype=torch.float32, device=gpu)
+    with pytest.raises(FxcError, match="too long"):
+        getattr(ops, kind)(z, w, np.array([0, 2]), z[:1].contiguous())
