@@ -60,6 +60,7 @@ _SIGNATURES = {
     "fxc_pws": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "fxc_tfpws": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_float,
                           c_float, c_float, c_float, c_void_p, c_void_p]),
+    "fxc_linear_stack": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "fxc_s2s_plan_create": (c_int, [POINTER(c_void_p), c_void_p]),
     "fxc_s2s_plan_destroy": (c_int, [c_void_p]),
     "fxc_s2s_plan_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int),

@@ -333,6 +333,25 @@ tfpws_final_kernel(const float2 *__restrict__ out_spec, const float2 *__restrict
   for (int t = threadIdx.x; t < n; t += blockDim.x) o[t] = lds[stack_pos(F, t)].x / (float)n;
 }
 
+// Linear stack of every pair: out[p][j] = (float)((sum_r (double)x[r][j]) / count_p)
+// over rows r = off[p] .. off[p+1]-1 in row order (a float64 running sum,
+// then a float64 division and one rounding, as _linear_stack_group does).
+// One thread per sample; a wave reads 64 consecutive samples of a row.
+constexpr int kLinThreads = 256;
+
+__global__ void __launch_bounds__(kLinThreads)
+linear_stack_kernel(const float *__restrict__ x, const long long *__restrict__ off, int n,
+                    float *__restrict__ out) {
+  const int j = blockIdx.x * kLinThreads + threadIdx.x;
+  const int pair = blockIdx.y;
+  if (j >= n) return;
+  const long long r0 = off[pair], r1 = off[pair + 1];
+  double acc = 0.0;
+  const float *p = x + (size_t)r0 * n + j;
+  for (long long r = r0; r < r1; ++r, p += n) acc += (double)*p;
+  out[(size_t)pair * n + j] = (float)(acc / (double)(r1 - r0));
+}
+
 struct StackWork {
   float2 *A = nullptr;      // [groups + pairs][n]
   long long *offsets = nullptr;
@@ -484,6 +503,27 @@ extern "C" int fxc_tfpws(const float *d_prestack, const float *d_weights,
   hipLaunchKernelGGL(tfpws_final_kernel, dim3((unsigned)npairs), dim3(kStkThreads), lds, st, w.spec,
                      w.A + (size_t)ng * n, nfreq, band_limited, df, fmin, fmax, taper_hz, S.dev,
                      S.tw_f, S.tw_i, d_out);
+  FXC_HIP(hipGetLastError());
+  FXC_HIP(hipStreamSynchronize(st));  // workspace is released on return
+  return FXC_OK;
+}
+
+extern "C" int fxc_linear_stack(const float *d_records, const long long *h_offsets, int npairs,
+                                int n, float *d_out, void *stream) {
+  FXC_CHECK_ARG(d_records && h_offsets && d_out && npairs >= 1 && n >= 1,
+                "fxc_linear_stack: invalid arguments");
+  FXC_CHECK_ARG(h_offsets[0] == 0, "fxc_linear_stack: offsets must start at 0");
+  for (int i = 0; i < npairs; ++i)
+    FXC_CHECK_ARG(h_offsets[i + 1] > h_offsets[i], "fxc_linear_stack: pair %d has no records", i);
+  FXC_CHECK_ARG(npairs <= 65535, "fxc_linear_stack: at most 65535 pairs per call");
+  hipStream_t st = (hipStream_t)stream;
+  StackWork w;
+  FXC_TRY(dmalloc((void **)&w.offsets, (size_t)(npairs + 1) * sizeof(long long)));
+  FXC_HIP(hipMemcpyAsync(w.offsets, h_offsets, (size_t)(npairs + 1) * sizeof(long long),
+                         hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(linear_stack_kernel,
+                     dim3((unsigned)ceil_div((size_t)n, (size_t)kLinThreads), (unsigned)npairs),
+                     dim3(kLinThreads), 0, st, d_records, w.offsets, n, d_out);
   FXC_HIP(hipGetLastError());
   FXC_HIP(hipStreamSynchronize(st));  // workspace is released on return
   return FXC_OK;

@@ -258,3 +258,22 @@ def tfpws(prestack, weights, offsets, linear, band=None, delta=1.0, out=None):
                           float(fmax), float(taper), out.data_ptr(), _stream_ptr(linear.device)),
           "fxc_tfpws")
     return out
+
+
+def linear_stack(records, offsets, out=None):
+    """Linear stack of every pair: float64 running sum of its rows (row order)
+    divided by the row count, rounded once to float32.  records [offsets[-1], n]."""
+    _require(records, torch.float32, "records")
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    npairs = off.shape[0] - 1
+    if records.ndim != 2 or records.shape[0] != off[-1] or npairs < 1:
+        raise ValueError("records must be [offsets[-1], n]")
+    n = records.shape[1]
+    if out is None:
+        out = torch.empty((npairs, n), dtype=torch.float32, device=records.device)
+    _require(out, torch.float32, "out")
+    if tuple(out.shape) != (npairs, n):
+        raise ValueError("out must be [npairs, n]")
+    check(lib().fxc_linear_stack(records.data_ptr(), off.ctypes.data, npairs, n, out.data_ptr(),
+                                 _stream_ptr(records.device)), "fxc_linear_stack")
+    return out

@@ -148,6 +148,15 @@ int fxc_tfpws(const float *d_prestack, const float *d_weights, const long long *
               const float *d_linear, int npairs, int n, int band_limited, float delta, float fmin,
               float fmax, float taper_hz, float *d_out, void *stream);
 
+/* Linear stack: d_out[p][j] = (float)(sum of (double)d_records[r][j] over
+ * rows r = h_offsets[p] .. h_offsets[p+1]-1, divided in double by their
+ * count).  Rows of a pair are summed in row order, so the result equals the
+ * float64 running sum and division of _linear_stack_group
+ * (fastxc/operators/stacking/linear_stack.py:228-247) bit for bit.
+ * d_records [h_offsets[npairs]][n] float; npairs <= 65535 per call. */
+int fxc_linear_stack(const float *d_records, const long long *h_offsets, int npairs, int n,
+                     float *d_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,6 +1,7 @@
 """FP64 oracle of the stacks -- test infrastructure only.
 
-  linear_stack      fastxc/operators/stacking/linear_stack.py:228-247 (float64 mean)
+  linear_stack      fastxc/operators/stacking/linear_stack.py:228-247 (float64 running
+                    sum in record order / count, rounded once to float32)
   pws_prepare       prepare_sourcepack_host_item, native/pws/io/sourcepack_io.cu:385-478
   analytic          R2C + hilbertTransformKernel, native/pws/worker/gpu.cu:195-205,
                     native/pws/kernels/cuda.pws.cu:3-26 (and the tfpws copy,
@@ -20,7 +21,8 @@ exp(i w t); the FFT S-transform equals a brute-force DFT evaluation on small
 N; the band blend with a band covering every voice equals plain tfpws and
 with a band above Nyquist returns the analytic linear stack's real part;
 group means/sums and weights against hand-computed values; linear_stack
-equals numpy's float64 mean.
+equals the correctly rounded mean (math.fsum) on float32 inputs and is
+invariant to a constant offset that exactly cancels.
 """
 from __future__ import annotations
 
@@ -28,8 +30,14 @@ import numpy as np
 
 
 def linear_stack(records: np.ndarray) -> np.ndarray:
-    """float64 mean over timestamps, stored as float32."""
-    return np.asarray(records, dtype=np.float64).mean(axis=0).astype(np.float32)
+    """float64 running sum in record (timestamp) order, float64 division by
+    the count, one rounding to float32 (_linear_stack_group,
+    fastxc/operators/stacking/linear_stack.py:228-247)."""
+    r = np.asarray(records, dtype=np.float32)
+    acc = r[0].astype(np.float64)
+    for row in r[1:]:
+        acc += row.astype(np.float64)
+    return (acc / float(r.shape[0])).astype(np.float32)
 
 
 def _groups(T: int, B: int):

@@ -5,8 +5,19 @@ from oracle import stack
 
 
 def test_linear_stack_is_float64_mean():
+    """Against the correctly rounded mean: float32 inputs summed in float64
+    lose nothing for short stacks, so one rounding of fsum/T must agree."""
+    import math
     r = np.random.default_rng(0).standard_normal((7, 33)).astype(np.float32)
-    assert np.array_equal(stack.linear_stack(r), r.astype(np.float64).mean(0).astype(np.float32))
+    want = np.array([math.fsum(c) / 7.0 for c in r.T.astype(np.float64)]).astype(np.float32)
+    assert np.array_equal(stack.linear_stack(r), want)
+    # a large offset that cancels exactly (the sum stays exact in float64)
+    big = np.vstack([r, np.full((1, 33), 2.0 ** 20, np.float32), np.full((1, 33), -2.0 ** 20,
+                                                                          np.float32)])
+    want9 = np.array([math.fsum(c) / 9.0 for c in big.T.astype(np.float64)]).astype(np.float32)
+    assert np.array_equal(stack.linear_stack(big), want9)
+    # row order matters only through rounding: a single record is returned as is
+    assert np.array_equal(stack.linear_stack(r[:1]), r[0])
 
 
 def test_analytic_signal_of_cosine():
