@@ -9,9 +9,9 @@
 // [SAC header][2H+1 float], parts cut at 4 GiB), out/xcpack/<ts>.done after
 // each timestamp and out/xcpack/_SUCCESS at the end, and the progress sidecar.
 // Extensions: --precision fp32|bf16x3|fp16|plain (default fp32, the
-// reference's arithmetic), --block-files n (block size; the reference
-// derives it from device memory, at most 4096) and --dry-run (print jobs
-// and tasks, no GPU).
+// reference's arithmetic, or the FASTXC_XC_PRECISION environment value),
+// --block-files n (block size; the reference derives it from device memory,
+// at most 4096) and --dry-run (print jobs and tasks, no GPU).
 //
 // Per timestamp: the StepPack payload [nstep][rows][nspec] is read into
 // pinned memory and uploaded once; each job's tasks go to fxc_xc_correlate
@@ -69,7 +69,8 @@ void usage() {
           "  -G <ids>             GPU list; the first id is used (one GPU per process)\n"
           "  -M <MiB,...>         Device memory cap for the first GPU; 0 means auto\n"
           "  -J <num>             Write queue depth (default 1, 0 writes synchronously)\n"
-          "  --precision <p>      fp32 (default) | bf16x3 | fp16 | plain\n"
+          "  --precision <p>      fp32 | bf16x3 | fp16 | plain (default: FASTXC_XC_PRECISION\n"
+          "                       if set, else fp32)\n"
           "  --block-files <n>    Station rows per block (default min(rows, 4096))\n"
           "  --dry-run            Print the jobs and tasks of every timestamp and exit\n"
           "  --debug              Write synchronously\n"
@@ -84,6 +85,10 @@ bool is_option(const char *s) { return s && s[0] == '-' && s[1] != '\0'; }
 
 bool parse_args(int argc, char **argv, Args *a, int *exit_code) {
   *exit_code = 1;
+  // deployment default for the contraction form (the stage command line
+  // built by the Python driver has no precision flag)
+  if (const char *env = getenv("FASTXC_XC_PRECISION"))
+    if (*env) a->precision = env;
   if (argc <= 1) {
     usage();
     return false;

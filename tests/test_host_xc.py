@@ -67,6 +67,17 @@ def test_usage_and_required_arguments(tmp_path):
     assert r.returncode == 1 and "event=argument_unknown" in r.stderr
 
 
+def test_precision_environment_default(tmp_path):
+    env = dict(os.environ, FASTXC_XC_PRECISION="bogus")
+    r = run(["-I", "x", "-P", "y", "-O", str(tmp_path), "-C", "10"], env=env)
+    assert r.returncode == 1 and 'option="--precision" value="bogus"' in r.stderr
+    # the command line wins over the environment
+    sp_dir, paths, *_ = make_inputs(str(tmp_path), nsta=3)
+    r = run(["-I", sp_dir, "-P", paths, "-O", str(tmp_path / "o"), "-C", "5", "--dry-run",
+             "--precision", "bf16x3"], env=env)
+    assert r.returncode == 0, r.stderr
+
+
 @pytest.mark.parametrize("nch,block", [(1, 0), (1, 3), (3, 0), (3, 5)])
 def test_dry_run_task_order_matches_oracle(tmp_path, nch, block):
     sp_dir, paths, nodes, spec, allowed = make_inputs(str(tmp_path), nsta=7, nch=nch)
