@@ -1,6 +1,6 @@
 """Benchmark of the xc_fast hot path on one MI355X.
 
-Workload (the BASELINE precision-form configuration): 1000 stations, Z only,
+Workload (the precision-form configuration of the metric): 1000 stations, Z only,
 one day at dt = 0.1 s cut into 24 one-hour windows, output FFT 37044
 (nspec = 18523 bins), all 499500 station pairs, max_lag 100 s (2001 lags).
 Spectra are synthetic (seeded complex Gaussian, the shape a StepPack payload
@@ -12,6 +12,11 @@ the tiled fp32 contraction, and "plain" -- the reference's own step order
 run on the same GPU.  Unit: pair-windows per second (pairs x windows / step
 time).  The CPU baseline is the multi-threaded C restatement in the oracle,
 timed on a bounded station subset.
+
+The other stages of the path are measured after xc, each at the scale of
+the same 1000-station day, under "stages" in the same JSON line: sac2spec (1000 stations x 24
+windows), PWS and the float64 linear stack (49950 pairs x 30 timestamps),
+TF-PWS (2000 pairs); --no-stages skips them, --stages-only runs only them.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision ...]
 """
@@ -50,6 +55,9 @@ def parse():
     ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
                     help="first run one device-to-device copy of this size (HBM counter calibration)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "xc_hbm_traffic_1000sta.txt"))
+    ap.add_argument("--no-stages", action="store_true",
+                    help="skip the sac2spec / stack stage measurements")
+    ap.add_argument("--stages-only", action="store_true", help="measure the stages, not xc")
     return ap.parse_args()
 
 
@@ -190,6 +198,161 @@ def calibrate_copy(gib):
     torch.cuda.empty_cache()
 
 
+# ----------------------------------------------------------------- stages --
+# The other stages of the path, each at the 1000-station-day scale, timed as whole
+# entry-point calls (allocation of their small workspaces included) between
+# device synchronisations.  Work units and algorithmic bytes/flops:
+#   sac2spec: station-windows; 4*seg in + 8*nspec out bytes each
+#   PWS:      pairs; 4*N*(G+2) bytes each (G group rows, linear stack, output)
+#   TF-PWS:   pairs; 5*N*log2(N) flops per complex transform, (G+1)*(N/2+1)
+#             voice transforms + 1 final transform per pair
+#   linear:   pairs; 4*N*(T+1) bytes each (T records in, one stack out)
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+
+
+def _time_calls(fn, reps):
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def _stage(name, units, unit, sec, work, bound, peak, punit, detail):
+    ach = work / sec / (1e9 if punit == "GB/s" else 1e12)
+    return {"stage": name, "value": round(units / sec, 1), "unit": unit,
+            "ms_per_call": round(sec * 1e3, 3), "detail": detail,
+            "roofline": {"bound": bound, "achieved": round(ach, 2), "peak": peak, "unit": punit,
+                         "frac": round(ach / peak, 4), "traffic": None}}
+
+
+def _butter_power(nfft, delta, lo, hi):
+    """|B/A|^2 of a 2nd-order Butterworth band-pass (scipy design), k = 0..nfft/2."""
+    import numpy as np
+    from scipy.signal import butter
+
+    b, a = butter(2, [lo, hi], btype="bandpass", fs=1.0 / delta)
+    z = np.exp(-1j * np.pi * np.arange(nfft // 2 + 1) / (nfft // 2))
+    return (np.abs(np.polyval(b[::-1], z) / np.polyval(a[::-1], z)) ** 2).astype(np.float32)
+
+
+def stage_sac2spec(dev, reps):
+    """1000 stations x one day at dt 0.1 s: 24 one-hour windows, RUN-ABS,
+    whitening after, band 0.1-2 Hz (seg 36000, filter FFT 36288, output FFT 37044)."""
+    import numpy as np
+    import torch
+    from fastxc_amd.ops import S2sPlan
+
+    nsta, npts, seg, nfft_f, nfft_o, nstep = 1000, 864000, 36000, 36288, 37044, 24
+    resp = _butter_power(nfft_f, 0.1, 0.1, 2.0)[None]
+    plan = S2sPlan(npts, seg, seg, nfft_f, nfft_o, 1, 0.1, 0.1, [240, 360, 7200, 9597], resp,
+                   [0.1], whiten=2, normalize=3)
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    x = torch.randn((nsta, npts), dtype=torch.float32, device=dev, generator=g)
+    out = torch.empty((nstep, nsta, nfft_o // 2 + 1), dtype=torch.complex64, device=dev)
+    steps = np.arange(nstep, dtype=np.int32)
+    sec = _time_calls(lambda: plan.process(x, steps, out), reps)
+    ok = bool(torch.isfinite(torch.view_as_real(out[:, :: 97])).all().item())
+    plan.close()
+    units = nsta * nstep
+    work = units * (4 * seg + 8 * (nfft_o // 2 + 1))
+    del x, out
+    torch.cuda.empty_cache()
+    return _stage("sac2spec", units, "station-windows/s", sec, work, "hbm", HBM_PEAK_GBS, "GB/s",
+                  {"stations": nsta, "windows": nstep, "seg": seg, "nfft_filter": nfft_f,
+                   "nfft_out": nfft_o, "normalize": "RUN-ABS", "whiten": "after",
+                   "output_finite": ok})
+
+
+def _stack_inputs(dev, npairs, T, B, n, kind, seed):
+    import numpy as np
+    import torch
+
+    G = -(-T // B)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    pre = torch.randn((npairs * G, n), dtype=torch.float32, device=dev, generator=g)
+    sizes = [min(B, T - i * B) for i in range(G)]
+    w = torch.tensor([s / T for s in sizes] * npairs, dtype=torch.float32, device=dev)
+    lin = pre.view(npairs, G, n).mean(1).contiguous()
+    off = np.arange(0, npairs * G + 1, G, dtype=np.int64)
+    return pre, w, lin, off, G
+
+
+def stage_pws(dev, reps):
+    """49950 pairs (a tenth of the 1000-station triangle), 30 timestamps in
+    sub-stacks of 10 (pre_stack_size), 2001 lags."""
+    import torch
+    from fastxc_amd import ops
+
+    npairs, T, B, n = 49950, 30, 10, 2001
+    pre, w, lin, off, G = _stack_inputs(dev, npairs, T, B, n, "pws", 11)
+    out = torch.empty_like(lin)
+    sec = _time_calls(lambda: ops.pws(pre, w, off, lin, out=out), reps)
+    res = _stage("pws", npairs, "pairs/s", sec, npairs * 4 * n * (G + 2), "hbm", HBM_PEAK_GBS,
+                 "GB/s", {"pairs": npairs, "timestamps": T, "substack": B, "groups": G, "n": n})
+    del pre, w, lin, out
+    torch.cuda.empty_cache()
+    return res
+
+
+def stage_tfpws(dev, reps):
+    """2000 pairs, 30 timestamps in sub-stacks of 10, 2001 lags, all 1001 voices."""
+    import math
+
+    import torch
+    from fastxc_amd import ops
+
+    npairs, T, B, n = 2000, 30, 10, 2001
+    pre, w, lin, off, G = _stack_inputs(dev, npairs, T, B, n, "tfpws", 13)
+    out = torch.empty_like(lin)
+    sec = _time_calls(lambda: ops.tfpws(pre, w, off, lin, out=out), reps)
+    flops = npairs * ((G + 1) * (n // 2 + 1) + 1) * 5.0 * n * math.log2(n)
+    res = _stage("tfpws", npairs, "pairs/s", sec, flops, "fp32-vector", FP32_VECTOR_PEAK_TFLOPS,
+                 "TFLOP/s", {"pairs": npairs, "timestamps": T, "substack": B, "groups": G, "n": n,
+                             "voices": n // 2 + 1})
+    del pre, w, lin, out
+    torch.cuda.empty_cache()
+    return res
+
+
+def stage_linear(dev, reps):
+    """49950 pairs x 30 timestamps x 2001 lags, float64 running sum."""
+    import numpy as np
+    import torch
+    from fastxc_amd import ops
+
+    npairs, T, n = 49950, 30, 2001
+    g = torch.Generator(device=dev)
+    g.manual_seed(17)
+    x = torch.randn((npairs * T, n), dtype=torch.float32, device=dev, generator=g)
+    off = np.arange(0, npairs * T + 1, T, dtype=np.int64)
+    out = torch.empty((npairs, n), dtype=torch.float32, device=dev)
+    sec = _time_calls(lambda: ops.linear_stack(x, off, out=out), reps)
+    res = _stage("linear_stack", npairs, "pairs/s", sec, npairs * 4 * n * (T + 1), "hbm",
+                 HBM_PEAK_GBS, "GB/s", {"pairs": npairs, "timestamps": T, "n": n,
+                                        "accumulator": "f64"})
+    del x, out
+    torch.cuda.empty_cache()
+    return res
+
+
+def stages(reps=3):
+    import torch
+
+    dev = torch.device("cuda:0")
+    out = []
+    for fn in (stage_sac2spec, stage_pws, stage_tfpws, stage_linear):
+        print(f"# stage {fn.__name__}", file=sys.stderr, flush=True)
+        out.append(fn(dev, reps))
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -201,6 +364,9 @@ def main():
 
     if args.calibrate_copy_gib > 0:
         calibrate_copy(args.calibrate_copy_gib)
+    if args.stages_only:
+        print(json.dumps({"stages": stages()}), flush=True)
+        return
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev)
     g.manual_seed(1234)
@@ -247,6 +413,10 @@ def main():
     for m in modes[1:]:
         res["ratio_vs_" + m["precision"]] = round(head["pair_windows_per_s"] /
                                                    m["pair_windows_per_s"], 3)
+    del spec, out
+    torch.cuda.empty_cache()
+    if not args.no_stages:
+        res["stages"] = stages()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(res), flush=True)

@@ -288,6 +288,93 @@ s2s_weight_kernel(const void *__restrict__ data, long long row_pitch, int width,
   }
 }
 
+// Tiled form of the same weights.  A workgroup owns kWTile consecutive
+// outputs of one frame; per channel it stages v[j] = |x[t0 - win/2 + j]|
+// (0 outside the row), j < kWTile + win - 1, in LDS as doubles (coalesced
+// loads), forms their inclusive prefix S in FP64 and takes every window sum
+// as S[k + win - 1] - S[k - 1].  The prefix is local to the tile, so the
+// difference loses at most ~1e-14 relative -- far below the float rounding
+// of the result -- where the rolling form of s2s_weight_kernel re-reads
+// each thread's window with scattered loads.  Channels are added in float
+// in channel order, as sum2DKernel adds the smoothed rows.
+constexpr int kWTile = 4096;
+constexpr int kWThreads = 256;
+constexpr int kWPerThread = kWTile / kWThreads;
+constexpr int kWTileMaxWin = 12000;  // LDS: (kWTile + win - 1) doubles
+
+template <bool kCplx>
+__global__ void __launch_bounds__(kWThreads)
+s2s_weight_tile_kernel(const void *__restrict__ data, long long row_pitch, int width, int nch,
+                       int win, float *__restrict__ wout, long long w_pitch) {
+  extern __shared__ double S[];
+  __shared__ double wave_tot[kWThreads / 64];
+  const long long frame = blockIdx.x;
+  const int t0 = blockIdx.y * kWTile;
+  const int T = min(kWTile, width - t0);
+  const int L = T + win - 1;
+  const int base = t0 - win / 2;
+  const double inv = 1.0 / (double)win;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int per = (L + kWThreads - 1) / kWThreads;
+  const int a = min(L, tid * per), b = min(L, a + per);
+  float acc[kWPerThread];
+#pragma unroll
+  for (int m = 0; m < kWPerThread; ++m) acc[m] = 0.f;
+  for (int ch = 0; ch < nch; ++ch) {
+    const long long r = frame * nch + ch;
+    __syncthreads();  // the previous channel's reads of S are done
+    for (int j = tid; j < L; j += kWThreads) {
+      const int i = base + j;
+      double v = 0.0;
+      if (i >= 0 && i < width) {
+        if (kCplx) {
+          const float2 z = ((const float2 *)data)[r * row_pitch + i];
+          v = (double)sqrtf(z.x * z.x + z.y * z.y);
+        } else {
+          v = (double)fabsf(((const float *)data)[r * row_pitch + i]);
+        }
+      }
+      S[j] = v;
+    }
+    __syncthreads();
+    double run = 0.0;
+    for (int j = a; j < b; ++j) run += S[j];
+    double incl = run;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
+    }
+    if (lane == 63) wave_tot[wid] = incl;
+    __syncthreads();
+    double pre = incl - run;
+    for (int w = 0; w < wid; ++w) pre += wave_tot[w];
+    for (int j = a; j < b; ++j) {
+      pre += S[j];
+      S[j] = pre;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kWPerThread; ++m) {
+      const int k = tid + m * kWThreads;
+      if (k < T) {
+        const double sum = S[k + win - 1] - (k > 0 ? S[k - 1] : 0.0);
+        acc[m] += (float)(inv * sum);
+      }
+    }
+  }
+  float *w = wout + frame * w_pitch + t0;
+#pragma unroll
+  for (int m = 0; m < kWPerThread; ++m) {
+    const int k = tid + m * kWThreads;
+    if (k < T) {
+      float v = acc[m];
+      if (!isfinite(v) || v < kMinVal) v = kMinVal;
+      w[k] = v;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Whitening forward: load (normalised) row, R2C(seg) x dt, sanitise; write
 // X[k] and |X[k]|, k < seg/2+1 (core/spectrum.cu:178-230 up to the weight).
@@ -679,6 +766,24 @@ static int s2s_ensure(fxc_s2s_plan *p, long long rows, long long frames, int nst
 
 static int launch_weight(bool cplx, const void *data, long long pitch, int width, int nch, int win,
                          long long frames, float *w, long long wpitch, hipStream_t st) {
+  if (win >= 1 && win <= kWTileMaxWin) {
+    const unsigned tiles = (unsigned)ceil_div((size_t)width, (size_t)kWTile);
+    const size_t lds = (size_t)(kWTile + win - 1) * sizeof(double);
+    const dim3 grid((unsigned)frames, tiles);
+    const void *fn = cplx ? (const void *)s2s_weight_tile_kernel<true>
+                          : (const void *)s2s_weight_tile_kernel<false>;
+    FXC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (cplx)
+      hipLaunchKernelGGL(s2s_weight_tile_kernel<true>, grid, dim3(kWThreads), lds, st, data, pitch,
+                         width, nch, win, w, wpitch);
+    else
+      hipLaunchKernelGGL(s2s_weight_tile_kernel<false>, grid, dim3(kWThreads), lds, st, data,
+                         pitch, width, nch, win, w, wpitch);
+    FXC_HIP(hipGetLastError());
+    return FXC_OK;
+  }
+  // very long windows (bands far below 0.001 cycles per sample): rolling form
+  FXC_CHECK_ARG(frames <= 65535, "sac2spec: %lld frames exceed one launch", frames);
   const int per_block = 256 * kWeightChunk;
   const unsigned gx = (unsigned)ceil_div((size_t)width, (size_t)per_block);
   const dim3 grid(gx, (unsigned)frames);
