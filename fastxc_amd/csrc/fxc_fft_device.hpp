@@ -117,6 +117,50 @@ __device__ __forceinline__ void dft(float2 *a, float sgn) {
   dft_generic<R>(a, sgn);
 }
 
+// The odd-radix DFT of dft_generic with its outputs handed to emit(q, b_q)
+// as they are formed (q = 0, then the pairs q, R-q), so no output array is
+// live: a[m] and a[R-m] are overwritten by t_m and u_m.  Used by the passes
+// for the large prime radices, where an R-value output array would double
+// the registers the butterfly holds.
+template <int R, typename Emit>
+__device__ __forceinline__ void dft_odd_stream(float2 *a, float sgn, Emit emit) {
+  static_assert((R & 1) == 1, "streaming butterfly is for odd radices");
+  constexpr RootTable<R> tb{};
+  constexpr int H = (R - 1) / 2;
+  float2 s0 = a[0];
+#pragma unroll
+  for (int m = 1; m <= H; ++m) {
+    const float2 x = a[m], y = a[R - m];
+    a[m] = cadd(x, y);
+    a[R - m] = csub(x, y);
+    s0 = cadd(s0, a[m]);
+  }
+  emit(0, s0);
+#pragma unroll
+  for (int q = 1; q <= H; ++q) {
+    float2 re = a[0];
+    float2 im = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int m = 1; m <= H; ++m) {
+      const int idx = (m * q) % R;
+      re.x += a[m].x * tb.c[idx];
+      re.y += a[m].y * tb.c[idx];
+      im.x += a[R - m].x * tb.s[idx];
+      im.y += a[R - m].y * tb.s[idx];
+    }
+    const float2 iim = cmul_isgn(im, sgn);
+    emit(q, cadd(re, iim));
+    emit(R - q, csub(re, iim));
+    // one output pair in flight: keeps the accumulators of later pairs
+    // from being hoisted (registers, not ILP, bound these butterflies)
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Radices whose passes use dft_odd_stream.
+template <int R>
+constexpr bool kStreamRadix = (R == 23 || R == 29);
+
 template <>
 __device__ __forceinline__ void dft<2>(float2 *a, float) {
   const float2 x0 = a[0], x1 = a[1];
@@ -254,10 +298,43 @@ __device__ __forceinline__ void dif_pass(float2 *buf, int ntrans, const FftPlanD
     float2 v[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) v[m] = base[m * stride];
-    dft<R>(v, sgn);
-    if (j > 0) apply_twiddles<R>(v, tw, j * tstep, n);
+    if constexpr (kStreamRadix<R>) {
+      // outputs leave as formed: b_q x w^(q e), e = j tstep (mod n).  The
+      // output offset goes through an empty asm operand so the compiler
+      // re-derives the store addresses instead of holding R load addresses.
+      int off = t * n + blk * span + j;
+      asm volatile("" : "+v"(off));
+      float2 *obase = buf + off;
+      const int e = j * tstep;
+      const int eR = (int)(((long long)e * R) % n);
+      int eq = 0;
+      dft_odd_stream<R>(v, sgn, [&](int q, float2 y) {
+        if (q != 0 && j > 0) {
+          int x;
+          if (q <= (R - 1) / 2) {
+            eq += e;
+            if (eq >= n) eq -= n;
+            x = eq;
+          } else {
+            x = eR - eq;  // (R - q) e = R e - q e, q e being the last eq
+            if (x < 0) x += n;
+          }
+          y = cmul(y, tw_lookup(tw, x));
+        }
+        obase[q * stride] = y;
+      });
+    } else {
+      dft<R>(v, sgn);
+      if (j > 0) apply_twiddles<R>(v, tw, j * tstep, n);
+      float2 *obase = base;
+      if constexpr (R >= 12) {  // as above, for the large composite butterflies
+        int off = t * n + blk * span + j;
+        asm volatile("" : "+v"(off));
+        obase = buf + off;
+      }
 #pragma unroll
-    for (int q = 0; q < R; ++q) base[q * stride] = v[q];
+      for (int q = 0; q < R; ++q) obase[q * stride] = v[q];
+    }
   }
   __syncthreads();
 }
@@ -285,9 +362,22 @@ __device__ __forceinline__ void dit_pass(float2 *buf, int ntrans, const FftPlanD
 #pragma unroll
     for (int m = 0; m < R; ++m) v[m] = base[m * stride];
     if (j > 0) apply_twiddles<R>(v, tw, j * tstep, n);
-    dft<R>(v, sgn);
+    if constexpr (kStreamRadix<R>) {
+      int off = t * n + blk * span + j;  // see dif_pass
+      asm volatile("" : "+v"(off));
+      float2 *obase = buf + off;
+      dft_odd_stream<R>(v, sgn, [&](int q, float2 y) { obase[q * stride] = y; });
+    } else {
+      dft<R>(v, sgn);
+      float2 *obase = base;
+      if constexpr (R >= 12) {  // see dif_pass
+        int off = t * n + blk * span + j;
+        asm volatile("" : "+v"(off));
+        obase = buf + off;
+      }
 #pragma unroll
-    for (int q = 0; q < R; ++q) base[q * stride] = v[q];
+      for (int q = 0; q < R; ++q) obase[q * stride] = v[q];
+    }
   }
   __syncthreads();
 }
@@ -342,7 +432,7 @@ constexpr unsigned kRadixAll = kRadixSmooth | (1u << 23) | (1u << 29);
 
 // Natural-order input -> digit-reversed output (tw: LDS twiddle table).
 template <unsigned kMask = kRadixSmooth>
-__device__ inline void lds_fft(float2 *buf, int ntrans, const FftPlanDev &plan, const float2 *tw,
+__device__ __forceinline__ void lds_fft(float2 *buf, int ntrans, const FftPlanDev &plan, const float2 *tw,
                                float sgn) {
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
@@ -356,7 +446,7 @@ __device__ inline void lds_fft(float2 *buf, int ntrans, const FftPlanDev &plan, 
 // whose output t is bin k = lower + (n/R) t, where the block starts at the
 // digit-reversed position of lower = k mod (n/R) (see last_pass_bin).
 template <unsigned kMask = kRadixSmooth>
-__device__ inline void lds_fft_head(float2 *buf, int ntrans, const FftPlanDev &plan,
+__device__ __forceinline__ void lds_fft_head(float2 *buf, int ntrans, const FftPlanDev &plan,
                                     const float2 *tw, float sgn, int npass_run) {
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
@@ -402,7 +492,7 @@ __device__ inline float2 last_pass_bin(const float2 *buf, const FftPlanDev &plan
 // Digit-reversed input -> natural-order output (the inverse ordering of
 // lds_fft with the same plan radices; tw built for the sign sgn).
 template <unsigned kMask = kRadixSmooth>
-__device__ inline void lds_fft_dit(float2 *buf, int ntrans, const FftPlanDev &plan,
+__device__ __forceinline__ void lds_fft_dit(float2 *buf, int ntrans, const FftPlanDev &plan,
                                    const float2 *tw, float sgn) {
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;

@@ -36,6 +36,7 @@ constexpr int kStkThreads = 256;
 constexpr int kStkBatch = 4;       // transforms held in LDS at once (at most)
 constexpr int kVoiceChunk = 32;    // TF-PWS voices per workgroup
 constexpr int kLdsReserve = 16;    // float2 slots left for static LDS
+constexpr int kVoiceThreads = 512; // PWS and TF-PWS voice workgroups
 
 // Length-n transform of the stacks: the direct mixed-radix plan when n
 // factors into the implemented radices (rows of n values, digit-reversed
@@ -72,7 +73,7 @@ __device__ __forceinline__ void stage_stack_tw(float2 *twl, const StackFftDev &F
 // F.ld whose input is natural order in [0, n).  The caller synchronises
 // before the call; the call ends with a barrier.  Output position of bin k:
 // stack_pos(F, k).
-__device__ void stack_fft(float2 *buf, int nt, const StackFftDev &F, const float2 *twl, float sgn) {
+__device__ __forceinline__ void stack_fft(float2 *buf, int nt, const StackFftDev &F, const float2 *twl, float sgn) {
   const float2 *twf = twl;
   const float2 *twi = twl + kTwFine + F.fwd.ncoarse;
   if (!F.blue) {
@@ -139,69 +140,94 @@ analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, StackFf
   }
 }
 
-// PWS: one workgroup per pair.
-__global__ void __launch_bounds__(kStkThreads)
+// PWS: one workgroup per pair.  Thread t owns transform positions
+// p = t + k kThreads (k < kPer), so W stays in registers and the LDS holds
+// only the group rows and the twiddles.
+template <int kThreads, int kPer>
+__global__ void __launch_bounds__(kThreads)
 pws_kernel(const float2 *__restrict__ A, const float *__restrict__ weights,
            const long long *__restrict__ offsets, const float *__restrict__ lin, StackFftDev F,
            const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i, int nbatch,
            float *__restrict__ out) {
   extern __shared__ float2 lds[];
   const int n = F.n, ld = F.ld;
-  float2 *W = lds;                       // [n], transform output positions
-  float2 *buf = lds + n;                 // [nbatch][ld]
+  float2 *buf = lds;                     // [nbatch][ld]
   float2 *twl = buf + (size_t)nbatch * ld;
   stage_stack_tw(twl, F, tw_f, tw_i);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) W[i] = make_float2(0.f, 0.f);
+  const int tid = threadIdx.x;
+  float2 W[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) W[k] = make_float2(0.f, 0.f);
   const int pair = blockIdx.x;
   const long long g0 = offsets[pair], g1 = offsets[pair + 1];
   const float invn = 1.f / (float)n;
   for (long long gb = g0; gb < g1; gb += nbatch) {
     const int nb = (int)std::min<long long>(nbatch, g1 - gb);
     __syncthreads();
-    for (int i = threadIdx.x; i < nb * n; i += blockDim.x) {
+    for (int i = tid; i < nb * n; i += kThreads) {
       const int b = i / n, c = i - b * n;
       buf[b * ld + c] = A[(gb + b) * n + c];
     }
     __syncthreads();
     stack_fft(buf, nb, F, twl, 1.f);
-    for (int p = threadIdx.x; p < n; p += blockDim.x) {
-      float2 acc = W[p];
-      for (int b = 0; b < nb; ++b) {
-        const float2 z = buf[b * ld + p];
-        const float re = z.x * invn, im = z.y * invn;
-        float mod = sqrtf(re * re + im * im);
-        mod = mod > 1e-7f ? mod : 1e-7f;
-        const float wg = weights[gb + b];
-        acc.x += (re / mod) * wg;
-        acc.y += (im / mod) * wg;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int p = tid + k * kThreads;
+      if (p < n) {
+        float2 acc = W[k];
+        for (int b = 0; b < nb; ++b) {
+          const float2 z = buf[b * ld + p];
+          const float re = z.x * invn, im = z.y * invn;
+          float mod = sqrtf(re * re + im * im);
+          mod = mod > 1e-7f ? mod : 1e-7f;
+          const float wg = weights[gb + b];
+          acc.x += (re / mod) * wg;
+          acc.y += (im / mod) * wg;
+        }
+        W[k] = acc;
       }
-      W[p] = acc;
     }
+  }
+  __syncthreads();
+  // W is indexed by transform position: publish it in LDS, read it by lag
+  float2 *Ws = buf;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int p = tid + k * kThreads;
+    if (p < n) Ws[p] = W[k];
   }
   __syncthreads();
   const float *l = lin + (size_t)pair * n;
   float *o = out + (size_t)pair * n;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) {
-    const float2 w = W[stack_pos(F, t)];
+  for (int t = tid; t < n; t += kThreads) {
+    const float2 w = Ws[stack_pos(F, t)];
     o[t] = l[t] * sqrtf(w.x * w.x + w.y * w.y);
   }
 }
 
 // TF-PWS voices [f0, f0 + kVoiceChunk) of one pair: out_spec[pair][f] =
-// sum_tau ST_lin,f(tau) |W_f(tau)|.
-__global__ void __launch_bounds__(kStkThreads)
+// sum_tau ST_lin,f(tau) |W_f(tau)|.  The LDS holds `nrows` rows; groups go
+// through them `nbatch` at a time, and the linear stack's modulated
+// spectrum rides in the row after the last batch's groups when one is free
+// (then a voice with few groups is a single transform call over G + 1 rows,
+// whose large-radix passes have enough butterflies to occupy the threads),
+// else it is transformed alone after them.  Thread t owns transform
+// positions p = t + k kThreads (k < kPer), so W_f stays in its registers;
+// LDS holds only the Gaussian, the rows and the twiddles (74 KB at n = 2001
+// with three groups: two workgroups per CU).
+template <int kThreads, int kPer>
+__global__ void __launch_bounds__(kThreads)
 tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Alin,
                    const float *__restrict__ weights, const long long *__restrict__ offsets,
                    StackFftDev F, const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i,
-                   int nbatch, int nfreq, const unsigned char *__restrict__ chunk_active,
-                   float2 *__restrict__ out_spec) {
+                   int nrows, int nbatch, int nfreq,
+                   const unsigned char *__restrict__ chunk_active, float2 *__restrict__ out_spec) {
   extern __shared__ float2 lds[];
   const int n = F.n, ld = F.ld;
-  float2 *W = lds;                                   // [n]
-  float *gauss = (float *)(lds + n);                 // [n] floats (n/2 complex slots)
-  float2 *buf = lds + n + (n + 1) / 2;               // [nbatch][ld]
-  float2 *twl = buf + (size_t)nbatch * ld;
-  __shared__ float red_x[kStkThreads / 64], red_y[kStkThreads / 64];
+  float *gauss = (float *)lds;                       // [n] floats (n/2 complex slots)
+  float2 *buf = lds + (n + 1) / 2;                   // [nrows][ld]
+  float2 *twl = buf + (size_t)nrows * ld;
+  __shared__ float red_x[kThreads / 64], red_y[kThreads / 64];
   const int pair = blockIdx.y;
   const int chunk = blockIdx.x;
   if (chunk_active && !chunk_active[chunk]) return;  // uniform per workgroup
@@ -212,10 +238,11 @@ tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Ali
   const int half = (n + 1) / 2;
   const float pi2 = -2.0f * (float)M_PI * (float)M_PI;
   const float nf = (float)n;
+  const int tid = threadIdx.x;
   for (int f = f0; f < std::min(nfreq, f0 + kVoiceChunk); ++f) {
     __syncthreads();
     // Gaussian window of voice f (gaussianModulateSub, scale 0.1)
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    for (int c = tid; c < n; c += kThreads) {
       float g;
       if (f == 0) {
         g = 1.f;
@@ -226,59 +253,73 @@ tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Ali
         g = expf(pi2 * (float)cm * (float)cm * 0.1f / ((float)f * (float)f));
       }
       gauss[c] = g;
-      W[c] = make_float2(0.f, 0.f);
     }
-    for (long long gb = g0; gb < g1; gb += nbatch) {
-      const int nb = (int)std::min<long long>(nbatch, g1 - gb);
+    float2 W[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) W[k] = make_float2(0.f, 0.f);
+    // batches of nbatch groups; the linear stack joins the last one when a
+    // row is free there, else it forms one more batch on its own (one call
+    // site for every transform keeps the kernel's registers at one copy)
+    const long long G = g1 - g0;
+    const long long nbat = (G + nbatch - 1) / nbatch;
+    const bool lin_joins = G - (nbat - 1) * nbatch < nrows;
+    const long long total = nbat + (lin_joins ? 0 : 1);
+    int lin_row = 0;
+    for (long long bi = 0; bi < total; ++bi) {
+      const long long gb = g0 + bi * nbatch;
+      const int nb = bi < nbat ? (int)std::min<long long>(nbatch, g1 - gb) : 0;
+      const int rows = nb + (bi == total - 1 ? 1 : 0);  // + linear stack in the last
       __syncthreads();
-      for (int i = threadIdx.x; i < nb * n; i += blockDim.x) {
-        const int b = i / n, c = i - b * n;
+      for (int i = tid; i < rows * n; i += kThreads) {
+        const int row = i / n, c = i - row * n;
         int src = f + c;
         src = src >= n ? src - n : src;
-        buf[b * ld + c] = cscale(Ag[(gb + b) * n + src], gauss[c]);
+        const float2 a = row == nb ? Al[src] : Ag[(gb + row) * n + src];
+        buf[row * ld + c] = cscale(a, gauss[c]);
       }
       __syncthreads();
-      stack_fft(buf, nb, F, twl, 1.f);
-      for (int p = threadIdx.x; p < n; p += blockDim.x) {
-        float2 acc = W[p];
-        for (int b = 0; b < nb; ++b) {
-          const float2 z = buf[b * ld + p];
-          float fa = sqrtf(z.x * z.x + z.y * z.y);
-          if (fa <= 1e-7f) fa = 1.f;
-          const float wg = weights[gb + b];
-          acc.x += wg * z.x / (fa * nf);
-          acc.y += wg * z.y / (fa * nf);
+      stack_fft(buf, rows, F, twl, 1.f);
+      lin_row = nb;
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int p = tid + k * kThreads;
+        if (p < n) {
+          float2 acc = W[k];
+          for (int b = 0; b < nb; ++b) {
+            const float2 z = buf[b * ld + p];
+            float fa = sqrtf(z.x * z.x + z.y * z.y);
+            if (fa <= 1e-7f) fa = 1.f;
+            const float wg = weights[gb + b];
+            acc.x += wg * z.x / (fa * nf);
+            acc.y += wg * z.y / (fa * nf);
+          }
+          W[k] = acc;
         }
-        W[p] = acc;
       }
     }
-    __syncthreads();
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-      int src = f + c;
-      src = src >= n ? src - n : src;
-      buf[c] = cscale(Al[src], gauss[c]);
-    }
-    __syncthreads();
-    stack_fft(buf, 1, F, twl, 1.f);
+    const float2 *L = buf + (size_t)lin_row * ld;
     float sx = 0.f, sy = 0.f;
-    for (int p = threadIdx.x; p < n; p += blockDim.x) {
-      const float2 w = W[p];
-      const float mag = sqrtf(w.x * w.x + w.y * w.y);
-      sx += buf[p].x * mag;
-      sy += buf[p].y * mag;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int p = tid + k * kThreads;
+      if (p < n) {
+        const float mag = sqrtf(W[k].x * W[k].x + W[k].y * W[k].y);
+        sx += L[p].x * mag;
+        sy += L[p].y * mag;
+      }
     }
     for (int off = 32; off > 0; off >>= 1) {
       sx += __shfl_down(sx, off);
       sy += __shfl_down(sy, off);
     }
-    if ((threadIdx.x & 63) == 0) {
-      red_x[threadIdx.x >> 6] = sx;
-      red_y[threadIdx.x >> 6] = sy;
+    if ((tid & 63) == 0) {
+      red_x[tid >> 6] = sx;
+      red_y[tid >> 6] = sy;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       float tx = 0.f, ty = 0.f;
-      for (int i = 0; i < kStkThreads / 64; ++i) {
+      for (int i = 0; i < kThreads / 64; ++i) {
         tx += red_x[i];
         ty += red_y[i];
       }
@@ -396,10 +437,17 @@ static int stack_fft_plan(int n, StackFft *S) {
   return FXC_OK;
 }
 
-// Rows of `ld` the LDS holds beside `fixed` other slots (0 when none fit).
-static int stack_batch(const StackFft &S, size_t fixed) {
-  const long long room = (long long)kLdsComplexCapacity - kLdsReserve - (long long)fixed - S.twc;
-  return room < S.dev.ld ? 0 : (int)std::min<long long>(kStkBatch, room / S.dev.ld);
+// Rows of `ld` the LDS holds beside `fixed` other slots and `extra_rows`
+// reserved rows, capped by the largest group count of the call (0 when none
+// fit).
+static int stack_batch(const StackFft &S, size_t fixed, int extra_rows, const long long *off,
+                       int npairs) {
+  long long gmax = 1;
+  for (int i = 0; i < npairs; ++i) gmax = std::max(gmax, off[i + 1] - off[i]);
+  const long long room = (long long)kLdsComplexCapacity - kLdsReserve - (long long)fixed - S.twc -
+                         (long long)extra_rows * S.dev.ld;
+  if (room < S.dev.ld) return 0;
+  return (int)std::min<long long>(std::min<long long>(kStkBatch, gmax), room / S.dev.ld);
 }
 
 static int stack_common(const float *d_pre, const float *d_lin, const long long *h_off, int npairs,
@@ -410,7 +458,8 @@ static int stack_common(const float *d_pre, const float *d_lin, const long long 
   FXC_TRY(dmalloc((void **)&w.offsets, (size_t)(npairs + 1) * sizeof(long long)));
   FXC_HIP(hipMemcpyAsync(w.offsets, h_off, (size_t)(npairs + 1) * sizeof(long long),
                          hipMemcpyHostToDevice, st));
-  const int per = std::max(1, std::min(8, (kLdsComplexCapacity - S.twc) / S.dev.ld));
+  // at most 3 rows per workgroup: three workgroups share a CU at n = 2001
+  const int per = std::max(1, std::min(3, (kLdsComplexCapacity - S.twc) / S.dev.ld));
   const size_t lds = ((size_t)per * S.dev.ld + S.twc) * sizeof(float2);
   FXC_HIP(hipFuncSetAttribute((const void *)analytic_kernel,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -424,6 +473,30 @@ static int stack_common(const float *d_pre, const float *d_lin, const long long 
                      S.tw_i, w.A + (size_t)ng * n);
   FXC_HIP(hipGetLastError());
   return FXC_OK;
+}
+
+// The voice kernel's per-thread W registers come in steps of 4 positions.
+static int launch_voice(int n, size_t lds, dim3 grid, hipStream_t st, const float2 *Ag,
+                        const float2 *Alin, const float *wts, const long long *off,
+                        const StackFft &S, int nrows, int nbatch, int nfreq,
+                        const unsigned char *active, float2 *spec) {
+  const int per = (n + kVoiceThreads - 1) / kVoiceThreads;
+#define FXC_VOICE(P)                                                                        \
+  if (per <= P) {                                                                          \
+    auto *k = tfpws_voice_kernel<kVoiceThreads, P>;                                        \
+    FXC_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)lds));                                                \
+    hipLaunchKernelGGL(k, grid, dim3(kVoiceThreads), lds, st, Ag, Alin, wts, off, S.dev,   \
+                       S.tw_f, S.tw_i, nrows, nbatch, nfreq, active, spec);                \
+    FXC_HIP(hipGetLastError());                                                            \
+    return FXC_OK;                                                                         \
+  }
+  FXC_VOICE(4)
+  FXC_VOICE(8)
+  FXC_VOICE(12)
+#undef FXC_VOICE
+  FXC_CHECK_ARG(false, "fxc_tfpws: length %d too long for the voice kernel", n);
+  return FXC_ERR_ARG;
 }
 
 static int check_stack_args(const float *pre, const float *wts, const long long *off,
@@ -445,17 +518,30 @@ extern "C" int fxc_pws(const float *d_prestack, const float *d_weights, const lo
   FXC_TRY(check_stack_args(d_prestack, d_weights, h_offsets, d_linear, npairs, n, d_out));
   StackFft S;
   FXC_TRY(stack_fft_plan(n, &S));
-  const int nbatch = stack_batch(S, (size_t)n);
-  FXC_CHECK_ARG(nbatch >= 1, "fxc_pws: length %d too long for the LDS transforms", n);
+  // the rows must also hold W (n values) after the last batch
+  const int nbatch = std::max(stack_batch(S, 0, 0, h_offsets, npairs),
+                              (int)ceil_div((size_t)n, (size_t)S.dev.ld));
+  FXC_CHECK_ARG((size_t)nbatch * S.dev.ld + S.twc + kLdsReserve <= (size_t)kLdsComplexCapacity,
+                "fxc_pws: length %d too long for the LDS transforms", n);
+  const int per = (n + kVoiceThreads - 1) / kVoiceThreads;
+  FXC_CHECK_ARG(per <= 16, "fxc_pws: length %d too long", n);
   hipStream_t st = (hipStream_t)stream;
   StackWork w;
   FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, S, st));
-  const size_t lds = ((size_t)n + (size_t)nbatch * S.dev.ld + S.twc) * sizeof(float2);
-  FXC_HIP(hipFuncSetAttribute((const void *)pws_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds));
-  hipLaunchKernelGGL(pws_kernel, dim3((unsigned)npairs), dim3(kStkThreads), lds, st, w.A, d_weights,
-                     w.offsets, d_linear, S.dev, S.tw_f, S.tw_i, nbatch, d_out);
-  FXC_HIP(hipGetLastError());
+  const size_t lds = ((size_t)nbatch * S.dev.ld + S.twc) * sizeof(float2);
+#define FXC_PWS(P)                                                                            \
+  if (per <= P) {                                                                            \
+    auto *k = pws_kernel<kVoiceThreads, P>;                                                  \
+    FXC_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                (int)lds));                                                  \
+    hipLaunchKernelGGL(k, dim3((unsigned)npairs), dim3(kVoiceThreads), lds, st, w.A,         \
+                       d_weights, w.offsets, d_linear, S.dev, S.tw_f, S.tw_i, nbatch, d_out); \
+    FXC_HIP(hipGetLastError());                                                              \
+  } else
+  FXC_PWS(4)
+  FXC_PWS(8)
+  FXC_PWS(16) {}
+#undef FXC_PWS
   FXC_HIP(hipStreamSynchronize(st));  // workspace is released on return
   return FXC_OK;
 }
@@ -468,8 +554,16 @@ extern "C" int fxc_tfpws(const float *d_prestack, const float *d_weights,
   FXC_CHECK_ARG(!band_limited || (delta > 0.f && fmax > fmin), "fxc_tfpws: invalid band");
   StackFft S;
   FXC_TRY(stack_fft_plan(n, &S));
-  const int nbatch = stack_batch(S, (size_t)n + (size_t)(n + 1) / 2);
-  FXC_CHECK_ARG(nbatch >= 1, "fxc_tfpws: length %d too long for the LDS transforms", n);
+  // rows: up to kStkBatch groups + the linear stack, as many as the LDS holds
+  long long gmax = 1;
+  for (int i = 0; i < npairs; ++i) gmax = std::max(gmax, h_offsets[i + 1] - h_offsets[i]);
+  const long long room = (long long)kLdsComplexCapacity - kLdsReserve - (n + 1) / 2 - S.twc;
+  const int nrows = (int)std::min<long long>(std::min<long long>(kStkBatch, gmax) + 1,
+                                             room / S.dev.ld);
+  FXC_CHECK_ARG(nrows >= 1, "fxc_tfpws: length %d too long for the LDS transforms", n);
+  const int nbatch = (int)std::min<long long>(nrows, std::min<long long>(kStkBatch, gmax));
+  FXC_CHECK_ARG((n + kVoiceThreads - 1) / kVoiceThreads <= 12,
+                "fxc_tfpws: length %d too long for the voice kernel", n);
   hipStream_t st = (hipStream_t)stream;
   StackWork w;
   FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, S, st));
@@ -490,13 +584,10 @@ extern "C" int fxc_tfpws(const float *d_prestack, const float *d_weights,
     FXC_HIP(hipMemcpyAsync(w.active, act.data(), nchunks, hipMemcpyHostToDevice, st));
   }
   const long long ng = h_offsets[npairs];
-  size_t lds = ((size_t)n + (n + 1) / 2 + (size_t)nbatch * S.dev.ld + S.twc) * sizeof(float2);
-  FXC_HIP(hipFuncSetAttribute((const void *)tfpws_voice_kernel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(tfpws_voice_kernel, dim3((unsigned)nchunks, (unsigned)npairs),
-                     dim3(kStkThreads), lds, st, w.A, w.A + (size_t)ng * n, d_weights, w.offsets,
-                     S.dev, S.tw_f, S.tw_i, nbatch, nfreq, (const unsigned char *)w.active, w.spec);
-  FXC_HIP(hipGetLastError());
+  size_t lds = ((size_t)(n + 1) / 2 + (size_t)nrows * S.dev.ld + S.twc) * sizeof(float2);
+  FXC_TRY(launch_voice(n, lds, dim3((unsigned)nchunks, (unsigned)npairs), st, w.A,
+                       w.A + (size_t)ng * n, d_weights, w.offsets, S, nrows, nbatch, nfreq,
+                       (const unsigned char *)w.active, w.spec));
   lds = ((size_t)S.dev.ld + S.twc) * sizeof(float2);
   FXC_HIP(hipFuncSetAttribute((const void *)tfpws_final_kernel,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -3,7 +3,8 @@
 Lengths cover the direct mixed-radix transform (87 = 3*29, 345, 2001 =
 3*23*29) and Bluestein's convolution for lengths with other prime factors
 (201 = 3*67, 401 prime, 1001 = 7*11*13), as the correlation length 2H+1 is
-arbitrary.  Inputs follow the reference host preparation in float32
+arbitrary; at 4999 (prime, convolution length 10000) one TF-PWS workgroup's
+LDS holds a single row, so groups and the linear stack take turns in it.  Inputs follow the reference host preparation in float32
 (native/pws/io/sourcepack_io.cu:385-478, native/tfpws/io/sourcepack_io.cu:403-490);
 the oracle prepares the same records in FP64.  Tolerance: relative L2
 1e-5 for PWS, 1e-4 for TF-PWS (a sum over n/2+1 voices of FP32 inverse
@@ -77,7 +78,8 @@ def test_pws_matches_oracle(gpu, N):
 
 
 @pytest.mark.parametrize("N,band", [(87, None), (345, None), (2001, None), (345, (0.1, 0.3, 0.02)),
-                                    (201, None), (1001, None), (401, (0.1, 0.3, 0.02))])
+                                    (201, None), (1001, None), (401, (0.1, 0.3, 0.02)),
+                                    (4999, None)])
 def test_tfpws_matches_oracle(gpu, N, band):
     import torch
     from fastxc_amd import ops
