@@ -222,12 +222,29 @@ def _time_calls(fn, reps):
     return (time.perf_counter() - t0) / reps
 
 
+STAGE_TRAFFIC = os.path.join(ROOT, "profiles", "stages_hbm_traffic_1000sta.txt")
+
+
+def _stage_traffic(name):
+    """HBM bytes per call of the stage (its kernels summed), measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE at these workloads; None when absent."""
+    try:
+        with open(STAGE_TRAFFIC) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == name:
+                    return int(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def _stage(name, units, unit, sec, work, bound, peak, punit, detail):
     ach = work / sec / (1e9 if punit == "GB/s" else 1e12)
     return {"stage": name, "value": round(units / sec, 1), "unit": unit,
             "ms_per_call": round(sec * 1e3, 3), "detail": detail,
             "roofline": {"bound": bound, "achieved": round(ach, 2), "peak": peak, "unit": punit,
-                         "frac": round(ach / peak, 4), "traffic": None}}
+                         "frac": round(ach / peak, 4), "traffic": _stage_traffic(name)}}
 
 
 def _butter_power(nfft, delta, lo, hi):

@@ -16,7 +16,7 @@
 // LDS by 256-thread workgroups with register butterflies up to radix 29;
 // lengths with other prime factors (2H+1 is arbitrary) use Bluestein's
 // convolution of a factorable length m >= 2n-1, also in LDS, so every
-// length up to 6658 (PWS) / 5685 (TF-PWS) is served.
+// length up to 8192 (PWS) / 6144 (TF-PWS) is served.
 // The analytic spectra of all groups are computed once; the TF-PWS voice
 // kernel keeps the modulated transforms, the weight W_f(tau) and the
 // Gaussian in LDS, so per voice only the (group-count + 1) spectra are read

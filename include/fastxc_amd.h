@@ -133,10 +133,10 @@ int fxc_s2s_process(fxc_s2s_plan *plan, const float *d_traces, int nfiles, const
  * Phase-weighted stacks of cross-correlations of length n (2H+1).  Groups of
  * pair p are rows h_offsets[p] .. h_offsets[p+1]-1 of d_prestack [groups][n];
  * d_weights [groups] = n_g/T; d_linear [npairs][n]; d_out [npairs][n].
- * Every n from 2 to 6658 (PWS) / 5685 (TF-PWS) is served: n with prime
- * factors in {2,3,5,7,23,29} by the direct transform (which also serves
- * such n up to 10005 / 8004), other n by Bluestein's convolution.  A length
- * whose transforms do not fit one workgroup's LDS is FXC_ERR_ARG ("too long"). 
+ * Every n from 2 to 8192 (PWS) / 6144 (TF-PWS) is served: n with prime
+ * factors in {2,3,5,7,23,29} by the direct transform, other n by Bluestein's
+ * convolution.  A longer n is FXC_ERR_ARG ("too long"), raised before any
+ * launch.
  * fxc_pws: prestack rows are group MEANS; out = lin * |sum_g w_g phi_g|
  * (compute_pws_host_batch, native/pws/worker/gpu.cu:87-282).
  * fxc_tfpws: prestack rows are group SUMS; Stockwell-domain PWS with
