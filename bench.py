@@ -132,8 +132,13 @@ def roofline(mode, args, npairs, default_workload):
         t = mode["contract_ms"] / max(mode["contract_launches_per_step"], 1)
         launches = max(mode["contract_launches_per_step"], 1)
     else:
-        # nfft 37044 runs the compile-time-plan lag kernel
-        name = "lag_kernel_fixed" if args.nspec == 18523 else "lag_kernel"
+        # nfft 37044 runs the compile-time-plan lag kernel (pass 0 in registers
+        # unless FXC_XC_LAG_R0=0 selects the all-LDS form)
+        if args.nspec == 18523:
+            name = "lag_kernel_fixed" if os.environ.get("FXC_XC_LAG_R0") == "0" \
+                else "lag_kernel_fixed_r0"
+        else:
+            name = "lag_kernel"
         t = mode["lag_ms"] / max(mode["lag_launches_per_step"], 1)
         launches = max(mode["lag_launches_per_step"], 1)
     byts = algorithmic_bytes(args, npairs) / launches

@@ -715,6 +715,148 @@ lag_kernel_fixed(const float2 *__restrict__ xspec, long long xpitch, long long n
   }
 }
 
+// Lag synthesis at nfft 37044 with the fold and the first DIF pass taken in
+// registers straight from the row, so the row reaches LDS once, already
+// through pass 0 (the store of the raw row, the fold sweep and pass 0's LDS
+// round trip of lag_kernel_fixed disappear, with two barriers).
+//
+// Pass 0 (radix R0, stride S0 = M/R0) butterfly j reads Z[j + S0 m]; the fold
+// of bin k needs X[k] and X[M-k], and M - (j + S0 m) = (S0 - j) + S0 (R0-1-m)
+// is leg R0-1-m of butterfly j' = S0 - j.  One thread therefore takes the
+// pair (j, j'): it loads the R0 legs of j and the R0 mirrored bins, forms the
+// 2 R0 folded values, runs both butterflies and writes them to LDS.  j = 0 is
+// its own mirror (leg m pairs with leg R0-m, bin 0 with X[M]).  The fold
+// factor of bin j + S0 m is w_j c_m with w_j = exp(2 pi i j/2M) and the
+// constant c_m = exp(2 pi i m/(2 R0)); w_{j'} = c_1 conj(w_j).  The pass-0
+// twiddles are the powers of t_j = exp(2 pi i j/M), with t_{j'} =
+// exp(2 pi i/R0) conj(t_j); w_j and t_j of a thread's pairs stay in
+// registers for the whole launch.  The next row is loaded into the same
+// registers once pass 0 is in LDS, and every table the later passes read
+// (base twiddles, the last radix's roots) is in LDS, so no global read waits
+// behind that load (vector-memory counters retire in order).
+template <class FP>
+__device__ __forceinline__ float2 fold_bin(float2 A, float2 B, float2 w) {
+  // Z = A + conj(B) + i w (A - conj(B))
+  const float2 s = make_float2(A.x + B.x, A.y - B.y);
+  const float2 d = cmul(w, make_float2(A.x - B.x, A.y + B.y));
+  return make_float2(s.x - d.y, s.y + d.x);
+}
+
+template <class FP>
+struct LagPass0 {
+  static constexpr int M = FP::kM;
+  static constexpr int R0 = FP::kRadix[0];
+  static constexpr int S0 = FP::stride(0);
+  static constexpr int kPairs = (S0 + 1) / 2;  // j = 0 and the pairs (j, S0 - j)
+  static constexpr int kPer = (kPairs + kLagThreads - 1) / kLagThreads;
+  static_assert(S0 % 2 == 1, "pass-0 butterflies pair up as (j, S0 - j)");
+};
+
+template <class FP>
+__device__ __forceinline__ void lag_r0_load(float2 (&a)[LagPass0<FP>::kPer][FP::kRadix[0]],
+                                            float2 (&b)[LagPass0<FP>::kPer][FP::kRadix[0]],
+                                            const float2 *__restrict__ X) {
+  using L = LagPass0<FP>;
+#pragma unroll
+  for (int it = 0; it < L::kPer; ++it) {
+    const int j = threadIdx.x + it * kLagThreads;
+    if (j < L::kPairs) {
+      if (j == 0) {
+#pragma unroll
+        for (int m = 0; m < L::R0; ++m) a[it][m] = X[L::S0 * m];
+        b[it][0] = X[L::M];
+      } else {
+#pragma unroll
+        for (int m = 0; m < L::R0; ++m) {
+          a[it][m] = X[j + L::S0 * m];
+          b[it][m] = X[L::M - j - L::S0 * m];
+        }
+      }
+    }
+  }
+}
+
+template <class FP>
+__device__ __forceinline__ void lag_r0_butterfly(float2 *lds, float2 (&u)[FP::kRadix[0]], int j,
+                                                 float2 t) {
+  using L = LagPass0<FP>;
+  dft<L::R0>(u, 1.f);
+  float2 tq = t;
+#pragma unroll
+  for (int q = 1; q < L::R0; ++q) {
+    u[q] = cmul(u[q], tq);
+    if (q + 1 < L::R0) tq = cmul(tq, t);
+  }
+#pragma unroll
+  for (int q = 0; q < L::R0; ++q) lds[j + L::S0 * q] = u[q];
+}
+
+template <class FP>
+__global__ void __launch_bounds__(kLagThreads)
+lag_kernel_fixed_r0(const float2 *__restrict__ xspec, long long xpitch, long long nrows,
+                    const int *__restrict__ row_out, float *__restrict__ cc, int half_cc,
+                    double weight, const float2 *__restrict__ twg, const float2 *__restrict__ twl_g,
+                    const float2 *__restrict__ foldg, const float2 *__restrict__ roots) {
+  using L = LagPass0<FP>;
+  constexpr int R0 = L::R0, S0 = L::S0;
+  constexpr int RL = FP::kRadix[FP::kPasses - 1];
+  extern __shared__ float2 lds[];  // [Z: M][base twiddles of passes 1..P-2][roots of RL]
+  float2 *twl = lds + L::M;
+  float2 *rts = twl + fixed_lds_tw_count<FP>();
+  for (int i = threadIdx.x; i < fixed_lds_tw_count<FP>(); i += kLagThreads) twl[i] = twl_g[i];
+  if (threadIdx.x < RL) rts[threadIdx.x] = roots[threadIdx.x];
+  constexpr RootTable<2 * R0> cr{};  // c_m = exp(2 pi i m / (2 R0))
+  constexpr RootTable<R0> er{};
+  const float2 e1 = make_float2(er.c[1], er.s[1]);  // exp(2 pi i / R0)
+  float2 a[L::kPer][R0], b[L::kPer][R0];
+  long long row = blockIdx.x;
+  if (row < nrows) lag_r0_load<FP>(a, b, xspec + row * xpitch);
+  for (; row < nrows; row += gridDim.x) {
+#pragma unroll
+    for (int it = 0; it < L::kPer; ++it) {
+      const int j = threadIdx.x + it * kLagThreads;
+      if (j >= L::kPairs) continue;
+      // bases of the pair, read per row (L2): waiting for them waits for the
+      // row's own loads, which this pass needs anyway
+      const float2 wjv = foldg[j];
+      const float2 tjv = twg[j * FP::tw_pitch(0)];
+      float2 u[R0];
+      if (j == 0) {
+        u[0] = make_float2(a[it][0].x + b[it][0].x, a[it][0].x - b[it][0].x);
+#pragma unroll
+        for (int m = 1; m < R0; ++m)
+          u[m] = fold_bin<FP>(a[it][m], a[it][R0 - m], make_float2(cr.c[m], cr.s[m]));
+        lag_r0_butterfly<FP>(lds, u, 0, make_float2(1.f, 0.f));
+      } else {
+        float2 v[R0];
+#pragma unroll
+        for (int m = 0; m < R0; ++m) {
+          const float2 w = cmul(wjv, make_float2(cr.c[m], cr.s[m]));
+          u[m] = fold_bin<FP>(a[it][m], b[it][m], w);
+          v[R0 - 1 - m] = fold_bin<FP>(b[it][m], a[it][m], make_float2(-w.x, w.y));
+        }
+        lag_r0_butterfly<FP>(lds, u, j, tjv);
+        lag_r0_butterfly<FP>(lds, v, S0 - j, cmul(e1, cconj(tjv)));
+      }
+    }
+    __syncthreads();
+    const long long nxt = row + gridDim.x;
+    if (nxt < nrows) lag_r0_load<FP>(a, b, xspec + nxt * xpitch);
+    fixed_dif_passes_lds<FP, 1, kLagThreads>(lds, twl, 1.f);
+    const int out_row = row_out ? row_out[row] : (int)row;
+    float *y = cc + (size_t)out_row * (2 * half_cc + 1);
+    const int n_lo = -((half_cc + 1) / 2), n_hi = half_cc / 2;
+    for (int i = threadIdx.x; i <= n_hi - n_lo; i += kLagThreads) {
+      const int n = n_lo + i;
+      const float2 z = fixed_last_pass_bin<FP>(lds, rts, n < 0 ? n + L::M : n);
+      const int m0 = 2 * n + half_cc;
+      if (m0 >= 0) y[m0] = (float)((double)z.x * weight);
+      if (m0 + 1 <= 2 * half_cc) y[m0 + 1] = (float)((double)z.y * weight);
+    }
+    __syncthreads();
+  }
+}
+
 // PLAIN form, one launch per window w (the reference's accumulation order):
 //   stack[r][k] += (conj(S[w][src_r][k]) * S[w][rec_r][k]) * scale
 // Work items are (row, 256-bin block) pairs, nrows * nbk of them, visited
@@ -1208,7 +1350,21 @@ static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long
   // reference weight: 1.0 / (width * dt) with width converted to float
   const double weight = 1.0 / (double)((float)p->nfft * p->dt);
   FXC_TRY(xc_mark(p, kTimeLag, true, st));
-  if (p->d_fix_tw) {
+  static const bool r0_off = [] {
+    const char *e = getenv("FXC_XC_LAG_R0");
+    return e && atoi(e) == 0;
+  }();
+  if (p->d_fix_tw && !r0_off) {
+    using FP = LagPlan18522;
+    const size_t flds = (size_t)(FP::kM + fixed_lds_tw_count<FP>() + FP::kRadix[FP::kPasses - 1]) *
+                        sizeof(float2);
+    auto fk = lag_kernel_fixed_r0<FP>;
+    FXC_HIP(hipFuncSetAttribute((const void *)fk, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)flds));
+    hipLaunchKernelGGL(fk, dim3((unsigned)grid), dim3(kLagThreads), flds, st, xs, xpitch, nrows,
+                       row_out, d_cc, p->half_cc, weight, p->d_fix_tw, p->d_fix_twl,
+                       p->d_fix_fold, p->d_fix_roots);
+  } else if (p->d_fix_tw) {
     using FP = LagPlan18522;
     const size_t flds = (size_t)(FP::kM + 1 + fixed_lds_tw_count<FP>()) * sizeof(float2);
     auto fk = lag_kernel_fixed<FP>;
