@@ -67,6 +67,15 @@ _SIGNATURES = {
                                   POINTER(c_int)]),
     "fxc_s2s_process": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "fxc_xc_get_timing": (c_int, [c_void_p, POINTER(ctypes.c_double), POINTER(c_longlong)]),
+    "fxc_stream_create": (c_int, [POINTER(c_void_p), c_void_p, c_int, c_int, c_int, c_int,
+                                  c_longlong, c_int, c_int]),
+    "fxc_stream_submit": (c_int, [c_void_p, c_void_p, c_int, c_void_p, POINTER(c_longlong)]),
+    "fxc_stream_wait": (c_int, [c_void_p, c_longlong, POINTER(ctypes.c_double)]),
+    "fxc_stream_info": (c_int, [c_void_p, POINTER(c_longlong), POINTER(c_longlong),
+                                POINTER(c_longlong), POINTER(c_longlong)]),
+    "fxc_stream_read": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_longlong, c_int, c_int,
+                                c_float, c_float, c_float, c_void_p, c_void_p, c_void_p]),
+    "fxc_stream_destroy": (c_int, [c_void_p]),
 }
 
 

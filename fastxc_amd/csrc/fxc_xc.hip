@@ -921,6 +921,11 @@ struct fxc_xc_plan {
   int *d_row_task = nullptr;      // global row -> task index
   int *d_row_src = nullptr;       // global row -> src station (PLAIN form)
   int *d_row_rec = nullptr;       // global row -> rec station (PLAIN form)
+  // grow-only capacities of the device index arrays, and the host copies
+  // the uploads read from (kept until the next set_pairs, so an upload
+  // ordered on a stream never reads freed memory)
+  size_t cap_tiles = 0, cap_tile_rows = 0, cap_rows = 0, cap_row_src = 0, cap_row_rec = 0;
+  std::vector<int> h_tile_rows, h_row_task, h_row_src, h_row_rec;
   // workspaces
   float2 *d_xspec = nullptr;
   size_t xspec_rows = 0;
@@ -963,10 +968,23 @@ static void xc_free_pairs(fxc_xc_plan *p) {
   p->d_row_task = nullptr;
   p->d_row_src = nullptr;
   p->d_row_rec = nullptr;
+  p->cap_tiles = p->cap_tile_rows = p->cap_rows = p->cap_row_src = p->cap_row_rec = 0;
   p->h_tiles.clear();
   p->chunk_tile_begin.clear();
   p->chunk_row_begin.clear();
   p->npairs = 0;
+}
+
+// Grows a device array to at least n elements (contents not kept).
+template <class T>
+static int xc_grow(T **d, size_t *cap, size_t n) {
+  if (n <= *cap && *d) return FXC_OK;
+  dfree(*d);
+  *d = nullptr;
+  *cap = 0;
+  FXC_TRY(dmalloc((void **)d, std::max<size_t>(n, 1) * sizeof(T)));
+  *cap = std::max<size_t>(n, 1);
+  return FXC_OK;
 }
 
 extern "C" int fxc_xc_plan_create(fxc_xc_plan **out, int nspec, int nstep, float dt, int half_cc,
@@ -1103,13 +1121,18 @@ extern "C" int fxc_xc_set_chunk_rows(fxc_xc_plan *p, long long rows) {
 
 // Builds the 32x32 station tiles for the task list (src[t], rec[t]) and the
 // row maps.  Tasks keep their index as the output row of fxc_xc_correlate.
-extern "C" int fxc_xc_set_pairs(fxc_xc_plan *p, int nsta, const int *h_src, const int *h_rec,
-                                long long npairs) {
+// Task list of a plan.  With `stream` set the index uploads are ordered on
+// it and the device arrays only grow, so a caller that reuses a plan with at
+// most the pairs it has seen neither allocates nor synchronises the device.
+int xc_set_pairs_on(fxc_xc_plan *p, int nsta, const int *h_src, const int *h_rec, long long npairs,
+                    hipStream_t stream) {
   FXC_CHECK_ARG(p && nsta >= 1 && npairs >= 0 && (npairs == 0 || (h_src && h_rec)),
                 "fxc_xc_set_pairs: invalid arguments");
-  xc_free_pairs(p);
+  p->h_tiles.clear();
+  p->chunk_tile_begin.clear();
+  p->chunk_row_begin.clear();
+  p->npairs = 0;
   p->nsta = nsta;
-  p->npairs = npairs;
   if (npairs == 0) return FXC_OK;
   const int nblk = (nsta + kTile - 1) / kTile;
   // tile key -> list of (local pair index, task)
@@ -1122,8 +1145,10 @@ extern "C" int fxc_xc_set_pairs(fxc_xc_plan *p, int nsta, const int *h_src, cons
     by_tile[key].push_back({(s % kTile) * kTile + (r % kTile), t});
   }
   const long long cap = p->chunk_rows_max > 0 ? p->chunk_rows_max : npairs;
-  std::vector<int> rows;
-  std::vector<int> row_task;
+  std::vector<int> &rows = p->h_tile_rows;
+  std::vector<int> &row_task = p->h_row_task;
+  rows.clear();
+  row_task.clear();
   rows.reserve(by_tile.size() * kTilePairs);
   row_task.reserve(npairs);
   long long chunk_rows = 0;
@@ -1160,26 +1185,39 @@ extern "C" int fxc_xc_set_pairs(fxc_xc_plan *p, int nsta, const int *h_src, cons
   }
   p->chunk_tile_begin.push_back((long long)p->h_tiles.size());
   p->chunk_row_begin.push_back((long long)row_task.size());
-  FXC_TRY(dmalloc((void **)&p->d_tiles, p->h_tiles.size() * sizeof(TileDesc)));
-  FXC_TRY(dmalloc((void **)&p->d_tile_rows, rows.size() * sizeof(int)));
-  FXC_TRY(dmalloc((void **)&p->d_row_task, row_task.size() * sizeof(int)));
-  FXC_HIP(hipMemcpy(p->d_tiles, p->h_tiles.data(), p->h_tiles.size() * sizeof(TileDesc),
-                    hipMemcpyHostToDevice));
-  FXC_HIP(hipMemcpy(p->d_tile_rows, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice));
-  FXC_HIP(hipMemcpy(p->d_row_task, row_task.data(), row_task.size() * sizeof(int),
-                    hipMemcpyHostToDevice));
+  FXC_TRY(xc_grow(&p->d_tiles, &p->cap_tiles, p->h_tiles.size()));
+  FXC_TRY(xc_grow(&p->d_tile_rows, &p->cap_tile_rows, rows.size()));
+  FXC_TRY(xc_grow(&p->d_row_task, &p->cap_rows, row_task.size()));
+  auto up = [&](void *dst, const void *src, size_t bytes) -> int {
+    if (stream)
+      FXC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    else
+      FXC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return FXC_OK;
+  };
+  FXC_TRY(up(p->d_tiles, p->h_tiles.data(), p->h_tiles.size() * sizeof(TileDesc)));
+  FXC_TRY(up(p->d_tile_rows, rows.data(), rows.size() * sizeof(int)));
+  FXC_TRY(up(p->d_row_task, row_task.data(), row_task.size() * sizeof(int)));
   if (p->precision == FXC_PREC_PLAIN) {
-    std::vector<int> rs(row_task.size()), rr(row_task.size());
+    std::vector<int> &rs = p->h_row_src, &rr = p->h_row_rec;
+    rs.resize(row_task.size());
+    rr.resize(row_task.size());
     for (size_t i = 0; i < row_task.size(); ++i) {
       rs[i] = h_src[row_task[i]];
       rr[i] = h_rec[row_task[i]];
     }
-    FXC_TRY(dmalloc((void **)&p->d_row_src, rs.size() * sizeof(int)));
-    FXC_TRY(dmalloc((void **)&p->d_row_rec, rr.size() * sizeof(int)));
-    FXC_HIP(hipMemcpy(p->d_row_src, rs.data(), rs.size() * sizeof(int), hipMemcpyHostToDevice));
-    FXC_HIP(hipMemcpy(p->d_row_rec, rr.data(), rr.size() * sizeof(int), hipMemcpyHostToDevice));
+    FXC_TRY(xc_grow(&p->d_row_src, &p->cap_row_src, rs.size()));
+    FXC_TRY(xc_grow(&p->d_row_rec, &p->cap_row_rec, rr.size()));
+    FXC_TRY(up(p->d_row_src, rs.data(), rs.size() * sizeof(int)));
+    FXC_TRY(up(p->d_row_rec, rr.data(), rr.size() * sizeof(int)));
   }
+  p->npairs = npairs;  // only a complete task list is ever used
   return FXC_OK;
+}
+
+extern "C" int fxc_xc_set_pairs(fxc_xc_plan *p, int nsta, const int *h_src, const int *h_rec,
+                                long long npairs) {
+  return xc_set_pairs_on(p, nsta, h_src, h_rec, npairs, nullptr);
 }
 
 extern "C" int fxc_xc_pairs_info(const fxc_xc_plan *p, long long *ntiles, long long *nchunks,

@@ -277,3 +277,89 @@ def linear_stack(records, offsets, out=None):
     check(lib().fxc_linear_stack(records.data_ptr(), off.ctypes.data, npairs, n, out.data_ptr(),
                                  _stream_ptr(records.device)), "fxc_linear_stack")
     return out
+
+
+class StreamEngine:
+    """Serving loop with device-resident per-pair stacks (fxc_stream_*).
+
+    cfg: the S2sPlan arguments of one segment (npts = one segment's samples);
+    channels are ids 0..nchan-1; every channel pair (a < b) of a segment is
+    correlated and added to that pair's float64 linear accumulator (and, with
+    history > 0, its last `history` correlations for PWS / TF-PWS)."""
+
+    KINDS = {"linear": 0, "pws": 1, "tfpws": 2}
+
+    def __init__(self, npts, seg, shift, nfft_filter, nfft_out, num_ch, delta, freq_low, idx,
+                 responses, freq_lows, nchan, half_cc, precision="bf16x3", whiten=2,
+                 normalize=3, phase_only=False, history=0, max_pairs=1 << 20, max_seg_chan=512,
+                 lanes=2):
+        resp = np.ascontiguousarray(responses, dtype=np.float32)
+        lows = np.ascontiguousarray(freq_lows, dtype=np.float32)
+        self._keep = (resp, lows)
+        cfg = S2sConfig(npts, seg, shift, nfft_filter, nfft_out, num_ch, float(delta),
+                        float(freq_low), (ctypes.c_int * 4)(*[int(v) for v in idx]), int(whiten),
+                        int(normalize), int(bool(phase_only)), resp.shape[0], resp.ctypes.data,
+                        lows.ctypes.data)
+        self._h = ctypes.c_void_p()
+        check(lib().fxc_stream_create(ctypes.byref(self._h), ctypes.byref(cfg), nchan, half_cc,
+                                      PRECISIONS[precision], history, max_pairs, max_seg_chan,
+                                      lanes), "fxc_stream_create")
+        self.npts, self.nchan = npts, nchan
+        self.ncc = 2 * min(half_cc, nfft_out // 2) + 1
+        self._inflight = {}
+
+    def submit(self, chan, traces) -> int:
+        """chan: channel ids (int); traces: float32 [len(chan), npts] host array,
+        kept alive here until the ticket is waited for."""
+        ch = np.ascontiguousarray(chan, dtype=np.int32)
+        tr = np.ascontiguousarray(traces, dtype=np.float32)
+        if tr.shape != (len(ch), self.npts):
+            raise ValueError("traces must be [len(chan), npts]")
+        t = ctypes.c_longlong()
+        check(lib().fxc_stream_submit(self._h, ch.ctypes.data, len(ch), tr.ctypes.data,
+                                      ctypes.byref(t)), "fxc_stream_submit")
+        self._inflight[t.value] = tr
+        return t.value
+
+    def wait(self, ticket) -> float:
+        """Blocks until the segment's update is applied; returns its latency (ms)."""
+        ms = ctypes.c_double()
+        try:
+            check(lib().fxc_stream_wait(self._h, ticket, ctypes.byref(ms)), "fxc_stream_wait")
+        finally:
+            self._inflight.pop(ticket, None)
+        return ms.value
+
+    def info(self):
+        a, b, c, d = (ctypes.c_longlong() for _ in range(4))
+        check(lib().fxc_stream_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                    ctypes.byref(d)), "fxc_stream_info")
+        return {"pairs_used": a.value, "segments_done": b.value, "state_bytes": c.value,
+                "lane_bytes": d.value}
+
+    def read(self, src, rec, kind="linear", substack=1, band=None, device="cuda:0"):
+        """Stacks of pairs (src[i] < rec[i]) after every submitted segment:
+        float32 [npairs, 2H+1] device tensor and the record counts."""
+        s = np.ascontiguousarray(src, dtype=np.int32)
+        r = np.ascontiguousarray(rec, dtype=np.int32)
+        out = torch.empty((len(s), self.ncc), dtype=torch.float32, device=device)
+        cnt = np.zeros(len(s), dtype=np.int64)
+        bl = band is not None
+        fmin, fmax, taper = band if bl else (0.0, 0.0, 0.0)
+        check(lib().fxc_stream_read(self._h, self.KINDS[kind], s.ctypes.data, r.ctypes.data,
+                                    len(s), int(substack), int(bl), float(fmin), float(fmax),
+                                    float(taper), out.data_ptr(), cnt.ctypes.data,
+                                    _stream_ptr(out.device)), "fxc_stream_read")
+        return out, cnt
+
+    def close(self):
+        if self._h:
+            lib().fxc_stream_destroy(self._h)
+            self._h = ctypes.c_void_p()
+        self._inflight.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -157,6 +157,45 @@ int fxc_tfpws(const float *d_prestack, const float *d_weights, const long long *
 int fxc_linear_stack(const float *d_records, const long long *h_offsets, int npairs, int n,
                      float *d_out, void *stream);
 
+/* ------------------------------------------------------------ streaming --
+ * Serving loop over the stages above, keeping per-pair stacks in HBM.  The
+ * reference has no counterpart (it stacks offline: the linear stack of
+ * fastxc/operators/stacking/linear_stack.py:228-247 and ncf_pws / ncf_tfpws
+ * over stored correlations); the per-segment work is fxc_s2s_process and
+ * fxc_xc_correlate, and the state update is the linear stack's float64
+ * running sum.
+ *
+ * A segment is the traces [n][cfg.npts] (host float) of n channels (ids
+ * 0..nchan-1, in groups of cfg.num_ch); every channel pair (a, b), a < b by
+ * id, is correlated and added to that pair's state: a float64 accumulator of
+ * the 2H+1 lags, a record count, and (history > 0) the last `history`
+ * correlations.  `lanes` segments are processed concurrently, each lane with
+ * its own stream and workspaces for up to max_seg_chan channels, allocated
+ * at creation; state updates are applied one segment at a time.  At most
+ * max_pairs pairs have state; a segment with more new pairs than free slots
+ * is refused (FXC_ERR_ALLOC from fxc_stream_wait).  history must be 0 or at
+ * least `lanes`. */
+typedef struct fxc_stream fxc_stream;
+int fxc_stream_create(fxc_stream **stream, const fxc_s2s_config *cfg, int nchan, int half_cc,
+                      int precision, int history, long long max_pairs, int max_seg_chan,
+                      int lanes);
+/* Queues a segment; h_traces must stay valid until its wait returns.  The
+ * latency fxc_stream_wait reports runs from this call to the state update. */
+int fxc_stream_submit(fxc_stream *stream, const int *h_chan, int n, const float *h_traces,
+                      long long *ticket);
+int fxc_stream_wait(fxc_stream *stream, long long ticket, double *latency_ms);
+int fxc_stream_info(fxc_stream *stream, long long *pairs_used, long long *segments_done,
+                    long long *state_bytes, long long *lane_bytes);
+/* After every submitted segment is applied: kind 0 the linear stack
+ * (float)(acc / count), 1 PWS and 2 TF-PWS of the history in submission
+ * order with sub-stacks of `substack` records (band arguments as fxc_tfpws).
+ * Pairs are channel ids src < rec; d_out [npairs][2H+1]; h_count (optional)
+ * receives each pair's record count. */
+int fxc_stream_read(fxc_stream *stream, int kind, const int *h_src, const int *h_rec,
+                    long long npairs, int substack, int band_limited, float fmin, float fmax,
+                    float taper_hz, float *d_out, long long *h_count, void *hip_stream);
+int fxc_stream_destroy(fxc_stream *stream);
+
 #ifdef __cplusplus
 }
 #endif
