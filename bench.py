@@ -17,6 +17,9 @@ The other stages of the path are measured after xc, each at the scale of
 the same 1000-station day, under "stages" in the same JSON line: sac2spec (1000 stations x 24
 windows), PWS and the float64 linear stack (49950 pairs x 30 timestamps),
 TF-PWS (2000 pairs); --no-stages skips them, --stages-only runs only them.
+The streaming loop (one-hour segments of random 10-500-station subnetworks,
+Poisson arrivals at a sweep of rates) reports p50/p99 segment-update latency
+and pair-windows/s under "stream"; --no-stream skips it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision ...]
 """
@@ -58,6 +61,13 @@ def parse():
     ap.add_argument("--no-stages", action="store_true",
                     help="skip the sac2spec / stack stage measurements")
     ap.add_argument("--stages-only", action="store_true", help="measure the stages, not xc")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip the streaming segment-update measurement")
+    ap.add_argument("--stream-only", action="store_true", help="measure the streaming loop only")
+    ap.add_argument("--stream-rates", default="16,64,128,192",
+                    help="Poisson arrival rates (segments/s) of the streaming sweep")
+    ap.add_argument("--stream-seconds", type=float, default=3.0,
+                    help="arrival time per rate of the streaming sweep")
     return ap.parse_args()
 
 
@@ -375,6 +385,66 @@ def stages(reps=3):
     return out
 
 
+# ----------------------------------------------------------------- stream --
+# Streaming segment updates (fxc_stream_*): one-hour segments at dt 0.1 s for
+# random subnetworks of 10-500 stations of a 2000-station array (Z
+# channels), Poisson arrivals at each rate of a sweep; every segment runs
+# sac2spec, xc of all its pairs (split-BF16) and the per-pair float64 linear
+# stack update with a 4-record correlation history.  Latency runs from
+# submit to the applied state update; pair-windows/s is the work of the
+# segments of one rate over the time from its first submit to its last
+# completion.  Traces come from a seeded pool (contiguous row blocks of a
+# random station permutation, so submitting copies nothing).
+def stream_sweep(rates, seconds, seed=5):
+    import numpy as np
+    from fastxc_amd.ops import StreamEngine
+
+    nsta, npts, seg, nfft_f, nfft_o, half = 2000, 36000, 36000, 36288, 37044, 1000
+    resp = _butter_power(nfft_f, 0.1, 0.1, 2.0)[None]
+    rng = np.random.default_rng(seed)
+    print("# stream: trace pool", file=sys.stderr, flush=True)
+    pool = rng.standard_normal((nsta, npts), dtype=np.float32)
+    perm = rng.permutation(nsta).astype(np.int32)
+    t0 = time.perf_counter()
+    eng = StreamEngine(npts, seg, seg, nfft_f, nfft_o, 1, 0.1, 0.1, [240, 360, 7200, 9597], resp,
+                       [0.1], nsta, half, precision="bf16x3", history=4,
+                       max_pairs=nsta * (nsta - 1) // 2, max_seg_chan=500, lanes=4)
+    setup_s = time.perf_counter() - t0
+    info0 = eng.info()
+    out = []
+    for rate in rates:
+        print(f"# stream: rate {rate}/s", file=sys.stderr, flush=True)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=max(1, int(rate * seconds))))
+        sizes = rng.integers(10, 501, size=len(arrivals))
+        tickets, pairs = [], 0
+        start = time.perf_counter()
+        for at, n in zip(arrivals, sizes):
+            delay = start + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            a = int(rng.integers(0, nsta - n + 1))
+            tickets.append(eng.submit(perm[a:a + n], pool[a:a + n]))
+            pairs += int(n) * (int(n) - 1) // 2
+        lat = np.array([eng.wait(t) for t in tickets])
+        wall = time.perf_counter() - start
+        out.append({"rate_per_s": rate, "segments": len(tickets),
+                    "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                    "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                    "max_ms": round(float(lat.max()), 2),
+                    "pair_windows_per_s": round(pairs / wall, 1)})
+    info = eng.info()
+    eng.close()
+    ok = [r["rate_per_s"] for r in out if r["p99_ms"] < 1000.0]
+    return {"workload": "one-hour segments (dt 0.1 s, nfft 37044, 2001 lags), random 10-500 "
+                        "station subnetworks of a 2000-station array, Z channels, Poisson arrivals",
+            "form": "bf16x3", "lanes": 4, "history": 4, "setup_s": round(setup_s, 2),
+            "state_gb": round(info0["state_bytes"] / 1e9, 1),
+            "lane_gb": round(info0["lane_bytes"] / 1e9, 2), "pairs_with_state": info["pairs_used"],
+            "rates": out, "rated_rate_per_s": max(ok) if ok else None,
+            "p99_ms_at_rated": next((r["p99_ms"] for r in out if ok and r["rate_per_s"] == max(ok)),
+                                    None)}
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -386,8 +456,12 @@ def main():
 
     if args.calibrate_copy_gib > 0:
         calibrate_copy(args.calibrate_copy_gib)
+    rates = [float(x) for x in args.stream_rates.split(",") if x]
     if args.stages_only:
         print(json.dumps({"stages": stages()}), flush=True)
+        return
+    if args.stream_only:
+        print(json.dumps({"stream": stream_sweep(rates, args.stream_seconds)}), flush=True)
         return
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev)
@@ -439,6 +513,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_stages:
         res["stages"] = stages()
+    if not args.no_stream:
+        res["stream"] = stream_sweep(rates, args.stream_seconds)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(res), flush=True)
