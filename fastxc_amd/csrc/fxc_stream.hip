@@ -16,9 +16,10 @@
 // at creation, so the steady state allocates nothing and never synchronises
 // the device).  Segments that arrive while others are on the GPU are
 // admitted as lanes free up, and their transfers and kernels overlap.  The
-// state updates of all lanes are ordered on one update stream, so a pair is
-// never updated by two segments at once; pair slots and history positions
-// are assigned in submission order.  A segment whose new pairs do not fit
+// state updates of all lanes are enqueued on one update stream in admission
+// order, so a pair is never updated by two segments at once, each history
+// position holds the newest record assigned to it, and the float64 sums add
+// in the order a sequential run adds them.  A segment whose new pairs do not fit
 // the state pool is refused (its ticket reports FXC_ERR_ALLOC).
 //
 // Read-out: the linear stack is acc / count rounded once to float (the
@@ -127,6 +128,7 @@ using Clock = std::chrono::steady_clock;
 
 struct StreamJob {
   long long ticket = 0;
+  long long seq = -1;  // admission order of an admitted segment
   std::vector<int> chan;
   const float *h_traces = nullptr;
   Clock::time_point t_submit;
@@ -169,7 +171,8 @@ struct fxc_stream {
   std::mutex upd_mu;
   // queue
   std::mutex mu;
-  std::condition_variable cv_q, cv_done;
+  std::condition_variable cv_q, cv_done, cv_seq;
+  long long admitted = 0, next_update = 0;  // updates are applied in admission order
   std::deque<StreamJob> queue;
   std::map<long long, StreamResult> results;
   long long next_ticket = 1, submitted = 0, finished = 0;
@@ -199,14 +202,38 @@ int lane_run(fxc_stream *s, StreamLane &L, const StreamJob &job, const std::vect
   FXC_HIP(hipMemcpyAsync(L.d_slot, L.h_slot, (size_t)np * sizeof(int), hipMemcpyHostToDevice, L.st));
   FXC_HIP(hipMemcpyAsync(L.d_hpos, L.h_hpos, (size_t)np * sizeof(int), hipMemcpyHostToDevice, L.st));
   FXC_HIP(hipEventRecord(L.ev_cc, L.st));
+  // State updates are enqueued in admission order: a history position is
+  // count mod history, so a record may only land after every earlier record
+  // of its pair (a slow lane could otherwise still hold record r when record
+  // r + history lands), and the float64 sums add in the order of a
+  // sequential run.  Only this small update waits; the transforms above
+  // overlap freely.
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_seq.wait(lk, [&] { return s->next_update == job.seq; });
+  }
+  int rc = FXC_OK;
   {
     std::lock_guard<std::mutex> g(s->upd_mu);
-    FXC_HIP(hipStreamWaitEvent(s->upd, L.ev_cc, 0));
-    hipLaunchKernelGGL(stream_update_kernel, row_grid(s->ncc, np), dim3(kStrThreads), 0, s->upd,
-                       L.d_cc, s->ncc, np, L.d_slot, L.d_hpos, s->d_acc, s->d_hist, s->history);
-    FXC_HIP(hipGetLastError());
-    FXC_HIP(hipEventRecord(L.ev_done, s->upd));
+    if (hipStreamWaitEvent(s->upd, L.ev_cc, 0) != hipSuccess) {
+      set_error("stream: hipStreamWaitEvent failed");
+      rc = FXC_ERR_HIP;
+    } else {
+      hipLaunchKernelGGL(stream_update_kernel, row_grid(s->ncc, np), dim3(kStrThreads), 0,
+                         s->upd, L.d_cc, s->ncc, np, L.d_slot, L.d_hpos, s->d_acc, s->d_hist,
+                         s->history);
+      if (hipGetLastError() != hipSuccess || hipEventRecord(L.ev_done, s->upd) != hipSuccess) {
+        set_error("stream: update launch failed");
+        rc = FXC_ERR_HIP;
+      }
+    }
   }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    ++s->next_update;
+  }
+  s->cv_seq.notify_all();
+  FXC_TRY(rc);
   FXC_HIP(hipEventSynchronize(L.ev_done));
   return FXC_OK;
 }
@@ -265,11 +292,19 @@ void lane_main(fxc_stream *s, StreamLane *L) {
           hpos[p] = s->history > 0 ? (int)(s->count[sl] % s->history) : 0;
           ++s->count[sl];
         }
+        job.seq = s->admitted++;
       }
     }
     if (status == FXC_OK) {
       status = lane_run(s, *L, job, src, rec, slots, hpos);
       if (status != FXC_OK) {
+        // a lane that failed before its update turn still passes the turn on
+        {
+          std::unique_lock<std::mutex> lk(s->mu);
+          s->cv_seq.wait(lk, [&] { return s->next_update >= job.seq; });
+          if (s->next_update == job.seq) ++s->next_update;
+        }
+        s->cv_seq.notify_all();
         msg = last_error();
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->broken.empty()) s->broken = "stream: segment " + std::to_string(job.ticket) +

@@ -7,8 +7,8 @@ correlations.  Checked against:
   - the FP64 oracle: sac2spec + xc of each segment, float64 mean over the
     segments a pair occurs in (relative L2 <= 1e-5 per pair, FP32 form);
   - the same engine with one lane (segments one at a time): results in the
-    mix equal the results alone (relative L2 <= 1e-6; only the float64
-    summation order may differ);
+    mix equal the results alone exactly (state updates are applied in
+    admission order, so the float64 sums add in the same order);
   - PWS / TF-PWS of the history against fxc_pws / fxc_tfpws on the same
     correlations prepared on the host as the reference does (1e-5 / 1e-4).
 """
@@ -97,7 +97,7 @@ def test_linear_stack_matches_oracle_and_mix_equals_alone(gpu, tmp_path):
     _run(alone, segs)
     got1, cnt1 = alone.read(src, rec, "linear")
     assert np.array_equal(cnt, cnt1)
-    assert rel_l2(got, got1.cpu().numpy()) < 1e-6
+    assert np.array_equal(got, got1.cpu().numpy())
     mix.close()
     alone.close()
 
@@ -109,7 +109,9 @@ def test_history_stacks_match_batch_kernels(gpu, tmp_path, kind, B, tol):
 
     plan = _plan(tmp_path)
     segs = _segments(2, 16)
-    eng = _engine(plan, lanes=3, history=4)
+    # 4 lanes and a 4-record ring: a slow lane must not let a newer record
+    # of a pair land before an older one in the same ring slot
+    eng = _engine(plan, lanes=4, history=4)
     _run(eng, segs)
     # the same correlations computed segment by segment through the batch
     # entry points are the records the reference's stacks would read
