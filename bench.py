@@ -19,7 +19,11 @@ windows), PWS and the float64 linear stack (49950 pairs x 30 timestamps),
 TF-PWS (2000 pairs); --no-stages skips them, --stages-only runs only them.
 The streaming loop (one-hour segments of random 10-500-station subnetworks,
 Poisson arrivals at a sweep of rates) reports p50/p99 segment-update latency
-and pair-windows/s under "stream"; --no-stream skips it.
+and pair-windows/s under "stream"; --no-stream skips it.  The SourcePack
+indexer (host C++, bin/sourcepack_index) is timed on the xcpack TSV of the
+same timestamp (499500 records) under "sourcepack", beside the Python
+restatement of the reference builder on a 50000-record sample;
+--no-sourcepack skips it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision ...]
 """
@@ -64,6 +68,8 @@ def parse():
     ap.add_argument("--no-stream", action="store_true",
                     help="skip the streaming segment-update measurement")
     ap.add_argument("--stream-only", action="store_true", help="measure the streaming loop only")
+    ap.add_argument("--no-sourcepack", action="store_true",
+                    help="skip the SourcePack indexer measurement")
     ap.add_argument("--stream-rates", default="16,64,128,192",
                     help="Poisson arrival rates (segments/s) of the streaming sweep")
     ap.add_argument("--stream-seconds", type=float, default=3.0,
@@ -445,6 +451,72 @@ def stream_sweep(rates, seconds, seed=5):
                                     None)}
 
 
+# ------------------------------------------------------------- sourcepack --
+_XC_COLUMNS = ("timestamp worker_id anchor_block target_begin_block target_end_block block_size "
+               "anchor_begin anchor_end target_begin target_end pack_path offset bytes src_id "
+               "rec_id src_network src_station src_location src_component rec_network "
+               "rec_station rec_location rec_component npts dt dist az baz "
+               "final_pair_path").split()
+
+
+def _write_xcpack(root, nsta, lags=2001, ts="20200502T04:00", limit=None):
+    """The xcpack TSV xc_fast writes for one timestamp of nsta stations (Z),
+    all pairs in task order, over a sparse pack file of the right size."""
+    x = os.path.join(root, "ncf", "xcpack")
+    os.makedirs(x, exist_ok=True)
+    stem = ts + ".i000000.j000000.w000.p000"
+    rb = 632 + 4 * lags
+    lines, n = ["\t".join(_XC_COLUMNS)], 0
+    for a in range(1, nsta + 1):
+        for b in range(a + 1, nsta + 1):
+            if limit is not None and n == limit:
+                break
+            sa, sb = "S%04d" % a, "S%04d" % b
+            lines.append(f"{ts}\t0\t0\t0\t0\t{nsta}\t0\t{nsta}\t0\t{nsta}\t{stem}.xcpack\t"
+                         f"{n * rb}\t{rb}\t{a}\t{b}\tXX\t{sa}\t\tBHZ\tXX\t{sb}\t\tBHZ\t{lags}\t0.1\t"
+                         f"{0.1 * (a + b):.1f}\t{a % 360}\t{b % 360}\t/out/ncf/XX.{sa}/XX.{sb}/"
+                         f"XX-XX.{sa}-{sb}.BHZ-BHZ.sac")
+            n += 1
+    with open(os.path.join(x, stem + ".tsv"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    with open(os.path.join(x, stem + ".xcpack"), "wb") as fh:
+        fh.truncate(n * rb)
+    with open(os.path.join(x, ts + ".done"), "w") as fh:
+        fh.write("DONE\n")
+    return x, n
+
+
+def sourcepack_measure(nsta=1000, sample=50000):
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = os.path.join(ROOT, "bin", "sourcepack_index")
+    tmp = tempfile.mkdtemp(prefix="fxc_sourcepack_")
+    try:
+        x, n = _write_xcpack(os.path.join(tmp, "full"), nsta)
+        t0 = time.perf_counter()
+        r = subprocess.run([exe, "-I", x, "-O", os.path.join(tmp, "full", "sp")],
+                           capture_output=True, text=True)
+        sec = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise RuntimeError("sourcepack_index failed: " + r.stderr[-500:])
+        xs, ns = _write_xcpack(os.path.join(tmp, "sample"), nsta, limit=sample)
+        from oracle import sourcepack as osp
+
+        t0 = time.perf_counter()
+        osp.build_all(xs, os.path.join(tmp, "sample", "sp"))
+        port = time.perf_counter() - t0
+        return {"records": n, "value": round(n / sec, 1), "unit": "records/s",
+                "sec": round(sec, 3), "threads": 1,
+                "cpu_port": {"value": round(ns / port, 1), "unit": "records/s", "cores": 1,
+                             "kind": "port", "sample": f"{ns} records, same TSV layout"},
+                "detail": "one timestamp of 1000 stations (all pairs), file to file "
+                          "(read xcpack TSV, sort, write index + _SUCCESS)"}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -515,6 +587,9 @@ def main():
         res["stages"] = stages()
     if not args.no_stream:
         res["stream"] = stream_sweep(rates, args.stream_seconds)
+    if not args.no_sourcepack:
+        print("# sourcepack indexer", file=sys.stderr, flush=True)
+        res["sourcepack"] = sourcepack_measure()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(res), flush=True)

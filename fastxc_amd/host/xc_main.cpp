@@ -11,7 +11,10 @@
 // Extensions: --precision fp32|bf16x3|fp16|plain (default fp32, the
 // reference's arithmetic, or the FASTXC_XC_PRECISION environment value),
 // --block-files n (block size; the reference derives it from device memory,
-// at most 4096) and --dry-run (print jobs and tasks, no GPU).
+// at most 4096), --dry-run (print jobs and tasks, no GPU) and
+// --sourcepack-dir d (each finished timestamp's SourcePack index built on a
+// side thread, as the reference's Python materializer does by polling the
+// .done markers; fastxc/operators/sourcepack/builder.py:26-160).
 //
 // Per timestamp: the StepPack payload [nstep][rows][nspec] is read into
 // pinned memory and uploaded once; each job's tasks go to fxc_xc_correlate
@@ -19,6 +22,7 @@
 // writer thread turns them into xcpack records while the GPU runs the next
 // job or timestamp.
 #include "fxc_host.hpp"
+#include "sourcepack.hpp"
 #include "stepack.hpp"
 #include "xc_tasks.hpp"
 
@@ -49,6 +53,8 @@ constexpr uint64_t kMaxPackBytes = 4ull << 30;  // xcpack part size
 
 struct Args {
   std::string input, paths, out, progress, gpu_ids = "0", mem_caps;
+  std::string sourcepack_dir, sourcepack_progress;
+  bool sourcepack_sort = true;
   float cclength = 0.f;
   int lazy_depth = 1;
   size_t block_files = 0;  // 0: min(rows, 4096)
@@ -75,6 +81,11 @@ void usage() {
           "  --dry-run            Print the jobs and tasks of every timestamp and exit\n"
           "  --debug              Write synchronously\n"
           "  --progress <file>    Write progress sidecar TSV\n\n"
+          "Source index:\n"
+          "  --sourcepack-dir <d> Build <d>/<timestamp>/sourcepack_index.tsv as each\n"
+          "                       timestamp is done, and <d>/_SUCCESS at the end\n"
+          "  --sourcepack-progress <file>  Progress sidecar of the index build\n"
+          "  --sourcepack-keep-order       Keep the xcpack order inside each index\n\n"
           "Logging:\n"
           "  FASTXC_LOG_LEVEL=ERROR|WARN|INFO|DEBUG (default INFO)\n\n"
           "Output:\n"
@@ -137,6 +148,14 @@ bool parse_args(int argc, char **argv, Args *a, int *exit_code) {
     } else if (o == "--block-files") {
       if (!(v = value("--block-files"))) break;
       a->block_files = (size_t)atoll(v);
+    } else if (o == "--sourcepack-dir") {
+      if (!(v = value("--sourcepack-dir"))) break;
+      a->sourcepack_dir = v;
+    } else if (o == "--sourcepack-progress") {
+      if (!(v = value("--sourcepack-progress"))) break;
+      a->sourcepack_progress = v;
+    } else if (o == "--sourcepack-keep-order") {
+      a->sourcepack_sort = false;
     } else if (o == "--debug") {
       a->debug = true;
     } else if (o == "--dry-run") {
@@ -477,6 +496,14 @@ int run(const Args &a) {
     FXH_LOG_ERROR("xcpack_dir_create_failed", "path=\"%s\"", pack_root.c_str());
     return 1;
   }
+  // indexes of finished timestamps are built on a side thread while the
+  // next timestamp correlates (the reference polls the .done markers)
+  std::unique_ptr<SourcePackMaterializer> sp;
+  if (!a.sourcepack_dir.empty()) {
+    sp.reset(new SourcePackMaterializer(a.out, a.sourcepack_dir, a.sourcepack_sort,
+                                        a.sourcepack_progress));
+    sp->start();
+  }
   Progress prog;
   prog.init(a.progress.empty() ? nullptr : a.progress.c_str());
   prog.set_rows({{"overall", "RUNNING", 0, inputs.size(), "timestamps", ""},
@@ -660,6 +687,7 @@ int run(const Args &a) {
       rc = 1;
       break;
     }
+    if (sp) sp->submit(work->timestamp);
     prog.update("current", "DONE", units, units, "xc_units", work->timestamp);
     prog.update("overall", ti + 1 == inputs.size() ? "DONE" : "RUNNING", ti + 1, inputs.size(),
                 "timestamps", work->timestamp);
@@ -671,6 +699,8 @@ int run(const Args &a) {
     FXH_LOG_ERROR("xcpack_success_marker_failed", "output=\"%s\"", a.out.c_str());
     rc = 1;
   }
+  if (sp && !sp->finish(rc == 0, "upstream XC did not complete; global success marker withheld"))
+    rc = 1;
   prog.finish(rc == 0 ? "DONE" : "FAILED", rc == 0);
   FXH_LOG_INFO("run_done", "timestamps=%zu", inputs.size());
   return rc;

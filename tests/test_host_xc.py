@@ -159,3 +159,29 @@ def test_file_to_file_matches_oracle(gpu, tmp_path, precision, tol):
         assert hd["stla"] == pytest.approx(q["stla"]) and hd["evla"] == pytest.approx(s["stla"])
         assert (hd["nzyear"], hd["nzjday"], hd["nzhour"], hd["nzmin"]) == (2020, 123, 4, 0)
         assert hd["nvhdr"] == 6 and hd["iftype"] == 1 and hd["leven"] == 1
+
+
+@pytest.mark.gpu
+def test_sourcepack_dir_indexes_each_timestamp(gpu, tmp_path):
+    """--sourcepack-dir: the index built beside the run equals the restated
+    reference builder over the finished xcpack output, byte for byte, and the
+    global _SUCCESS / progress sidecar follow the materializer's finish()."""
+    from oracle import sourcepack as osp
+
+    sp_dir, paths, nodes, spec, allowed = make_inputs(str(tmp_path), nsta=5, nch=3, nstep=2,
+                                                      nspec=257)
+    out = tmp_path / "ncf"
+    spdir = tmp_path / "sourcepack"
+    prog = tmp_path / "progress" / "sourcepack_progress.tsv"
+    r = run(["-I", sp_dir, "-P", paths, "-O", str(out), "-C", "10", "--sourcepack-dir",
+             str(spdir), "--sourcepack-progress", str(prog)])
+    print(r.stderr[-2000:])
+    assert r.returncode == 0
+    tot = osp.build_all(str(out / "xcpack"), str(tmp_path / "oracle"))
+    from tests.test_host_sourcepack import tree
+    got, want = tree(spdir), tree(tmp_path / "oracle")
+    assert got.pop("_SUCCESS") == ("timestamps\t1\nrecords\t%d\nsources\t%d\nbytes_indexed\t%d\n"
+                                   % (tot["records"], tot["sources"], tot["bytes"])).encode()
+    assert got == want
+    assert tot["records"] == len(ff.read_xcpack_dir(str(out / "xcpack"))) > 0
+    assert "overall\tDONE\t1\t" in prog.read_text()
