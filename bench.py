@@ -16,7 +16,8 @@ timed on a bounded station subset.
 The other stages of the path are measured after xc, each at the scale of
 the same 1000-station day, under "stages" in the same JSON line: sac2spec (1000 stations x 24
 windows), PWS and the float64 linear stack (49950 pairs x 30 timestamps),
-TF-PWS (2000 pairs); --no-stages skips them, --stages-only runs only them.
+TF-PWS (2000 pairs), the ENZ -> RTZ rotation (49950 pairs x 9 components);
+--no-stages skips them, --stages-only runs only them.
 The streaming loop (one-hour segments of random 10-500-station subnetworks,
 Poisson arrivals at a sweep of rates) reports p50/p99 segment-update latency
 and pair-windows/s under "stream"; --no-stream skips it.  The SourcePack
@@ -65,6 +66,9 @@ def parse():
     ap.add_argument("--no-stages", action="store_true",
                     help="skip the sac2spec / stack stage measurements")
     ap.add_argument("--stages-only", action="store_true", help="measure the stages, not xc")
+    ap.add_argument("--stage", default="",
+                    help="comma list of stages to measure (sac2spec,pws,tfpws,linear_stack,"
+                         "rotate); default all")
     ap.add_argument("--no-stream", action="store_true",
                     help="skip the streaming segment-update measurement")
     ap.add_argument("--stream-only", action="store_true", help="measure the streaming loop only")
@@ -380,12 +384,40 @@ def stage_linear(dev, reps):
     return res
 
 
-def stages(reps=3):
+def stage_rotate(dev, reps):
+    """49950 station pairs x 9 components x 2001 lags, ENZ -> RTZ."""
+    import numpy as np
+    import torch
+    from fastxc_amd import ops
+
+    npairs, n = 49950, 2001
+    g = torch.Generator(device=dev)
+    g.manual_seed(23)
+    x = torch.randn((npairs, 9, n), dtype=torch.float32, device=dev, generator=g)
+    ang = np.deg2rad(np.random.default_rng(23).uniform(0, 360, (npairs, 2))).astype(np.float32)
+    trig = torch.from_numpy(np.stack([np.sin(ang[:, 0]), np.cos(ang[:, 0]), np.sin(ang[:, 1]),
+                                      np.cos(ang[:, 1])], axis=1)).to(dev)
+    out = torch.empty_like(x)
+    sec = _time_calls(lambda: ops.rotate_rtz(x, trig, out=out), max(reps, 20))
+    res = _stage("rotate", npairs, "pairs/s", sec, npairs * (72 * n + 16), "hbm", HBM_PEAK_GBS,
+                 "GB/s", {"pairs": npairs, "components": 9, "n": n})
+    del x, out
+    torch.cuda.empty_cache()
+    return res
+
+
+_STAGES = {"sac2spec": "stage_sac2spec", "pws": "stage_pws", "tfpws": "stage_tfpws",
+           "linear_stack": "stage_linear", "rotate": "stage_rotate"}
+
+
+def stages(reps=3, only=""):
     import torch
 
     dev = torch.device("cuda:0")
     out = []
-    for fn in (stage_sac2spec, stage_pws, stage_tfpws, stage_linear):
+    names = [x for x in only.split(",") if x] or list(_STAGES)
+    for name in names:
+        fn = globals()[_STAGES[name]]
         print(f"# stage {fn.__name__}", file=sys.stderr, flush=True)
         out.append(fn(dev, reps))
     return out
@@ -530,7 +562,7 @@ def main():
         calibrate_copy(args.calibrate_copy_gib)
     rates = [float(x) for x in args.stream_rates.split(",") if x]
     if args.stages_only:
-        print(json.dumps({"stages": stages()}), flush=True)
+        print(json.dumps({"stages": stages(only=args.stage)}), flush=True)
         return
     if args.stream_only:
         print(json.dumps({"stream": stream_sweep(rates, args.stream_seconds)}), flush=True)
@@ -584,7 +616,7 @@ def main():
     del spec, out
     torch.cuda.empty_cache()
     if not args.no_stages:
-        res["stages"] = stages()
+        res["stages"] = stages(only=args.stage)
     if not args.no_stream:
         res["stream"] = stream_sweep(rates, args.stream_seconds)
     if not args.no_sourcepack:

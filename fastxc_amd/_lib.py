@@ -61,6 +61,7 @@ _SIGNATURES = {
     "fxc_tfpws": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_float,
                           c_float, c_float, c_float, c_void_p, c_void_p]),
     "fxc_linear_stack": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "fxc_rotate_rtz": (c_int, [c_void_p, c_void_p, c_longlong, c_int, c_void_p, c_void_p]),
     "fxc_s2s_plan_create": (c_int, [POINTER(c_void_p), c_void_p]),
     "fxc_s2s_plan_destroy": (c_int, [c_void_p]),
     "fxc_s2s_plan_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int),

@@ -28,13 +28,13 @@ from __future__ import annotations
 import csv
 import heapq
 import shutil
-import struct
-import sys
 from dataclasses import dataclass
 from pathlib import Path
 from typing import BinaryIO, Iterable, Iterator, Sequence
 
 import numpy as np
+
+from .sacrec import ITIME, SAC_HEADER_BYTES, SacHeader as _Header
 
 SOURCEPACK_INDEX_HEADER = [
     "timestamp", "path_id", "component_slot", "source_key", "receiver_key", "src_id", "rec_id",
@@ -48,13 +48,10 @@ _TEMPLATE_FIELDS = ("path_id", "component_slot", "source_key", "receiver_key", "
                     "src_network", "src_station", "src_location", "src_component", "rec_network",
                     "rec_station", "rec_location", "rec_component", "dist", "az", "baz")
 
-SAC_HEADER_BYTES = 632
-ITIME, IXY = 1, 4
 # word offsets in the 632-byte header (70 floats, then 40 ints)
 _F_DELTA, _F_USER0 = 0, 40
-_I_BASE = 70 * 4
 _I_NZ = range(0, 6)  # nzyear nzjday nzhour nzmin nzsec nzmsec
-_I_NVHDR, _I_NPTS, _I_IFTYPE = 6, 9, 15
+_I_NPTS, _I_IFTYPE = 9, 15
 
 # Rows staged per GPU call are bounded by bytes and by the kernel's pair limit.
 _BATCH_BYTES = 256 << 20
@@ -67,61 +64,6 @@ class StackResult:
     output_path: Path
     segment_count: int
     sample_count: int
-
-
-class _Header:
-    """A SAC header kept as bytes in its own byte order."""
-
-    def __init__(self, raw: bytes):
-        if len(raw) != SAC_HEADER_BYTES:
-            raise ValueError(f"SAC header must be {SAC_HEADER_BYTES} bytes, got {len(raw)}")
-        self.raw = bytearray(raw)
-        self.endian = _byte_order(self.raw)
-
-    def f(self, word: int) -> float:
-        return struct.unpack_from(self.endian + "f", self.raw, 4 * word)[0]
-
-    def set_f(self, word: int, v: float) -> None:
-        struct.pack_into(self.endian + "f", self.raw, 4 * word, float(v))
-
-    def i(self, k: int) -> int:
-        return struct.unpack_from(self.endian + "i", self.raw, _I_BASE + 4 * k)[0]
-
-    def set_i(self, k: int, v: int) -> None:
-        struct.pack_into(self.endian + "i", self.raw, _I_BASE + 4 * k, int(v))
-
-    @property
-    def npts(self) -> int:
-        return self.i(_I_NPTS)
-
-    @property
-    def data_count(self) -> int:
-        n = self.npts
-        if n <= 0:
-            raise ValueError(f"Invalid SAC npts: {n}")
-        return 2 * n if self.i(_I_IFTYPE) == IXY else n
-
-    @property
-    def dtype(self) -> np.dtype:
-        return np.dtype(self.endian + "f4")
-
-
-def _byte_order(raw: bytes) -> str:
-    """'<' or '>' from nvhdr (6 or 7), else from a plausible npts and delta."""
-    native = "<" if sys.byteorder == "little" else ">"
-    off_v = _I_BASE + 4 * _I_NVHDR
-    if struct.unpack_from("<i", raw, off_v)[0] in (6, 7):
-        return "<"
-    if struct.unpack_from(">i", raw, off_v)[0] in (6, 7):
-        return ">"
-    ok = {}
-    for e in "<>":
-        n = struct.unpack_from(e + "i", raw, _I_BASE + 4 * _I_NPTS)[0]
-        d = struct.unpack_from(e + "f", raw, 0)[0]
-        ok[e] = 0 < n < 1_000_000_000 and np.isfinite(d)
-    if ok["<"] != ok[">"]:
-        return "<" if ok["<"] else ">"
-    return native
 
 
 @dataclass(frozen=True)

@@ -279,6 +279,27 @@ def linear_stack(records, offsets, out=None):
     return out
 
 
+def rotate_rtz(enz, trig, out=None):
+    """ENZ -> RTZ rotation of every pair: enz [npairs, 9, n] (EE EN EZ NE NN NZ
+    ZE ZN ZZ), trig [npairs, 4] float (sin az, cos az, sin baz, cos baz) ->
+    [npairs, 9, n] (RR RT RZ TR TT TZ ZR ZT ZZ).  out may be enz."""
+    _require(enz, torch.float32, "enz")
+    _require(trig, torch.float32, "trig")
+    if enz.ndim != 3 or enz.shape[1] != 9 or enz.shape[0] < 1:
+        raise ValueError("enz must be [npairs, 9, n]")
+    npairs, _, n = enz.shape
+    if tuple(trig.shape) != (npairs, 4):
+        raise ValueError("trig must be [npairs, 4]")
+    if out is None:
+        out = torch.empty_like(enz)
+    _require(out, torch.float32, "out")
+    if tuple(out.shape) != tuple(enz.shape):
+        raise ValueError("out must be [npairs, 9, n]")
+    check(lib().fxc_rotate_rtz(enz.data_ptr(), trig.data_ptr(), npairs, n, out.data_ptr(),
+                               _stream_ptr(enz.device)), "fxc_rotate_rtz")
+    return out
+
+
 class StreamEngine:
     """Serving loop with device-resident per-pair stacks (fxc_stream_*).
 

@@ -157,6 +157,20 @@ int fxc_tfpws(const float *d_prestack, const float *d_weights, const long long *
 int fxc_linear_stack(const float *d_records, const long long *h_offsets, int npairs, int n,
                      float *d_out, void *stream);
 
+/* ------------------------------------------------------------- rotation --
+ * ENZ -> RTZ rotation of stacked pair correlations; replaces
+ * rotate_enz_to_rtz / build_rotate_matrix
+ * (fastxc/operators/rotation/python_rotate.py:34-104), applied per pair by
+ * rotate_sourcepack_stack (:264-390).  d_enz / d_rtz [npairs][9][n] float,
+ * components in the order EE EN EZ NE NN NZ ZE ZN ZZ -> RR RT RZ TR TT TZ ZR
+ * ZT ZZ; d_trig [npairs][4] float (sin az, cos az, sin baz, cos baz, each the
+ * float sine / cosine of the float radian angle), 16-byte aligned.  Each
+ * matrix entry is the float product the reference forms; each output is a
+ * fused multiply-add chain in input order.  d_rtz may equal d_enz.
+ * Asynchronous on the stream. */
+int fxc_rotate_rtz(const float *d_enz, const float *d_trig, long long npairs, int n,
+                   float *d_rtz, void *stream);
+
 /* ------------------------------------------------------------ streaming --
  * Serving loop over the stages above, keeping per-pair stacks in HBM.  The
  * reference has no counterpart (it stacks offline: the linear stack of

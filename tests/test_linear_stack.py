@@ -94,11 +94,13 @@ def test_output_naming():
 
 def test_byte_order_detection():
     le = bytes(ff.sac_null_bytes())
-    assert ls._byte_order(le) == "<"
+    from fastxc_amd.sacrec import byte_order
+
+    assert byte_order(le) == "<"
     f = struct.unpack("<70f", le[:280])
     i = struct.unpack("<40i", le[280:440])
     be = struct.pack(">70f", *f) + struct.pack(">40i", *i) + le[440:]
-    assert ls._byte_order(be) == ">"
+    assert byte_order(be) == ">"
 
 
 def test_missing_index_fields(tmp_path):
