@@ -24,7 +24,9 @@ and pair-windows/s under "stream"; --no-stream skips it.  The SourcePack
 indexer (host C++, bin/sourcepack_index) is timed on the xcpack TSV of the
 same timestamp (499500 records) under "sourcepack", beside the Python
 restatement of the reference builder on a 50000-record sample;
---no-sourcepack skips it.
+--no-sourcepack skips it.  The inventory path planner's all-pairs loop (host
+C++) is timed on 2000 stations under "planner", beside the oracle's Python
+loop on 200 of them; --no-planner skips it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision ...]
 """
@@ -74,6 +76,8 @@ def parse():
     ap.add_argument("--stream-only", action="store_true", help="measure the streaming loop only")
     ap.add_argument("--no-sourcepack", action="store_true",
                     help="skip the SourcePack indexer measurement")
+    ap.add_argument("--no-planner", action="store_true",
+                    help="skip the inventory path-planner measurement")
     ap.add_argument("--stream-rates", default="16,64,128,192",
                     help="Poisson arrival rates (segments/s) of the streaming sweep")
     ap.add_argument("--stream-seconds", type=float, default=3.0,
@@ -549,6 +553,39 @@ def sourcepack_measure(nsta=1000, sample=50000):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+# ---------------------------------------------------------------- planner --
+def planner_measure(nsta=2000, sample=200, seed=9):
+    """All station pairs (with autocorrelations) of nsta stations in a
+    15 x 20 degree box: the native planner (threads = this process's cores,
+    at most 16) against the plain Python loop of the oracle on the pairs of
+    the first `sample` stations."""
+    import numpy as np
+    from fastxc_amd.geodesy import distkm_az_baz_rudoe
+    from fastxc_amd.planner import GeoNode, PairFilterConfig, build_valid_paths
+    from oracle import planner as opl
+
+    rng = np.random.default_rng(seed)
+    nodes = [GeoNode(k + 1, "1", "XX", f"S{k:04d}", "00", "x",
+                     lat=float(np.float32(rng.uniform(30, 45))),
+                     lon=float(np.float32(rng.uniform(-120, -100)))) for k in range(nsta)]
+    threads = min(16, len(os.sched_getaffinity(0)))
+    t0 = time.perf_counter()
+    paths, checks = build_valid_paths(nodes, PairFilterConfig(), threads=threads)
+    sec = time.perf_counter() - t0
+    sub = [(n.lat, n.lon, n.group) for n in nodes[:sample]]
+    cfg = dict(allow_autocorr=True, autocorr_only=False, group_mode="all", use_distance=False,
+               min_km=0.0, max_km=50000.0, use_azimuth=False, ranges=((0.0, 360.0),))
+    t0 = time.perf_counter()
+    _, pchecks = opl.valid_paths(sub, cfg, distkm_az_baz_rudoe)
+    psec = time.perf_counter() - t0
+    return {"stations": nsta, "pairs": len(paths), "value": round(checks / sec, 1),
+            "unit": "pairs/s", "sec": round(sec, 3), "threads": threads,
+            "cpu_port": {"value": round(pchecks / psec, 1), "unit": "pairs/s", "cores": 1,
+                         "kind": "port", "sample": f"{pchecks} pairs of {sample} stations"},
+            "detail": "Rudoe geometry and filters of every station pair, float64 libm, "
+                      "bit-identical values"}
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -622,6 +659,9 @@ def main():
     if not args.no_sourcepack:
         print("# sourcepack indexer", file=sys.stderr, flush=True)
         res["sourcepack"] = sourcepack_measure()
+    if not args.no_planner:
+        print("# path planner", file=sys.stderr, flush=True)
+        res["planner"] = planner_measure()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(res), flush=True)

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_char_p, c_float, c_int, c_longlong, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_longlong, c_void_p
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfastxc_amd.so")
@@ -62,6 +62,13 @@ _SIGNATURES = {
                           c_float, c_float, c_float, c_void_p, c_void_p]),
     "fxc_linear_stack": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "fxc_rotate_rtz": (c_int, [c_void_p, c_void_p, c_longlong, c_int, c_void_p, c_void_p]),
+    "fxc_plan_paths": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                               c_int, c_double, c_double, c_int, c_void_p, c_int, c_int,
+                               POINTER(c_void_p), POINTER(c_longlong), POINTER(c_longlong)]),
+    "fxc_plan_paths_fetch": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fxc_plan_paths_write": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_char_p, c_char_p,
+                                     c_int]),
+    "fxc_plan_paths_free": (c_int, [c_void_p]),
     "fxc_s2s_plan_create": (c_int, [POINTER(c_void_p), c_void_p]),
     "fxc_s2s_plan_destroy": (c_int, [c_void_p]),
     "fxc_s2s_plan_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int),

@@ -171,6 +171,36 @@ int fxc_linear_stack(const float *d_records, const long long *h_offsets, int npa
 int fxc_rotate_rtz(const float *d_enz, const float *d_trig, long long npairs, int n,
                    float *d_rtz, void *stream);
 
+/* ------------------------------------------------------------- planning --
+ * Station-pair planning on the host (no GPU): replaces build_valid_paths /
+ * _build_path_record_if_valid / azimuth_in_ranges / distkm_az_baz_rudoe
+ * (fastxc/inventory/planner.py:294-312, 564-617, 833-1000).  Nodes are in id
+ * order (id = index + 1); lat/lon in degrees, has_geo 0 where either is
+ * unknown; group ids compare equal for equal group names.  Pairs (i, j),
+ * j >= i (j > i without autocorrelation, j == i only with autocorr_only),
+ * pass the group rule (group_mode 0 all, 1 same group, 2 cross group; never
+ * applied to autocorrelations), then the distance [min_km, max_km] and the
+ * azimuth ranges (az_ranges [nranges][2] degrees, either the azimuth or the
+ * back azimuth inside).  The geometry is float64 libm arithmetic in the
+ * reference's order (bit-identical values).  Threads: nthreads (0 = all
+ * cores).  Results stay in an opaque set until fetched: npaths pairs in
+ * (src, rec) order, nchecks pairs examined. */
+int fxc_plan_paths(int nnodes, const double *lat, const double *lon,
+                   const unsigned char *has_geo, const int *group, int allow_autocorr,
+                   int autocorr_only, int group_mode, int use_distance, double min_km,
+                   double max_km, int use_azimuth, const double *az_ranges, int nranges,
+                   int nthreads, void **out_set, long long *npaths, long long *nchecks);
+/* src/rec [npaths] node indices, has_geo [npaths], geo [npaths][4] (great
+ * circle degrees, azimuth, back azimuth, distance km; 0 without geometry). */
+int fxc_plan_paths_fetch(void *set, int *src, int *rec, unsigned char *has_geo, double *geo);
+/* allowed_paths.tsv and allowed_path_ids.txt of the set as write_path_plan
+ * (fastxc/inventory/planner.py:351-395) writes them: csv.writer rows (tab,
+ * minimal quoting, "\r\n"), ids "%08d" / "%04d", node_text[i] the node's
+ * group.network.station.location, numbers "%.7g", empty without geometry. */
+int fxc_plan_paths_write(void *set, int nnodes, const int *node_id, const char *const *node_text,
+                         const char *tsv_path, const char *ids_path, int nthreads);
+int fxc_plan_paths_free(void *set);
+
 /* ------------------------------------------------------------ streaming --
  * Serving loop over the stages above, keeping per-pair stacks in HBM.  The
  * reference has no counterpart (it stacks offline: the linear stack of
