@@ -100,9 +100,16 @@ __device__ __forceinline__ void fixed_dif_pass(float2 *buf, const float2 *__rest
   constexpr int pitch = FP::tw_pitch(p);
   const float2 *__restrict__ tp = twg + FP::tw_offset(p);
   const float2 *tl = twl + fixed_lds_tw_offset<FP>(p);
+  // Short strides take the butterflies block-major (consecutive lanes in
+  // consecutive blocks, one leg index j): the legs of neighbouring lanes are
+  // then a whole block apart, which spreads them over the LDS banks, where
+  // j-major runs of s < 32 lanes collide; the shared twiddle read becomes a
+  // broadcast.
+  constexpr bool kBlockMajor = s < 32;
+  constexpr int nblk = FP::kM / S;
   for (int b = threadIdx.x; b < nbt; b += kThreads) {
-    const int blk = b / s;
-    const int j = b - blk * s;
+    const int blk = kBlockMajor ? b % nblk : b / s;
+    const int j = kBlockMajor ? b / nblk : b - blk * s;
     float2 *base = buf + blk * S + j;
     float2 w[R - 1];
     if constexpr (kLdsTw) {
@@ -138,9 +145,10 @@ __device__ __forceinline__ void fixed_dif_passes_lds(float2 *buf, const float2 *
 // Bin k after fixed_fft_head: the direct R-point DFT (R = last radix) of the
 // block that holds it, Z[k] = sum_m buf[base + m] w_R^(m t) with
 // t = k div (M/R) and base the digit-reversed position of k mod (M/R).
-// roots[u] = exp(sgn*2*pi*i*u/R); the powers are chained and re-read from
-// roots every 8 terms.
-template <class FP>
+// roots[u] = exp(sgn*2*pi*i*u/R).  kDirect (roots in LDS): every power is
+// read as roots[(m t) mod R]; otherwise the powers are chained and re-read
+// from roots every 8 terms.
+template <class FP, bool kDirect = false>
 __device__ __forceinline__ float2 fixed_last_pass_bin(const float2 *buf,
                                                       const float2 *__restrict__ roots, int k) {
   constexpr int P = FP::kPasses;
@@ -155,18 +163,31 @@ __device__ __forceinline__ float2 fixed_last_pass_bin(const float2 *buf,
     base += (lower - q * FP::kRadix[p]) * FP::stride(p);
     lower = q;
   }
-  const float2 wt = roots[t];
   float2 z = buf[base];
-  float2 wm = wt;
-  int e = t;
+  if constexpr (kDirect) {
+    int e = 0;
 #pragma unroll
-  for (int m = 1; m < R; ++m) {
-    const float2 v = buf[base + m];
-    z.x += v.x * wm.x - v.y * wm.y;
-    z.y += v.x * wm.y + v.y * wm.x;
-    e += t;
-    if (e >= R) e -= R;
-    if (m + 1 < R) wm = ((m + 1) & 7) == 0 ? roots[e] : cmul(wm, wt);
+    for (int m = 1; m < R; ++m) {
+      e += t;
+      if (e >= R) e -= R;
+      const float2 v = buf[base + m];
+      const float2 wm = roots[e];
+      z.x += v.x * wm.x - v.y * wm.y;
+      z.y += v.x * wm.y + v.y * wm.x;
+    }
+  } else {
+    const float2 wt = roots[t];
+    float2 wm = wt;
+    int e = t;
+#pragma unroll
+    for (int m = 1; m < R; ++m) {
+      const float2 v = buf[base + m];
+      z.x += v.x * wm.x - v.y * wm.y;
+      z.y += v.x * wm.y + v.y * wm.x;
+      e += t;
+      if (e >= R) e -= R;
+      if (m + 1 < R) wm = ((m + 1) & 7) == 0 ? roots[e] : cmul(wm, wt);
+    }
   }
   return z;
 }

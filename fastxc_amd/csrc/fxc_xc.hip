@@ -792,6 +792,11 @@ __device__ __forceinline__ void lag_r0_butterfly(float2 *lds, float2 (&u)[FP::kR
 }
 
 template <class FP>
+constexpr int lag_r0_lds_count() {
+  return FP::kM + fixed_lds_tw_count<FP>() + FP::kRadix[FP::kPasses - 1];
+}
+
+template <class FP>
 __global__ void __launch_bounds__(kLagThreads)
 lag_kernel_fixed_r0(const float2 *__restrict__ xspec, long long xpitch, long long nrows,
                     const int *__restrict__ row_out, float *__restrict__ cc, int half_cc,
@@ -831,9 +836,14 @@ lag_kernel_fixed_r0(const float2 *__restrict__ xspec, long long xpitch, long lon
         float2 v[R0];
 #pragma unroll
         for (int m = 0; m < R0; ++m) {
+          // the bin and its mirror share s = A + conj B and p = w (A - conj B):
+          //   Z = s + i p,  Z' = B + conj A + i (-conj w)(B - conj A) = conj s + i conj p
           const float2 w = cmul(wjv, make_float2(cr.c[m], cr.s[m]));
-          u[m] = fold_bin<FP>(a[it][m], b[it][m], w);
-          v[R0 - 1 - m] = fold_bin<FP>(b[it][m], a[it][m], make_float2(-w.x, w.y));
+          const float2 A = a[it][m], B = b[it][m];
+          const float2 sv = make_float2(A.x + B.x, A.y - B.y);
+          const float2 pv = cmul(w, make_float2(A.x - B.x, A.y + B.y));
+          u[m] = make_float2(sv.x - pv.y, sv.y + pv.x);
+          v[R0 - 1 - m] = make_float2(sv.x + pv.y, pv.x - sv.y);
         }
         lag_r0_butterfly<FP>(lds, u, j, tjv);
         lag_r0_butterfly<FP>(lds, v, S0 - j, cmul(e1, cconj(tjv)));
@@ -848,7 +858,7 @@ lag_kernel_fixed_r0(const float2 *__restrict__ xspec, long long xpitch, long lon
     const int n_lo = -((half_cc + 1) / 2), n_hi = half_cc / 2;
     for (int i = threadIdx.x; i <= n_hi - n_lo; i += kLagThreads) {
       const int n = n_lo + i;
-      const float2 z = fixed_last_pass_bin<FP>(lds, rts, n < 0 ? n + L::M : n);
+      const float2 z = fixed_last_pass_bin<FP, true>(lds, rts, n < 0 ? n + L::M : n);
       const int m0 = 2 * n + half_cc;
       if (m0 >= 0) y[m0] = (float)((double)z.x * weight);
       if (m0 + 1 <= 2 * half_cc) y[m0 + 1] = (float)((double)z.y * weight);
@@ -1394,8 +1404,8 @@ static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long
   }();
   if (p->d_fix_tw && !r0_off) {
     using FP = LagPlan18522;
-    const size_t flds = (size_t)(FP::kM + fixed_lds_tw_count<FP>() + FP::kRadix[FP::kPasses - 1]) *
-                        sizeof(float2);
+    static_assert(lag_r0_lds_count<FP>() <= kLdsComplexCapacity, "lag LDS fits one CU");
+    const size_t flds = (size_t)lag_r0_lds_count<FP>() * sizeof(float2);
     auto fk = lag_kernel_fixed_r0<FP>;
     FXC_HIP(hipFuncSetAttribute((const void *)fk, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)flds));
