@@ -105,3 +105,49 @@ def test_chunked_matches_unchunked(gpu):
     ca = a.correlate(spec).cpu().numpy()
     cb = b.correlate(spec).cpu().numpy()
     assert np.array_equal(ca, cb)
+
+
+def test_production_length_bf16x3(gpu):
+    """The headline form at nfft 37044: split-bf16 contraction + the fixed-plan
+    lag kernel, against FP64."""
+    import torch
+    from fastxc_amd.ops import XcPlan
+
+    rng = np.random.default_rng(8)
+    nsta, nstep, seg, nfft, H, dt = 40, 6, 36000, 37044, 1000, 0.1
+    spec = _spectra(rng, nstep, nsta, seg, nfft, dt)
+    src, rec = _all_pairs(nsta)
+    plan = XcPlan(nfft // 2 + 1, nstep, dt, H, "bf16x3")
+    plan.set_pairs(nsta, src, rec)
+    cc = plan.correlate(torch.from_numpy(spec).to(gpu)).cpu().numpy()
+    want = xc.correlate(spec, src, rec, nfft, H, dt)
+    err = xc.rel_l2(cc, want)
+    print(f"bf16x3 nfft 37044 rel_l2={err:.3e}")
+    assert err < TOL["bf16x3"]
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+def test_uneven_segments_with_gaps(gpu, precision):
+    """Segments of different station and window counts one after another, 10 %
+    of station-windows zero-filled (gaps).  As the reference does, a gap
+    still counts in the 1/nstep of its segment.  One plan per window count,
+    its pair list re-set for every segment."""
+    import torch
+    from fastxc_amd.ops import XcPlan
+
+    rng = np.random.default_rng(33)
+    seg, nfft, H, dt = 600, 1764, 200, 0.1
+    plans = {}
+    for s in range(8):
+        nsta = int(rng.integers(20, 61))
+        nstep = int(rng.choice([6, 9, 12]))
+        spec = _spectra(rng, nstep, nsta, seg, nfft, dt)
+        gaps = rng.random((nstep, nsta)) < 0.1
+        spec[gaps] = 0
+        src, rec = _all_pairs(nsta)
+        plan = plans.setdefault(nstep, XcPlan(nfft // 2 + 1, nstep, dt, H, precision))
+        plan.set_pairs(nsta, src, rec)
+        cc = plan.correlate(torch.from_numpy(spec).to(gpu)).cpu().numpy()
+        want = xc.correlate(spec, src, rec, nfft, H, dt)
+        err = xc.rel_l2(cc, want)
+        assert err < TOL[precision], (s, nsta, nstep, err)
