@@ -276,6 +276,30 @@ def _stage(name, units, unit, sec, work, bound, peak, punit, detail):
                          "frac": round(ach / peak, 4), "traffic": _stage_traffic(name)}}
 
 
+STAGE_CPU_SECONDS = 3.0
+
+
+def _stage_cpu(fn, per_call, unit, sample, seconds=None):
+    """The stage's FP64 numpy oracle timed on one host thread (BLAS and FFT
+    pools limited to one thread) over a bounded sample: calls of `fn`, each
+    worth `per_call` work units, repeated until `seconds` have passed."""
+    from threadpoolctl import threadpool_limits
+
+    seconds = STAGE_CPU_SECONDS if seconds is None else seconds
+    with threadpool_limits(limits=1):
+        fn()
+        calls = 0
+        t0 = time.perf_counter()
+        while True:
+            fn()
+            calls += 1
+            dt = time.perf_counter() - t0
+            if dt >= seconds:
+                break
+    return {"value": round(calls * per_call / dt, 3), "unit": unit, "cores": 1, "kind": "port",
+            "sample": f"{calls} x {sample}, {dt:.1f} s on one thread"}
+
+
 def _butter_power(nfft, delta, lo, hi):
     """|B/A|^2 of a 2nd-order Butterworth band-pass (scipy design), k = 0..nfft/2."""
     import numpy as np
@@ -286,7 +310,7 @@ def _butter_power(nfft, delta, lo, hi):
     return (np.abs(np.polyval(b[::-1], z) / np.polyval(a[::-1], z)) ** 2).astype(np.float32)
 
 
-def stage_sac2spec(dev, reps):
+def stage_sac2spec(dev, reps, cpu=True):
     """1000 stations x one day at dt 0.1 s: 24 one-hour windows, RUN-ABS,
     whitening after, band 0.1-2 Hz (seg 36000, filter FFT 36288, output FFT 37044)."""
     import numpy as np
@@ -309,10 +333,22 @@ def stage_sac2spec(dev, reps):
     work = units * (4 * seg + 8 * (nfft_o // 2 + 1))
     del x, out
     torch.cuda.empty_cache()
-    return _stage("sac2spec", units, "station-windows/s", sec, work, "hbm", HBM_PEAK_GBS, "GB/s",
-                  {"stations": nsta, "windows": nstep, "seg": seg, "nfft_filter": nfft_f,
-                   "nfft_out": nfft_o, "normalize": "RUN-ABS", "whiten": "after",
-                   "output_finite": ok})
+    res = _stage("sac2spec", units, "station-windows/s", sec, work, "hbm", HBM_PEAK_GBS, "GB/s",
+                 {"stations": nsta, "windows": nstep, "seg": seg, "nfft_filter": nfft_f,
+                  "nfft_out": nfft_o, "normalize": "RUN-ABS", "whiten": "after",
+                  "output_finite": ok})
+    if cpu:
+        from oracle import sac2spec as o
+
+        oplan = o.Plan(npts=npts, delta=0.1, seg=seg, shift=seg, nstep=nstep, nfft_out=nfft_o,
+                       nspec=nfft_o // 2 + 1, nfft_filter=nfft_f, freq_low=0.1, freq_high=2.0,
+                       idx=(240, 360, 7200, 9597), responses=[resp[0].astype(np.float64)],
+                       freq_lows=[0.1])
+        xs = np.random.default_rng(7).standard_normal((4, 1, seg))
+        res["cpu_baseline"] = _stage_cpu(
+            lambda: o.process_batch(xs, oplan, whiten=2, normalize=3), 4, "station-windows/s",
+            "4 station-windows (oracle/sac2spec.py process_batch, FP64, same plan)")
+    return res
 
 
 def _stack_inputs(dev, npairs, T, B, n, kind, seed):
@@ -330,7 +366,18 @@ def _stack_inputs(dev, npairs, T, B, n, kind, seed):
     return pre, w, lin, off, G
 
 
-def stage_pws(dev, reps):
+def _stack_inputs_host(npairs, T, B, n, seed):
+    """Host copy of the stack-stage input shapes for the CPU leg: sub-stack
+    means [npairs, G, n], weights [G], linear stacks [npairs, n] (FP64)."""
+    import numpy as np
+
+    G = -(-T // B)
+    pre = np.random.default_rng(seed).standard_normal((npairs, G, n))
+    w = np.array([min(B, T - i * B) / T for i in range(G)])
+    return pre, w, pre.mean(axis=1)
+
+
+def stage_pws(dev, reps, cpu=True):
     """49950 pairs (a tenth of the 1000-station triangle), 30 timestamps in
     sub-stacks of 10 (pre_stack_size), 2001 lags."""
     import torch
@@ -344,10 +391,17 @@ def stage_pws(dev, reps):
                  "GB/s", {"pairs": npairs, "timestamps": T, "substack": B, "groups": G, "n": n})
     del pre, w, lin, out
     torch.cuda.empty_cache()
+    if cpu:
+        from oracle import stack as o
+
+        pre_c, w_c, lin_c = _stack_inputs_host(16, T, B, n, 11)
+        res["cpu_baseline"] = _stage_cpu(
+            lambda: [o.pws(pre_c[i], w_c, lin_c[i]) for i in range(16)], 16, "pairs/s",
+            f"16 pairs of {G} sub-stacks x {n} lags (oracle/stack.py pws, FP64)")
     return res
 
 
-def stage_tfpws(dev, reps):
+def stage_tfpws(dev, reps, cpu=True):
     """2000 pairs, 30 timestamps in sub-stacks of 10, 2001 lags, all 1001 voices."""
     import math
 
@@ -364,10 +418,18 @@ def stage_tfpws(dev, reps):
                              "voices": n // 2 + 1})
     del pre, w, lin, out
     torch.cuda.empty_cache()
+    if cpu:
+        from oracle import stack as o
+
+        pre_c, w_c, lin_c = _stack_inputs_host(1, T, B, n, 13)
+        res["cpu_baseline"] = _stage_cpu(
+            lambda: o.tfpws(pre_c[0], w_c, lin_c[0]), 1, "pairs/s",
+            f"1 pair of {G} sub-stacks x {n} lags, {n // 2 + 1} voices "
+            "(oracle/stack.py tfpws, FP64)")
     return res
 
 
-def stage_linear(dev, reps):
+def stage_linear(dev, reps, cpu=True):
     """49950 pairs x 30 timestamps x 2001 lags, float64 running sum."""
     import numpy as np
     import torch
@@ -385,10 +447,17 @@ def stage_linear(dev, reps):
                                         "accumulator": "f64"})
     del x, out
     torch.cuda.empty_cache()
+    if cpu:
+        from oracle import stack as o
+
+        recs = np.random.default_rng(17).standard_normal((64, T, n)).astype(np.float32)
+        res["cpu_baseline"] = _stage_cpu(
+            lambda: [o.linear_stack(recs[i]) for i in range(64)], 64, "pairs/s",
+            f"64 pairs of {T} records x {n} lags (oracle/stack.py linear_stack, FP64 sum)")
     return res
 
 
-def stage_rotate(dev, reps):
+def stage_rotate(dev, reps, cpu=True):
     """49950 station pairs x 9 components x 2001 lags, ENZ -> RTZ."""
     import numpy as np
     import torch
@@ -407,6 +476,14 @@ def stage_rotate(dev, reps):
                  "GB/s", {"pairs": npairs, "components": 9, "n": n})
     del x, out
     torch.cuda.empty_cache()
+    if cpu:
+        from oracle import rotation as o
+
+        enz = np.random.default_rng(23).standard_normal((64, 9, n)).astype(np.float32)
+        angs = np.rad2deg(ang[:64].astype(np.float64))
+        res["cpu_baseline"] = _stage_cpu(
+            lambda: [o.rotate64(enz[i], angs[i, 0], angs[i, 1]) for i in range(64)], 64,
+            "pairs/s", f"64 pairs of 9 components x {n} lags (oracle/rotation.py rotate64, FP64)")
     return res
 
 
@@ -414,7 +491,7 @@ _STAGES = {"sac2spec": "stage_sac2spec", "pws": "stage_pws", "tfpws": "stage_tfp
            "linear_stack": "stage_linear", "rotate": "stage_rotate"}
 
 
-def stages(reps=3, only=""):
+def stages(reps=3, only="", cpu=True):
     import torch
 
     dev = torch.device("cuda:0")
@@ -423,7 +500,7 @@ def stages(reps=3, only=""):
     for name in names:
         fn = globals()[_STAGES[name]]
         print(f"# stage {fn.__name__}", file=sys.stderr, flush=True)
-        out.append(fn(dev, reps))
+        out.append(fn(dev, reps, cpu))
     return out
 
 
@@ -599,7 +676,7 @@ def main():
         calibrate_copy(args.calibrate_copy_gib)
     rates = [float(x) for x in args.stream_rates.split(",") if x]
     if args.stages_only:
-        print(json.dumps({"stages": stages(only=args.stage)}), flush=True)
+        print(json.dumps({"stages": stages(only=args.stage, cpu=not args.no_cpu)}), flush=True)
         return
     if args.stream_only:
         print(json.dumps({"stream": stream_sweep(rates, args.stream_seconds)}), flush=True)
@@ -653,7 +730,7 @@ def main():
     del spec, out
     torch.cuda.empty_cache()
     if not args.no_stages:
-        res["stages"] = stages(only=args.stage)
+        res["stages"] = stages(only=args.stage, cpu=not args.no_cpu)
     if not args.no_stream:
         res["stream"] = stream_sweep(rates, args.stream_seconds)
     if not args.no_sourcepack:
