@@ -56,6 +56,17 @@ _SIGNATURES = {
     "fxc_xc_cross_spectra": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "fxc_xc_lags": (c_int, [c_void_p, c_void_p, c_longlong, c_void_p, c_void_p]),
     "fxc_xc_correlate": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fxc_xc_batch_create": (c_int, [POINTER(c_void_p), c_int, c_float, c_int, c_int, c_int,
+                                    c_void_p, c_void_p, c_int, c_longlong]),
+    "fxc_xc_batch_destroy": (c_int, [c_void_p]),
+    "fxc_xc_batch_info": (c_int, [c_void_p, POINTER(c_longlong), POINTER(c_longlong),
+                                  POINTER(c_longlong), POINTER(c_int), POINTER(c_int)]),
+    "fxc_xc_batch_set_grouping": (c_int, [c_void_p, c_int]),
+    "fxc_xc_batch_correlate": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fxc_xc_batch_tune": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                  POINTER(ctypes.c_double)]),
+    "fxc_xc_batch_capture": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fxc_xc_batch_replay": (c_int, [c_void_p, c_void_p]),
     "fxc_xc_set_timing": (c_int, [c_void_p, c_int]),
     "fxc_pws": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "fxc_tfpws": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_float,

@@ -30,7 +30,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <climits>
 #include <map>
+#include <new>
 #include <vector>
 
 namespace fxc {
@@ -101,15 +103,19 @@ __global__ void station_scale_kernel(const unsigned int *__restrict__ amax_bits,
 constexpr int kSplitSta = 64;
 constexpr int kSplitBins = 16;
 
+// One workgroup's part: source stations [s0, s0 + 64) of spectra
+// [nstep][nsta][nspec], bins [k0, k0 + 16).  Source station s lands at operand
+// row sta_off + s of an operand with nsta_dst rows; station-windows whose
+// valid byte ([nstep][nsta], when given) is 0 enter as zeros.
 template <int kMode>
-__global__ void __launch_bounds__(256)
-split_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec, int kh,
-             const float *__restrict__ inv_scale, unsigned short *__restrict__ hi,
-             unsigned short *__restrict__ lo) {
+__device__ __forceinline__ void split_part(const float2 *__restrict__ spec, int nstep, int nsta,
+                                           int nspec, int kh, const float *__restrict__ inv_scale,
+                                           const unsigned char *__restrict__ valid, int sta_off,
+                                           int nsta_dst, int s0, int k0,
+                                           unsigned short *__restrict__ hi,
+                                           unsigned short *__restrict__ lo) {
   __shared__ float2 tile[8][kSplitSta][kSplitBins + 1];
   const int tid = threadIdx.x;
-  const int s0 = blockIdx.y * kSplitSta;
-  const int k0 = blockIdx.x * kSplitBins;
   const int ngrp = kh / 4;  // K groups of 8 values: kh/8 real + kh/8 imaginary
   for (int wb = 0; wb < kh; wb += 8) {
     __syncthreads();
@@ -120,7 +126,8 @@ split_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec, in
       const int ww = i / (kSplitBins * kSplitSta);
       const int w = wb + ww, sidx = s0 + ss, k = k0 + kk;
       float2 z = make_float2(0.f, 0.f);
-      if (w < nstep && sidx < nsta && k < nspec) z = spec[((size_t)w * nsta + sidx) * nspec + k];
+      if (w < nstep && sidx < nsta && k < nspec && (!valid || valid[(size_t)w * nsta + sidx]))
+        z = spec[((size_t)w * nsta + sidx) * nspec + k];
       tile[ww][ss][kk] = z;
     }
     __syncthreads();
@@ -131,7 +138,7 @@ split_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec, in
       const int kk = i / kSplitSta;
       const int sidx = s0 + ss, k = k0 + kk;
       if (sidx >= nsta || k >= nspec) continue;
-      const float sc = (kMode == FXC_PREC_FP16) ? inv_scale[sidx] : 1.f;
+      const float sc = (kMode == FXC_PREC_FP16) ? inv_scale[sta_off + sidx] : 1.f;
 #pragma unroll
       for (int part = 0; part < 2; ++part) {  // 0: real parts, 1: imaginary parts
         unsigned short h[8], l[8];
@@ -149,7 +156,7 @@ split_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec, in
           }
         }
         const int g = part * (kh / 8) + wb / 8;
-        const size_t o = (((size_t)k * ngrp + g) * nsta + sidx) * 8;
+        const size_t o = (((size_t)k * ngrp + g) * nsta_dst + sta_off + sidx) * 8;
         uint4 hv;
         hv.x = h[0] | ((unsigned)h[1] << 16); hv.y = h[2] | ((unsigned)h[3] << 16);
         hv.z = h[4] | ((unsigned)h[5] << 16); hv.w = h[6] | ((unsigned)h[7] << 16);
@@ -163,6 +170,40 @@ split_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec, in
       }
     }
   }
+}
+
+template <int kMode>
+__global__ void __launch_bounds__(256)
+split_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec, int kh,
+             const float *__restrict__ inv_scale, unsigned short *__restrict__ hi,
+             unsigned short *__restrict__ lo) {
+  split_part<kMode>(spec, nstep, nsta, nspec, kh, inv_scale, nullptr, 0, nsta,
+                    blockIdx.y * kSplitSta, blockIdx.x * kSplitBins, hi, lo);
+}
+
+// Uneven segments in one launch: segment z = blockIdx.z holds spectra
+// [nstep][nsta][nspec] at spec_off complex values and valid bytes [nstep][nsta]
+// at valid_off; its stations take operand rows [sta_off, sta_off + nsta).
+struct SplitSeg {
+  long long spec_off;
+  long long valid_off;
+  int nsta;
+  int nstep;
+  int sta_off;
+  int npairs;
+};
+
+template <int kMode>
+__global__ void __launch_bounds__(256)
+split_segments_kernel(const float2 *__restrict__ spec, const unsigned char *__restrict__ valid,
+                      const SplitSeg *__restrict__ segs, int nsta_dst, int nspec, int kh,
+                      unsigned short *__restrict__ hi, unsigned short *__restrict__ lo) {
+  const SplitSeg sg = segs[blockIdx.z];
+  const int s0 = blockIdx.y * kSplitSta;
+  if (s0 >= sg.nsta) return;  // the whole workgroup leaves before any barrier
+  split_part<kMode>(spec + sg.spec_off, sg.nstep, sg.nsta, nspec, kh, nullptr,
+                    valid ? valid + sg.valid_off : nullptr, sg.sta_off, nsta_dst, s0,
+                    blockIdx.x * kSplitBins, hi, lo);
 }
 
 // ---------------------------------------------------------------------------
@@ -585,7 +626,8 @@ template <unsigned kRadixLag>
 __global__ void __launch_bounds__(kLagThreads)
 lag_kernel(const float2 *__restrict__ xspec, long long xpitch, int nspec, long long nrows,
            const int *__restrict__ row_out, float *__restrict__ cc, int half_cc, double weight,
-           FftPlanDev plan, const float2 *__restrict__ tw, const float2 *__restrict__ fold) {
+           const float *__restrict__ task_scale, FftPlanDev plan, const float2 *__restrict__ tw,
+           const float2 *__restrict__ fold) {
   extern __shared__ float2 lds[];
   const int M = plan.n;  // nspec == M + 1
   stage_real_tables(lds, plan, tw, fold);
@@ -628,14 +670,15 @@ lag_kernel(const float2 *__restrict__ xspec, long long xpitch, int nspec, long l
     lds_fft_head<kRadixLag>(lds, 1, plan, twl, 1.f, plan.npass - 1);
     const int out_row = row_out ? row_out[row] : (int)row;
     float *y = cc + (size_t)out_row * (2 * half_cc + 1);
+    const double wr = task_scale ? weight * (double)task_scale[out_row] : weight;
     // lag l = 2n + par reads Re (par 0) or Im (par 1) of Z[n mod M]
     const int n_lo = -((half_cc + 1) / 2), n_hi = half_cc / 2;
     for (int i = threadIdx.x; i <= n_hi - n_lo; i += blockDim.x) {
       const int n = n_lo + i;
       const float2 z = last_pass_bin(lds, plan, twl, n < 0 ? n + M : n);
       const int m0 = 2 * n + half_cc;
-      if (m0 >= 0) y[m0] = (float)((double)z.x * weight);
-      if (m0 + 1 <= 2 * half_cc) y[m0 + 1] = (float)((double)z.y * weight);
+      if (m0 >= 0) y[m0] = (float)((double)z.x * wr);
+      if (m0 + 1 <= 2 * half_cc) y[m0 + 1] = (float)((double)z.y * wr);
     }
     __syncthreads();
   }
@@ -657,7 +700,8 @@ template <class FP>
 __global__ void __launch_bounds__(kLagThreads)
 lag_kernel_fixed(const float2 *__restrict__ xspec, long long xpitch, long long nrows,
                  const int *__restrict__ row_out, float *__restrict__ cc, int half_cc,
-                 double weight, const float2 *__restrict__ twg, const float2 *__restrict__ twl_g,
+                 double weight, const float *__restrict__ task_scale,
+                 const float2 *__restrict__ twg, const float2 *__restrict__ twl_g,
                  const float2 *__restrict__ foldg, const float2 *__restrict__ roots) {
   extern __shared__ float2 lds[];  // [row: M+1][base twiddles of passes 1..P-2]
   constexpr int M = FP::kM;
@@ -703,13 +747,14 @@ lag_kernel_fixed(const float2 *__restrict__ xspec, long long xpitch, long long n
     fixed_dif_passes_lds<FP, 1, kLagThreads>(lds, twl, 1.f);
     const int out_row = row_out ? row_out[row] : (int)row;
     float *y = cc + (size_t)out_row * (2 * half_cc + 1);
+    const double wr = task_scale ? weight * (double)task_scale[out_row] : weight;
     const int n_lo = -((half_cc + 1) / 2), n_hi = half_cc / 2;
     for (int i = threadIdx.x; i <= n_hi - n_lo; i += kLagThreads) {
       const int n = n_lo + i;
       const float2 z = fixed_last_pass_bin<FP>(lds, roots, n < 0 ? n + M : n);
       const int m0 = 2 * n + half_cc;
-      if (m0 >= 0) y[m0] = (float)((double)z.x * weight);
-      if (m0 + 1 <= 2 * half_cc) y[m0 + 1] = (float)((double)z.y * weight);
+      if (m0 >= 0) y[m0] = (float)((double)z.x * wr);
+      if (m0 + 1 <= 2 * half_cc) y[m0 + 1] = (float)((double)z.y * wr);
     }
     __syncthreads();
   }
@@ -800,7 +845,8 @@ template <class FP>
 __global__ void __launch_bounds__(kLagThreads)
 lag_kernel_fixed_r0(const float2 *__restrict__ xspec, long long xpitch, long long nrows,
                     const int *__restrict__ row_out, float *__restrict__ cc, int half_cc,
-                    double weight, const float2 *__restrict__ twg, const float2 *__restrict__ twl_g,
+                    double weight, const float *__restrict__ task_scale,
+                    const float2 *__restrict__ twg, const float2 *__restrict__ twl_g,
                     const float2 *__restrict__ foldg, const float2 *__restrict__ roots) {
   using L = LagPass0<FP>;
   constexpr int R0 = L::R0, S0 = L::S0;
@@ -855,13 +901,14 @@ lag_kernel_fixed_r0(const float2 *__restrict__ xspec, long long xpitch, long lon
     fixed_dif_passes_lds<FP, 1, kLagThreads>(lds, twl, 1.f);
     const int out_row = row_out ? row_out[row] : (int)row;
     float *y = cc + (size_t)out_row * (2 * half_cc + 1);
+    const double wr = task_scale ? weight * (double)task_scale[out_row] : weight;
     const int n_lo = -((half_cc + 1) / 2), n_hi = half_cc / 2;
     for (int i = threadIdx.x; i <= n_hi - n_lo; i += kLagThreads) {
       const int n = n_lo + i;
       const float2 z = fixed_last_pass_bin<FP, true>(lds, rts, n < 0 ? n + L::M : n);
       const int m0 = 2 * n + half_cc;
-      if (m0 >= 0) y[m0] = (float)((double)z.x * weight);
-      if (m0 + 1 <= 2 * half_cc) y[m0 + 1] = (float)((double)z.y * weight);
+      if (m0 >= 0) y[m0] = (float)((double)z.x * wr);
+      if (m0 + 1 <= 2 * half_cc) y[m0 + 1] = (float)((double)z.y * wr);
     }
     __syncthreads();
   }
@@ -1382,8 +1429,10 @@ static int xc_contract_chunk(fxc_xc_plan *p, const float2 *d_spec, long long c, 
   return FXC_OK;
 }
 
+// task_scale (per task, may be null) multiplies the output scale of each row.
 static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long long nrows,
-                        const int *row_out, float *d_cc, hipStream_t st) {
+                        const int *row_out, float *d_cc, hipStream_t st,
+                        const float *task_scale = nullptr) {
   if (nrows == 0) return FXC_OK;
   const RealFftPlan *rp = nullptr;
   FXC_TRY(plan_cache_real(p->nfft, 1, &rp, p->lag_max_radix, true));
@@ -1410,7 +1459,7 @@ static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long
     FXC_HIP(hipFuncSetAttribute((const void *)fk, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)flds));
     hipLaunchKernelGGL(fk, dim3((unsigned)grid), dim3(kLagThreads), flds, st, xs, xpitch, nrows,
-                       row_out, d_cc, p->half_cc, weight, p->d_fix_tw, p->d_fix_twl,
+                       row_out, d_cc, p->half_cc, weight, task_scale, p->d_fix_tw, p->d_fix_twl,
                        p->d_fix_fold, p->d_fix_roots);
   } else if (p->d_fix_tw) {
     using FP = LagPlan18522;
@@ -1419,11 +1468,11 @@ static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long
     FXC_HIP(hipFuncSetAttribute((const void *)fk, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)flds));
     hipLaunchKernelGGL(fk, dim3((unsigned)grid), dim3(kLagThreads), flds, st, xs, xpitch, nrows,
-                       row_out, d_cc, p->half_cc, weight, p->d_fix_tw, p->d_fix_twl,
+                       row_out, d_cc, p->half_cc, weight, task_scale, p->d_fix_tw, p->d_fix_twl,
                        p->d_fix_fold, p->d_fix_roots);
   } else {
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kLagThreads), lds, st, xs, xpitch,
-                       p->nspec, nrows, row_out, d_cc, p->half_cc, weight, rp->half.dev,
+                       p->nspec, nrows, row_out, d_cc, p->half_cc, weight, task_scale, rp->half.dev,
                        rp->half.d_tw, rp->d_fold);
   }
   FXC_HIP(hipGetLastError());
@@ -1489,4 +1538,380 @@ extern "C" int fxc_xc_plan_info(const fxc_xc_plan *p, int *nfft, int *half_cc, i
   if (half_cc) *half_cc = p->half_cc;
   if (cc_size) *cc_size = p->cc_size;
   return FXC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Uneven time segments in one set of launches.
+//
+// Segment s holds nsta[s] stations and nstep[s] windows; its pairs are all
+// (i, j), i < j, of its stations, in row-major order, and the batch's output
+// rows are the segments' pairs one segment after another.  Segments share
+// plans ("groups"): the stations of a group's segments sit side by side on
+// one operand axis, each segment starting on a 32-station tile boundary, so
+// every contraction tile belongs to one segment and a group runs one split
+// launch, one persistent contraction and one lag launch per row chunk.  A
+// group's operand depth K is its plan's kh, fixed by its largest window
+// count; segments with fewer windows carry zeros in the rest.  Grouping 0
+// puts every segment in one group (fewest launches, deepest K); grouping 1
+// makes one group per K depth (16, 32, 64, ...), so short segments are not
+// padded to the longest.  fxc_xc_batch_tune times both on the caller's data
+// and keeps the faster.
+//
+// The contraction of a group divides by the group's window count; each row's
+// output scale then carries group_nstep / divisor, where the divisor is the
+// segment's window count (gap mode KEEP, the reference's rule: zero-filled
+// station-windows still count) or, per pair, the number of windows in which
+// both stations are valid (gap mode EXCLUDE).  Station-windows whose valid
+// byte is 0 enter the contraction as zeros in both modes.
+// ---------------------------------------------------------------------------
+__global__ void batch_task_scale_kernel(const int4 *__restrict__ tinfo,
+                                        const SplitSeg *__restrict__ segs,
+                                        const unsigned char *__restrict__ valid, long long ntasks,
+                                        float *__restrict__ task_scale) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntasks) return;
+  const int4 ti = tinfo[t];  // segment, local src, local rec, group window count
+  const SplitSeg sg = segs[ti.x];
+  const unsigned char *v = valid + sg.valid_off;
+  int cnt = 0;
+  for (int w = 0; w < sg.nstep; ++w)
+    cnt += (v[(size_t)w * sg.nsta + ti.y] != 0 && v[(size_t)w * sg.nsta + ti.z] != 0) ? 1 : 0;
+  task_scale[t] = cnt > 0 ? (float)((double)ti.w / (double)cnt) : 0.f;
+}
+
+struct XcBatchGroup {
+  fxc_xc_plan *plan = nullptr;
+  std::vector<int> segs;
+  int max_nsta = 0;
+  int nsta_dst = 0;
+  SplitSeg *d_segs = nullptr;  // the group's segments, sta_off on its own axis
+  int *d_row_out = nullptr;    // plan workspace row -> batch task
+};
+
+struct fxc_xc_batch {
+  int nspec = 0, half_cc = 0, precision = FXC_PREC_BF16X3, gap_mode = FXC_GAPS_KEEP;
+  float dt = 0.f;
+  long long max_rows = 0;
+  int grouping = 0;
+  std::vector<int> nsta, nstep;
+  std::vector<long long> task_base;  // first batch task of each segment
+  long long ntasks = 0, spec_elems = 0, valid_elems = 0;
+  std::vector<SplitSeg> segs;  // batch offsets; sta_off unused here
+  std::vector<XcBatchGroup> groups;
+  SplitSeg *d_segs = nullptr;
+  int4 *d_tinfo = nullptr;
+  float *d_scale_keep = nullptr;  // KEEP-mode row scales (fixed by the shapes)
+  float *d_scale = nullptr;       // EXCLUDE-mode row scales (from the valid bytes)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static int batch_kh(int nstep) { return nstep <= 16 ? 16 : (nstep + 31) / 32 * 32; }
+
+static void batch_free_groups(fxc_xc_batch *b) {
+  for (auto &g : b->groups) {
+    fxc_xc_plan_destroy(g.plan);
+    dfree(g.d_segs);
+    dfree(g.d_row_out);
+  }
+  b->groups.clear();
+  dfree(b->d_tinfo);
+  dfree(b->d_scale_keep);
+  b->d_tinfo = nullptr;
+  b->d_scale_keep = nullptr;
+}
+
+static void batch_drop_graph(fxc_xc_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// (Re)builds the groups of the current grouping: plans, tiles, row maps and
+// the per-task tables.
+static int batch_build(fxc_xc_batch *b) {
+  batch_drop_graph(b);
+  batch_free_groups(b);
+  const int nseg = (int)b->nsta.size();
+  std::map<int, int> by_kh;  // K depth -> group index (grouping 1)
+  std::vector<int> seg_group(nseg);
+  for (int s = 0; s < nseg; ++s) {
+    const int key = b->grouping ? batch_kh(b->nstep[s]) : 0;
+    auto it = by_kh.find(key);
+    if (it == by_kh.end()) {
+      it = by_kh.emplace(key, (int)b->groups.size()).first;
+      b->groups.emplace_back();
+    }
+    seg_group[s] = it->second;
+    b->groups[it->second].segs.push_back(s);
+  }
+  std::vector<int4> tinfo((size_t)b->ntasks);
+  std::vector<float> keep((size_t)b->ntasks);
+  for (auto &g : b->groups) {
+    int gstep = 1;
+    for (int s : g.segs) gstep = std::max(gstep, b->nstep[s]);
+    std::vector<SplitSeg> gs;
+    std::vector<int> src, rec;
+    std::vector<long long> task;  // plan task -> batch task
+    int off = 0;
+    for (int s : g.segs) {
+      SplitSeg d = b->segs[s];
+      d.sta_off = off;
+      gs.push_back(d);
+      long long t = b->task_base[s];
+      for (int i = 0; i < b->nsta[s]; ++i)
+        for (int j = i + 1; j < b->nsta[s]; ++j, ++t) {
+          src.push_back(off + i);
+          rec.push_back(off + j);
+          task.push_back(t);
+          tinfo[(size_t)t] = make_int4(s, i, j, gstep);
+          keep[(size_t)t] = (float)((double)gstep / (double)b->nstep[s]);
+        }
+      g.max_nsta = std::max(g.max_nsta, b->nsta[s]);
+      off += (b->nsta[s] + kTile - 1) / kTile * kTile;
+    }
+    g.nsta_dst = std::max(off, 1);
+    FXC_TRY(fxc_xc_plan_create(&g.plan, b->nspec, gstep, b->dt, b->half_cc, b->precision));
+    FXC_TRY(fxc_xc_set_chunk_rows(g.plan, b->max_rows));
+    FXC_TRY(fxc_xc_set_pairs(g.plan, g.nsta_dst, src.data(), rec.data(), (long long)src.size()));
+    std::vector<int> row_out(g.plan->h_row_task.size());
+    for (size_t r = 0; r < row_out.size(); ++r)
+      row_out[r] = (int)task[(size_t)g.plan->h_row_task[r]];
+    FXC_TRY(dmalloc((void **)&g.d_segs, gs.size() * sizeof(SplitSeg)));
+    FXC_HIP(hipMemcpy(g.d_segs, gs.data(), gs.size() * sizeof(SplitSeg), hipMemcpyHostToDevice));
+    if (!row_out.empty()) {
+      FXC_TRY(dmalloc((void **)&g.d_row_out, row_out.size() * sizeof(int)));
+      FXC_HIP(hipMemcpy(g.d_row_out, row_out.data(), row_out.size() * sizeof(int),
+                        hipMemcpyHostToDevice));
+    }
+  }
+  if (b->ntasks > 0) {
+    FXC_TRY(dmalloc((void **)&b->d_tinfo, tinfo.size() * sizeof(int4)));
+    FXC_HIP(hipMemcpy(b->d_tinfo, tinfo.data(), tinfo.size() * sizeof(int4),
+                      hipMemcpyHostToDevice));
+    FXC_TRY(dmalloc((void **)&b->d_scale_keep, keep.size() * sizeof(float)));
+    FXC_HIP(hipMemcpy(b->d_scale_keep, keep.data(), keep.size() * sizeof(float),
+                      hipMemcpyHostToDevice));
+  }
+  return FXC_OK;
+}
+
+extern "C" int fxc_xc_batch_create(fxc_xc_batch **out, int nspec, float dt, int half_cc,
+                                   int precision, int nseg, const int *nsta, const int *nstep,
+                                   int gap_mode, long long max_rows) {
+  FXC_CHECK_ARG(out != nullptr, "fxc_xc_batch_create: null output");
+  *out = nullptr;
+  FXC_CHECK_ARG(nseg >= 1 && nseg <= 65535 && nsta && nstep,
+                "fxc_xc_batch_create: need 1..65535 segments");
+  FXC_CHECK_ARG(gap_mode == FXC_GAPS_KEEP || gap_mode == FXC_GAPS_EXCLUDE,
+                "fxc_xc_batch_create: unknown gap mode %d", gap_mode);
+  FXC_CHECK_ARG(max_rows >= 0, "fxc_xc_batch_create: negative row cap");
+  if (precision != FXC_PREC_BF16X3) {
+    set_error("fxc_xc_batch_create: the batched form runs the split-BF16 contraction only");
+    return FXC_ERR_UNSUPPORTED;
+  }
+  auto *b = new (std::nothrow) fxc_xc_batch();
+  if (!b) {
+    set_error("fxc_xc_batch_create: out of host memory");
+    return FXC_ERR_ALLOC;
+  }
+  b->nspec = nspec;
+  b->dt = dt;
+  b->half_cc = half_cc;
+  b->precision = precision;
+  b->gap_mode = gap_mode;
+  b->max_rows = max_rows;
+  long long soff = 0, voff = 0, t = 0;
+  for (int s = 0; s < nseg; ++s) {
+    if (nsta[s] < 1 || nstep[s] < 1 || batch_kh(nstep[s]) > 128) {
+      set_error("fxc_xc_batch_create: segment %d has %d stations and %d windows (need >= 1 and "
+                "<= 128 windows)", s, nsta[s], nstep[s]);
+      delete b;
+      return FXC_ERR_ARG;
+    }
+    b->nsta.push_back(nsta[s]);
+    b->nstep.push_back(nstep[s]);
+    b->task_base.push_back(t);
+    SplitSeg d{};
+    d.spec_off = soff;
+    d.valid_off = voff;
+    d.nsta = nsta[s];
+    d.nstep = nstep[s];
+    d.npairs = (int)((long long)nsta[s] * (nsta[s] - 1) / 2);
+    b->segs.push_back(d);
+    soff += (long long)nstep[s] * nsta[s] * nspec;
+    voff += (long long)nstep[s] * nsta[s];
+    t += (long long)nsta[s] * (nsta[s] - 1) / 2;
+  }
+  b->spec_elems = soff;
+  b->valid_elems = voff;
+  b->ntasks = t;
+  if (t > INT_MAX) {
+    set_error("fxc_xc_batch_create: %lld pairs exceed the row index range", t);
+    delete b;
+    return FXC_ERR_ARG;
+  }
+  int rc = dmalloc((void **)&b->d_segs, b->segs.size() * sizeof(SplitSeg));
+  if (rc == FXC_OK && hipMemcpy(b->d_segs, b->segs.data(), b->segs.size() * sizeof(SplitSeg),
+                                hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("fxc_xc_batch_create: segment table upload failed");
+    rc = FXC_ERR_HIP;
+  }
+  if (rc == FXC_OK && b->ntasks > 0)
+    rc = dmalloc((void **)&b->d_scale, (size_t)b->ntasks * sizeof(float));
+  if (rc == FXC_OK) rc = batch_build(b);
+  if (rc != FXC_OK) {
+    fxc_xc_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return FXC_OK;
+}
+
+extern "C" int fxc_xc_batch_destroy(fxc_xc_batch *b) {
+  if (!b) return FXC_OK;
+  batch_drop_graph(b);
+  batch_free_groups(b);
+  dfree(b->d_segs);
+  dfree(b->d_scale);
+  delete b;
+  return FXC_OK;
+}
+
+extern "C" int fxc_xc_batch_info(const fxc_xc_batch *b, long long *ntasks, long long *spec_elems,
+                                 long long *valid_elems, int *ngroups, int *grouping) {
+  FXC_CHECK_ARG(b, "fxc_xc_batch_info: null batch");
+  if (ntasks) *ntasks = b->ntasks;
+  if (spec_elems) *spec_elems = b->spec_elems;
+  if (valid_elems) *valid_elems = b->valid_elems;
+  if (ngroups) *ngroups = (int)b->groups.size();
+  if (grouping) *grouping = b->grouping;
+  return FXC_OK;
+}
+
+extern "C" int fxc_xc_batch_set_grouping(fxc_xc_batch *b, int grouping) {
+  FXC_CHECK_ARG(b && (grouping == 0 || grouping == 1),
+                "fxc_xc_batch_set_grouping: grouping must be 0 or 1");
+  if (grouping == b->grouping && !b->groups.empty()) return FXC_OK;
+  b->grouping = grouping;
+  return batch_build(b);
+}
+
+// Every launch of one batch run on `st`; allocates nothing once the plans'
+// workspaces exist (the first run creates them).
+static int batch_run(fxc_xc_batch *b, const float2 *spec, const unsigned char *valid, float *cc,
+                     hipStream_t st) {
+  const float *scale = b->d_scale_keep;
+  if (b->gap_mode == FXC_GAPS_EXCLUDE && valid && b->ntasks > 0) {
+    hipLaunchKernelGGL(batch_task_scale_kernel, dim3((unsigned)ceil_div(b->ntasks, 256)),
+                       dim3(256), 0, st, b->d_tinfo, b->d_segs, valid, b->ntasks, b->d_scale);
+    FXC_HIP(hipGetLastError());
+    scale = b->d_scale;
+  }
+  for (auto &g : b->groups) {
+    fxc_xc_plan *p = g.plan;
+    if (p->npairs == 0) continue;
+    long long maxrows = 0;
+    FXC_TRY(fxc_xc_pairs_info(p, nullptr, nullptr, &maxrows));
+    FXC_TRY(xc_ensure_workspace(p, (size_t)maxrows));
+    const dim3 grid((unsigned)ceil_div(p->nspec, kSplitBins),
+                    (unsigned)ceil_div(g.max_nsta, kSplitSta), (unsigned)g.segs.size());
+    hipLaunchKernelGGL(split_segments_kernel<FXC_PREC_BF16X3>, grid, dim3(256), 0, st, spec,
+                       valid, g.d_segs, g.nsta_dst, p->nspec, p->kh, p->d_hi, p->d_lo);
+    FXC_HIP(hipGetLastError());
+    const long long nc = (long long)p->chunk_tile_begin.size() - 1;
+    for (long long c = 0; c < nc; ++c) {
+      FXC_TRY(xc_contract_chunk(p, spec, c, p->d_xspec, st));
+      const long long r0 = p->chunk_row_begin[c], r1 = p->chunk_row_begin[c + 1];
+      FXC_TRY(xc_lags_rows(p, p->d_xspec, p->xpitch, r1 - r0, g.d_row_out + r0, cc, st, scale));
+    }
+  }
+  return FXC_OK;
+}
+
+static int batch_check_io(const fxc_xc_batch *b, const float *d_spec, const float *d_cc,
+                          const char *who) {
+  FXC_CHECK_ARG(b && d_spec && d_cc, "%s: invalid arguments", who);
+  return FXC_OK;
+}
+
+// cc[task][0..2H] for every pair of every segment.  d_spec: the segments'
+// spectra [nstep][nsta][nspec] complex, one after another; d_valid: null
+// (every station-window valid) or bytes [nstep][nsta], one segment after
+// another.
+extern "C" int fxc_xc_batch_correlate(fxc_xc_batch *b, const float *d_spec,
+                                      const unsigned char *d_valid, float *d_cc, void *stream) {
+  FXC_TRY(batch_check_io(b, d_spec, d_cc, "fxc_xc_batch_correlate"));
+  return batch_run(b, (const float2 *)d_spec, d_valid, d_cc, (hipStream_t)stream);
+}
+
+// Records one batch run on `stream` (not the null stream) into a HIP graph
+// bound to these buffers, after one eager run that creates every workspace.
+// fxc_xc_batch_replay launches it again; new data are written into the same
+// buffers between replays.
+extern "C" int fxc_xc_batch_capture(fxc_xc_batch *b, const float *d_spec,
+                                    const unsigned char *d_valid, float *d_cc, void *stream) {
+  FXC_TRY(batch_check_io(b, d_spec, d_cc, "fxc_xc_batch_capture"));
+  FXC_CHECK_ARG(stream != nullptr, "fxc_xc_batch_capture: capture needs a non-null stream");
+  hipStream_t st = (hipStream_t)stream;
+  batch_drop_graph(b);
+  FXC_TRY(batch_run(b, (const float2 *)d_spec, d_valid, d_cc, st));
+  FXC_HIP(hipStreamSynchronize(st));
+  FXC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+  const int rc = batch_run(b, (const float2 *)d_spec, d_valid, d_cc, st);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc != FXC_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  FXC_HIP(e);
+  b->graph = g;
+  FXC_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return FXC_OK;
+}
+
+extern "C" int fxc_xc_batch_replay(fxc_xc_batch *b, void *stream) {
+  FXC_CHECK_ARG(b && b->exec, "fxc_xc_batch_replay: no captured graph");
+  FXC_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return FXC_OK;
+}
+
+// Runs the batch under both groupings (one warm-up run and `reps` timed runs
+// each, HIP events on `stream`) and keeps the faster; ms[2] receives the
+// mean time per run of grouping 0 and 1.  Any captured graph is dropped.
+extern "C" int fxc_xc_batch_tune(fxc_xc_batch *b, const float *d_spec,
+                                 const unsigned char *d_valid, float *d_cc, int reps,
+                                 void *stream, double *ms) {
+  FXC_TRY(batch_check_io(b, d_spec, d_cc, "fxc_xc_batch_tune"));
+  FXC_CHECK_ARG(reps >= 1, "fxc_xc_batch_tune: reps must be >= 1");
+  hipStream_t st = (hipStream_t)stream;
+  double t[2] = {0.0, 0.0};
+  for (int mode = 0; mode < 2; ++mode) {
+    FXC_TRY(fxc_xc_batch_set_grouping(b, mode));
+    FXC_TRY(batch_run(b, (const float2 *)d_spec, d_valid, d_cc, st));
+    hipEvent_t e0, e1;
+    FXC_HIP(hipEventCreate(&e0));
+    FXC_HIP(hipEventCreate(&e1));
+    int rc = hipEventRecord(e0, st) == hipSuccess ? FXC_OK : FXC_ERR_HIP;
+    for (int r = 0; r < reps && rc == FXC_OK; ++r)
+      rc = batch_run(b, (const float2 *)d_spec, d_valid, d_cc, st);
+    float el = 0.f;
+    if (rc == FXC_OK && (hipEventRecord(e1, st) != hipSuccess ||
+                         hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(&el, e0, e1) != hipSuccess)) {
+      set_error("fxc_xc_batch_tune: event timing failed");
+      rc = FXC_ERR_HIP;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    FXC_TRY(rc);
+    t[mode] = (double)el / reps;
+  }
+  if (ms) {
+    ms[0] = t[0];
+    ms[1] = t[1];
+  }
+  return fxc_xc_batch_set_grouping(b, t[1] < t[0] ? 1 : 0);
 }

@@ -159,6 +159,136 @@ class XcPlan:
         return out
 
 
+class XcBatch:
+    """Uneven time segments correlated in one set of launches (split-BF16).
+
+    Segment s has nsta[s] stations and nstep[s] windows; its pairs are all
+    (i, j), i < j, in row-major order, and output rows are the segments'
+    pairs one segment after another.  spec: complex64, the segments'
+    [nstep, nsta, nspec] spectra flattened back to back (see pack); valid:
+    None or uint8, one byte per station-window ([nstep, nsta] per segment,
+    back to back), 0 = a gap, which enters the stack as zeros.
+    gaps="keep" divides by the segment's window count (the reference's rule
+    for zero-filled windows); gaps="exclude" divides each pair by the windows
+    in which both stations are valid.
+    """
+
+    GAPS = {"keep": 0, "exclude": 1}
+
+    def __init__(self, nspec: int, dt: float, half_cc: int, nsta, nstep, gaps: str = "keep",
+                 precision: str = "bf16x3", max_rows: int = 0):
+        self.nsta = [int(n) for n in nsta]
+        self.nstep = [int(n) for n in nstep]
+        if len(self.nsta) != len(self.nstep) or not self.nsta:
+            raise ValueError("nsta and nstep must be equal-length, non-empty")
+        if gaps not in self.GAPS:
+            raise ValueError(f"gaps must be one of {sorted(self.GAPS)}")
+        self._h = ctypes.c_void_p()
+        a = np.ascontiguousarray(self.nsta, dtype=np.int32)
+        b = np.ascontiguousarray(self.nstep, dtype=np.int32)
+        check(lib().fxc_xc_batch_create(ctypes.byref(self._h), int(nspec), float(dt),
+                                        int(half_cc), PRECISIONS[precision], len(a),
+                                        a.ctypes.data, b.ctypes.data, self.GAPS[gaps],
+                                        int(max_rows)), "fxc_xc_batch_create")
+        self.nspec, self.dt, self.half_cc, self.gaps = nspec, dt, half_cc, gaps
+        self.cc_size = 2 * half_cc + 1
+        info = self.info()
+        self.ntasks, self.spec_elems, self.valid_elems = (info["ntasks"], info["spec_elems"],
+                                                          info["valid_elems"])
+        self.task_base = np.concatenate([[0], np.cumsum([n * (n - 1) // 2 for n in self.nsta])])
+        self._stream = None
+
+    def close(self):
+        if self._h:
+            lib().fxc_xc_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        t, s, v = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        g, m = ctypes.c_int(), ctypes.c_int()
+        check(lib().fxc_xc_batch_info(self._h, ctypes.byref(t), ctypes.byref(s), ctypes.byref(v),
+                                      ctypes.byref(g), ctypes.byref(m)), "fxc_xc_batch_info")
+        return {"ntasks": t.value, "spec_elems": s.value, "valid_elems": v.value,
+                "groups": g.value, "grouping": m.value}
+
+    def set_grouping(self, grouping: int) -> None:
+        check(lib().fxc_xc_batch_set_grouping(self._h, int(grouping)),
+              "fxc_xc_batch_set_grouping")
+
+    @staticmethod
+    def pack(parts, device) -> torch.Tensor:
+        """Concatenates per-segment arrays (host or device) into one flat device tensor."""
+        return torch.cat([torch.as_tensor(p).reshape(-1) for p in parts]).to(device).contiguous()
+
+    def segment_rows(self, s: int) -> slice:
+        return slice(int(self.task_base[s]), int(self.task_base[s + 1]))
+
+    def _args(self, spec, valid, out):
+        _require(spec, torch.complex64, "spec")
+        if spec.numel() != self.spec_elems:
+            raise ValueError(f"spec holds {spec.numel()} values, the segments {self.spec_elems}")
+        vp = None
+        if valid is not None:
+            _require(valid, torch.uint8, "valid")
+            if valid.numel() != self.valid_elems or valid.device != spec.device:
+                raise ValueError(f"valid must hold {self.valid_elems} bytes on spec's device")
+            vp = valid.data_ptr()
+        if out is None:
+            out = torch.empty((self.ntasks, self.cc_size), dtype=torch.float32, device=spec.device)
+        _require(out, torch.float32, "out")
+        if tuple(out.shape) != (self.ntasks, self.cc_size):
+            raise ValueError("out must be [ntasks, 2*half_cc+1]")
+        return vp, out
+
+    def correlate(self, spec: torch.Tensor, valid: torch.Tensor | None = None,
+                  out: torch.Tensor | None = None) -> torch.Tensor:
+        vp, out = self._args(spec, valid, out)
+        check(lib().fxc_xc_batch_correlate(self._h, spec.data_ptr(), vp, out.data_ptr(),
+                                           _stream_ptr(spec.device)), "fxc_xc_batch_correlate")
+        return out
+
+    def tune(self, spec: torch.Tensor, valid: torch.Tensor | None = None,
+             out: torch.Tensor | None = None, reps: int = 2):
+        """Times grouping 0 and 1 on this data and keeps the faster."""
+        vp, out = self._args(spec, valid, out)
+        ms = (ctypes.c_double * 2)()
+        check(lib().fxc_xc_batch_tune(self._h, spec.data_ptr(), vp, out.data_ptr(), int(reps),
+                                      _stream_ptr(spec.device), ms), "fxc_xc_batch_tune")
+        return {"ms": [ms[0], ms[1]], "grouping": self.info()["grouping"]}
+
+    def capture(self, spec: torch.Tensor, valid: torch.Tensor | None = None,
+                out: torch.Tensor | None = None) -> torch.Tensor:
+        """Records one run into a HIP graph bound to these tensors (on a side
+        stream ordered after the current one); replay() re-runs it."""
+        vp, out = self._args(spec, valid, out)
+        cur = torch.cuda.current_stream(spec.device)
+        self._stream = torch.cuda.Stream(device=spec.device)
+        self._stream.wait_stream(cur)
+        check(lib().fxc_xc_batch_capture(self._h, spec.data_ptr(), vp, out.data_ptr(),
+                                         ctypes.c_void_p(self._stream.cuda_stream)),
+              "fxc_xc_batch_capture")
+        cur.wait_stream(self._stream)
+        self._graph_io = (spec, valid, out)
+        return out
+
+    def replay(self) -> torch.Tensor:
+        if self._stream is None:
+            raise RuntimeError("capture() first")
+        spec = self._graph_io[0]
+        cur = torch.cuda.current_stream(spec.device)
+        self._stream.wait_stream(cur)
+        check(lib().fxc_xc_batch_replay(self._h, ctypes.c_void_p(self._stream.cuda_stream)),
+              "fxc_xc_batch_replay")
+        cur.wait_stream(self._stream)
+        return self._graph_io[2]
+
+
 class S2sPlan:
     """sac2spec per-window pipeline (preprocess, prefilter, normalisation,
     whitening, output spectrum).

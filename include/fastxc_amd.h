@@ -94,6 +94,44 @@ int fxc_xc_get_timing(fxc_xc_plan *plan, double *ms3, long long *launches3);
  * native/xc/worker/async_workers.cu:18-86, minus file I/O). */
 int fxc_xc_correlate(fxc_xc_plan *plan, const float *d_spec, float *d_cc, void *stream);
 
+/* Uneven time segments in one set of launches (split-bf16 form only).
+ * Segment s has nsta[s] stations and nstep[s] <= 128 windows; its pairs are
+ * (i, j), i < j, row-major, and the output rows [ntasks][2*half_cc+1] are the
+ * segments' pairs one segment after another.  d_spec: the segments' spectra
+ * [nstep][nsta][nspec] complex back to back; d_valid: null or one byte per
+ * station-window ([nstep][nsta] per segment, back to back), 0 = a gap, which
+ * enters the stack as zeros.  Gap mode KEEP divides by the segment's window
+ * count, as the reference does for zero-filled windows
+ * (native/xc/worker/async_workers.cu:66); EXCLUDE divides each pair by the
+ * windows in which both stations are valid (0 of them: a zero row).
+ * Per segment the result is fxc_xc_correlate's for that segment alone.
+ * max_rows bounds each plan's cross-spectrum workspace (0 = no bound).
+ * No reference counterpart: the reference runs one timestamp's job at a
+ * time (native/xc/worker/async_workers.cu:18-86). */
+#define FXC_GAPS_KEEP 0
+#define FXC_GAPS_EXCLUDE 1
+typedef struct fxc_xc_batch fxc_xc_batch;
+int fxc_xc_batch_create(fxc_xc_batch **batch, int nspec, float dt, int half_cc, int precision,
+                        int nseg, const int *nsta, const int *nstep, int gap_mode,
+                        long long max_rows);
+int fxc_xc_batch_destroy(fxc_xc_batch *batch);
+int fxc_xc_batch_info(const fxc_xc_batch *batch, long long *ntasks, long long *spec_elems,
+                      long long *valid_elems, int *ngroups, int *grouping);
+/* grouping 0: every segment in one plan (operand depth of the longest
+ * segment); 1: one plan per operand depth (16, 32, 64, 96, 128 windows). */
+int fxc_xc_batch_set_grouping(fxc_xc_batch *batch, int grouping);
+int fxc_xc_batch_correlate(fxc_xc_batch *batch, const float *d_spec, const unsigned char *d_valid,
+                           float *d_cc, void *stream);
+/* Times both groupings on these buffers (reps runs each, after a warm-up)
+ * and keeps the faster; ms[2] gets the mean ms per run of each. */
+int fxc_xc_batch_tune(fxc_xc_batch *batch, const float *d_spec, const unsigned char *d_valid,
+                      float *d_cc, int reps, void *stream, double *ms);
+/* Captures one run into a HIP graph bound to these buffers (stream must not
+ * be the null stream); replay launches it again on any stream. */
+int fxc_xc_batch_capture(fxc_xc_batch *batch, const float *d_spec, const unsigned char *d_valid,
+                         float *d_cc, void *stream);
+int fxc_xc_batch_replay(fxc_xc_batch *batch, void *stream);
+
 /* ------------------------------------------------------------ sac2spec --
  * Per station-window preprocessing, normalisation, whitening and output
  * spectrum.  Replaces ComputeWorkerBatch (native/sac2spec/worker/compute.cu:8-272)

@@ -21,7 +21,13 @@ Pinning (tests/test_oracle_xc.py):
                          the upper block triangle appears exactly once, oriented
                          src = lower nsl id, in the documented loop order, plus a
                          brute-force enumeration of the same rule on small inputs;
-  finalize_shape         pinned by closed forms on hand-computed cases.
+  finalize_shape         pinned by closed forms on hand-computed cases;
+  cross_spectrum_valid   (the gap-excluding stack of the batched uneven form,
+                         which has no reference counterpart) pinned by its
+                         reduction to cross_spectrum when every window is
+                         valid, and by pairs whose stations share exactly one
+                         valid window (the stack is that window's product) or
+                         none (zero).
 """
 from __future__ import annotations
 
@@ -164,6 +170,26 @@ def cross_spectrum(spec: np.ndarray, src, rec) -> np.ndarray:
     for w in range(nstep):
         acc += np.conj(s[w, src, :]) * s[w, rec, :]
     return acc / nstep
+
+
+def cross_spectrum_valid(spec: np.ndarray, valid: np.ndarray, src, rec) -> np.ndarray:
+    """Gap-excluding stack: sum over the windows w in which both valid[w, src]
+    and valid[w, rec] are set of conj(S[w,src,k]) S[w,rec,k], divided by the
+    number of such windows (0 windows: a zero row); FP64.  valid [nstep, nsta]."""
+    s = np.asarray(spec, dtype=np.complex128)
+    v = np.asarray(valid).astype(bool)
+    src = np.asarray(src)
+    rec = np.asarray(rec)
+    acc = np.zeros((len(src), s.shape[2]), dtype=np.complex128)
+    cnt = np.zeros(len(src))
+    for w in range(s.shape[0]):
+        both = v[w, src] & v[w, rec]
+        acc[both] += np.conj(s[w, src[both], :]) * s[w, rec[both], :]
+        cnt += both
+    out = np.zeros_like(acc)
+    nz = cnt > 0
+    out[nz] = acc[nz] / cnt[nz, None]
+    return out
 
 
 def lags(xspec: np.ndarray, nfft: int, half_cc: int, dt: float) -> np.ndarray:

@@ -97,3 +97,30 @@ def test_task_order_within_diagonal_block():
     tasks, _ = xc.build_tasks_for_job(ids, allowed, job)
     got = [(t.src_meta, t.rec_meta) for t in tasks]
     assert got == [(0, 0), (0, 1), (0, 2), (0, 3), (1, 1), (1, 2), (1, 3), (2, 2), (2, 3), (3, 3)]
+
+
+def test_valid_stack_all_valid_is_the_reference_stack():
+    rng = np.random.default_rng(41)
+    spec = rng.standard_normal((7, 5, 33)) + 1j * rng.standard_normal((7, 5, 33))
+    src, rec = np.triu_indices(5, k=1)
+    got = xc.cross_spectrum_valid(spec, np.ones((7, 5), np.uint8), src, rec)
+    np.testing.assert_allclose(got, xc.cross_spectrum(spec, src, rec), rtol=1e-13, atol=0)
+
+
+def test_valid_stack_divides_by_shared_windows():
+    # station 0 valid in windows {0, 1}, station 1 in {1, 2}, station 2 in {3}:
+    # pair (0, 1) shares window 1 only, pairs (0, 2) and (1, 2) share none
+    rng = np.random.default_rng(42)
+    spec = rng.standard_normal((4, 3, 9)) + 1j * rng.standard_normal((4, 3, 9))
+    valid = np.zeros((4, 3), np.uint8)
+    valid[[0, 1], 0] = 1
+    valid[[1, 2], 1] = 1
+    valid[3, 2] = 1
+    got = xc.cross_spectrum_valid(spec, valid, [0, 0, 1], [1, 2, 2])
+    np.testing.assert_allclose(got[0], np.conj(spec[1, 0]) * spec[1, 1], rtol=1e-14)
+    assert not got[1:].any()
+    # two shared windows: their mean, not the mean over all four windows
+    valid[2, 0] = 1
+    got = xc.cross_spectrum_valid(spec, valid, [0], [1])
+    want = (np.conj(spec[1, 0]) * spec[1, 1] + np.conj(spec[2, 0]) * spec[2, 1]) / 2
+    np.testing.assert_allclose(got[0], want, rtol=1e-14)
