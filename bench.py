@@ -26,7 +26,9 @@ same timestamp (499500 records) under "sourcepack", beside the Python
 restatement of the reference builder on a 50000-record sample;
 --no-sourcepack skips it.  The inventory path planner's all-pairs loop (host
 C++) is timed on 2000 stations under "planner", beside the oracle's Python
-loop on 200 of them; --no-planner skips it.
+loop on 200 of them; --no-planner skips it.  The batched uneven-segment form
+(64 segments of 20-400 stations and 6-48 windows, 10 % gaps) is timed eager
+and as a replayed HIP graph under "batch"; --no-batch skips it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision ...]
 """
@@ -76,6 +78,10 @@ def parse():
     ap.add_argument("--stream-only", action="store_true", help="measure the streaming loop only")
     ap.add_argument("--no-sourcepack", action="store_true",
                     help="skip the SourcePack indexer measurement")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched uneven-segment measurement")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="measure the batched uneven-segment form only")
     ap.add_argument("--no-planner", action="store_true",
                     help="skip the inventory path-planner measurement")
     ap.add_argument("--stream-rates", default="16,64,128,192",
@@ -599,6 +605,83 @@ def _write_xcpack(root, nsta, lags=2001, ts="20200502T04:00", limit=None):
     return x, n
 
 
+# ------------------------------------------------------------------ batch --
+# Uneven time segments in one set of launches (fxc_xc_batch_*): 64 segments,
+# stations ~U[20,400], windows ~U[6,48], 10 % of station-windows gaps,
+# one-hour windows at dt 0.1 s (nspec 18523, 2001 lags), gaps excluded from
+# each pair's stack.  The grouping tuner picks one plan or one plan per
+# operand depth; the chosen batch is captured into a HIP graph and replayed.
+# pair-windows = sum over segments of pairs x windows.  Memory is estimated
+# first and the segment count lowered (and said so) if it would not fit.
+def batch_measure(nseg=64, seed=13, reps=3, budget_gb=235.0, max_rows=65536):
+    import numpy as np
+    import torch
+    from fastxc_amd.ops import XcBatch
+
+    nspec, half = 18523, 1000
+    rng = np.random.default_rng(seed)
+    nsta = rng.integers(20, 401, size=nseg)
+    nstep = rng.integers(6, 49, size=nseg)
+
+    def depth(w):
+        return 16 if w <= 16 else (w + 31) // 32 * 32
+
+    def gbytes(k):
+        """Peak device bytes (GB) of grouping 0 and of grouping 1."""
+        a, w = nsta[:k], nstep[:k]
+        fixed = float((a * w).sum()) * nspec * 8 + float((a * (a - 1) // 2).sum()) * (2 * half + 1) * 4
+        ws = max_rows * 18528 * 8.0
+        padded = (a + 31) // 32 * 32
+        one = float(padded.sum()) * nspec * 2 * depth(int(w.max())) * 4 + ws
+        kinds = sorted({depth(int(x)) for x in w})
+        per = sum(float(padded[[depth(int(x)) == d for x in w]].sum()) * nspec * 2 * d * 4
+                  for d in kinds) + len(kinds) * ws
+        return (fixed + one) / 1e9, (fixed + per) / 1e9
+
+    k = nseg
+    while k > 1 and min(gbytes(k)) > budget_gb:
+        k -= 1
+    nsta, nstep = nsta[:k], nstep[:k]
+    g0, g1 = gbytes(k)
+    dev = torch.device("cuda:0")
+    print(f"# batch: {k} segments, ~{g0:.0f} / {g1:.0f} GB (grouping 0 / 1)", file=sys.stderr,
+          flush=True)
+    b = XcBatch(nspec, 0.1, half, nsta.tolist(), nstep.tolist(), gaps="exclude",
+                max_rows=max_rows)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    spec = torch.randn(b.spec_elems, dtype=torch.complex64, device=dev, generator=g)
+    valid = (torch.rand(b.valid_elems, device=dev, generator=g) >= 0.1).to(torch.uint8)
+    out = torch.empty((b.ntasks, b.cc_size), dtype=torch.float32, device=dev)
+    if max(g0, g1) <= budget_gb:
+        tune = b.tune(spec, valid, out, reps=1)
+        tuner = {"ms": [round(x, 2) for x in tune["ms"]], "grouping": tune["grouping"]}
+    else:
+        # the tuner's candidates are the groupings that fit the memory budget
+        b.set_grouping(0 if g0 <= budget_gb else 1)
+        tuner = {"ms": None, "grouping": b.info()["grouping"],
+                 "skipped": f"grouping {1 if g0 <= budget_gb else 0} needs "
+                            f"~{max(g0, g1):.0f} GB"}
+    sec = _time_calls(lambda: b.correlate(spec, valid, out), reps)
+    b.capture(spec, valid, out)
+    torch.cuda.synchronize()
+    gsec = _time_calls(lambda: b.replay(), reps)
+    pw = float((nsta * (nsta - 1) // 2 * nstep).sum())
+    res = {"segments": int(k), "pairs": int(b.ntasks), "pair_windows": pw,
+           "stations_total": int(nsta.sum()),
+           "windows_min_max": [int(nstep.min()), int(nstep.max())],
+           "gaps": "10 % of station-windows, excluded per pair", "nspec": nspec,
+           "lags": 2 * half + 1, "max_rows": max_rows, "tuner": tuner,
+           "groups": b.info()["groups"], "ms_eager": round(sec * 1e3, 2),
+           "ms_graph": round(gsec * 1e3, 2),
+           "pair_windows_per_s": round(pw / min(sec, gsec), 1),
+           "est_gb": [round(g0, 1), round(g1, 1)]}
+    b.close()
+    del spec, valid, out
+    torch.cuda.empty_cache()
+    return res
+
+
 def sourcepack_measure(nsta=1000, sample=50000):
     import shutil
     import subprocess
@@ -678,6 +761,9 @@ def main():
     if args.stages_only:
         print(json.dumps({"stages": stages(only=args.stage, cpu=not args.no_cpu)}), flush=True)
         return
+    if args.batch_only:
+        print(json.dumps({"batch": batch_measure()}), flush=True)
+        return
     if args.stream_only:
         print(json.dumps({"stream": stream_sweep(rates, args.stream_seconds)}), flush=True)
         return
@@ -736,6 +822,12 @@ def main():
     if not args.no_sourcepack:
         print("# sourcepack indexer", file=sys.stderr, flush=True)
         res["sourcepack"] = sourcepack_measure()
+    if not args.no_batch:
+        try:
+            res["batch"] = batch_measure()
+        except Exception as e:  # the headline line must still be printed
+            res["batch"] = {"error": str(e)[:300]}
+            torch.cuda.empty_cache()
     if not args.no_planner:
         print("# path planner", file=sys.stderr, flush=True)
         res["planner"] = planner_measure()
