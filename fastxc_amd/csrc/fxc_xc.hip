@@ -1374,7 +1374,7 @@ static int launch_mfma_mode(int kh, dim3 grid, size_t lds, hipStream_t st,
 #define FXC_KH(V) \
   case V: return launch_mfma_kh<kMode, V>(grid, lds, st, hi, lo, nsta, nspec, sc, tiles, trows, dst, xpitch, scale);
   switch (kh) {
-    FXC_KH(16) FXC_KH(32) FXC_KH(64) FXC_KH(96) FXC_KH(128)
+    FXC_KH(16) FXC_KH(32) FXC_KH(48) FXC_KH(64) FXC_KH(80) FXC_KH(96) FXC_KH(112) FXC_KH(128)
     default: break;
   }
 #undef FXC_KH
@@ -1553,8 +1553,8 @@ extern "C" int fxc_xc_plan_info(const fxc_xc_plan *p, int *nfft, int *half_cc, i
 // group's operand depth K is its plan's kh, fixed by its largest window
 // count; segments with fewer windows carry zeros in the rest.  Grouping 0
 // puts every segment in one group (fewest launches, deepest K); grouping 1
-// makes one group per K depth (16, 32, 64, ...), so short segments are not
-// padded to the longest.  fxc_xc_batch_tune times both on the caller's data
+// makes one group per K depth in steps of 16 windows (16, 32, 48, ...), so
+// short segments are not padded to the longest.  fxc_xc_batch_tune times both on the caller's data
 // and keeps the faster.
 //
 // The contraction of a group divides by the group's window count; each row's
@@ -1606,7 +1606,11 @@ struct fxc_xc_batch {
   hipGraphExec_t exec = nullptr;
 };
 
+// Operand depth of a plan: the plan's own rule (16, then multiples of 32,
+// where the receiver's B2 half-swap moves whole K steps) and the finer
+// classes of grouping 1 (multiples of 16: less zero padding, B2 loaded).
 static int batch_kh(int nstep) { return nstep <= 16 ? 16 : (nstep + 31) / 32 * 32; }
+static int batch_depth(int nstep) { return (nstep + 15) / 16 * 16; }
 
 static void batch_free_groups(fxc_xc_batch *b) {
   for (auto &g : b->groups) {
@@ -1637,7 +1641,7 @@ static int batch_build(fxc_xc_batch *b) {
   std::map<int, int> by_kh;  // K depth -> group index (grouping 1)
   std::vector<int> seg_group(nseg);
   for (int s = 0; s < nseg; ++s) {
-    const int key = b->grouping ? batch_kh(b->nstep[s]) : 0;
+    const int key = b->grouping ? batch_depth(b->nstep[s]) : 0;
     auto it = by_kh.find(key);
     if (it == by_kh.end()) {
       it = by_kh.emplace(key, (int)b->groups.size()).first;
@@ -1673,6 +1677,7 @@ static int batch_build(fxc_xc_batch *b) {
     }
     g.nsta_dst = std::max(off, 1);
     FXC_TRY(fxc_xc_plan_create(&g.plan, b->nspec, gstep, b->dt, b->half_cc, b->precision));
+    if (b->grouping) g.plan->kh = batch_depth(gstep);  // before any workspace exists
     FXC_TRY(fxc_xc_set_chunk_rows(g.plan, b->max_rows));
     FXC_TRY(fxc_xc_set_pairs(g.plan, g.nsta_dst, src.data(), rec.data(), (long long)src.size()));
     std::vector<int> row_out(g.plan->h_row_task.size());

@@ -25,7 +25,7 @@ def _spectra(rng, nstep, nsta, seg, nfft, dt, common=0.5):
     return (np.fft.rfft(x, n=nfft, axis=-1) * dt).astype(np.complex64)
 
 
-def _segments(rng, nseg, seg, nfft, dt, sta_range=(2, 45), steps=(3, 6, 17, 24, 40)):
+def _segments(rng, nseg, seg, nfft, dt, sta_range=(2, 45), steps=(3, 6, 17, 24, 40, 47, 70)):
     nsta = [int(rng.integers(*sta_range)) for _ in range(nseg)]
     nsta[1] = 1  # a segment without pairs
     nstep = [int(rng.choice(steps)) for _ in range(nseg)]
@@ -70,9 +70,7 @@ def test_batch_matches_oracle(gpu, gaps, grouping):
     cc = b.correlate(spec, vmask).cpu().numpy()
     info = b.info()
     assert info["grouping"] == grouping
-    assert info["groups"] == (1 if grouping == 0 else len({min(128, 16 if w <= 16 else
-                                                              (w + 31) // 32 * 32)
-                                                          for w in nstep}))
+    assert info["groups"] == (1 if grouping == 0 else len({(w + 15) // 16 for w in nstep}))
     _check_segments(b, cc, specs, valid, gaps, nfft, H, dt)
     b.close()
 
