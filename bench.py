@@ -623,10 +623,7 @@ def batch_measure(nseg=64, seed=13, reps=3, budget_gb=235.0, max_rows=65536):
     nsta = rng.integers(20, 401, size=nseg)
     nstep = rng.integers(6, 49, size=nseg)
 
-    def depth(w):  # operand depth of one plan (grouping 0)
-        return 16 if w <= 16 else (w + 31) // 32 * 32
-
-    def depth16(w):  # operand depth classes of grouping 1
+    def depth(w):  # operand depth of a plan: windows rounded up to 16
         return (w + 15) // 16 * 16
 
     def gbytes(k):
@@ -636,8 +633,8 @@ def batch_measure(nseg=64, seed=13, reps=3, budget_gb=235.0, max_rows=65536):
         ws = max_rows * 18528 * 8.0
         padded = (a + 31) // 32 * 32
         one = float(padded.sum()) * nspec * 2 * depth(int(w.max())) * 4 + ws
-        kinds = sorted({depth16(int(x)) for x in w})
-        per = sum(float(padded[[depth16(int(x)) == d for x in w]].sum()) * nspec * 2 * d * 4
+        kinds = sorted({depth(int(x)) for x in w})
+        per = sum(float(padded[[depth(int(x)) == d for x in w]].sum()) * nspec * 2 * d * 4
                   for d in kinds) + len(kinds) * ws
         return (fixed + one) / 1e9, (fixed + per) / 1e9
 

@@ -118,8 +118,8 @@ int fxc_xc_batch_destroy(fxc_xc_batch *batch);
 int fxc_xc_batch_info(const fxc_xc_batch *batch, long long *ntasks, long long *spec_elems,
                       long long *valid_elems, int *ngroups, int *grouping);
 /* grouping 0: every segment in one plan (operand depth of the longest
- * segment: 16 or a multiple of 32 windows); 1: one plan per operand depth in
- * steps of 16 windows (16, 32, 48, ..., 128). */
+ * segment); 1: one plan per operand depth, the window count rounded up to 16
+ * (16, 32, 48, ..., 128). */
 int fxc_xc_batch_set_grouping(fxc_xc_batch *batch, int grouping);
 int fxc_xc_batch_correlate(fxc_xc_batch *batch, const float *d_spec, const unsigned char *d_valid,
                            float *d_cc, void *stream);
