@@ -93,9 +93,10 @@ __global__ void station_scale_kernel(const unsigned int *__restrict__ amax_bits,
 }
 
 // Split pass of the MFMA forms.  Operand layout [bin][K group][station][8]:
-// K' = 2*kh values per (station, bin) -- window w's real part at K index w,
-// its imaginary part at kh + w, zero for w >= nstep -- cut into groups of 8
-// consecutive K indices (16 bytes), with the stations of one group adjacent,
+// K' = 2*kh values per (station, bin) -- window w's real part at K index 2w,
+// its imaginary part at 2w + 1, zero for w >= nstep -- cut into groups of 8
+// consecutive K indices (16 bytes, 4 windows), with the stations of one
+// group adjacent,
 // so the 16 lanes that hold one K group of 16 stations in an MFMA operand
 // read 256 contiguous bytes.  One workgroup converts 64 stations x 16 bins:
 // each block of 8 windows is staged through LDS so the reads run along the
@@ -140,12 +141,12 @@ __device__ __forceinline__ void split_part(const float2 *__restrict__ spec, int 
       if (sidx >= nsta || k >= nspec) continue;
       const float sc = (kMode == FXC_PREC_FP16) ? inv_scale[sta_off + sidx] : 1.f;
 #pragma unroll
-      for (int part = 0; part < 2; ++part) {  // 0: real parts, 1: imaginary parts
+      for (int part = 0; part < 2; ++part) {  // windows wb + 4 part .. + 3
         unsigned short h[8], l[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-          const float2 z = tile[j][ss][kk];
-          const float v = (part == 0 ? z.x : z.y) * sc;
+          const float2 z = tile[4 * part + j / 2][ss][kk];
+          const float v = ((j & 1) ? z.y : z.x) * sc;
           if (kMode == FXC_PREC_FP16) {
             h[j] = __builtin_bit_cast(unsigned short, (_Float16)v);
             l[j] = 0;
@@ -155,7 +156,7 @@ __device__ __forceinline__ void split_part(const float2 *__restrict__ spec, int 
             l[j] = bf16_bits_rne(v - bf16_bits_to_float(hb));
           }
         }
-        const int g = part * (kh / 8) + wb / 8;
+        const int g = 2 * (wb / 8) + part;
         const size_t o = (((size_t)k * ngrp + g) * nsta_dst + sta_off + sidx) * 8;
         uint4 hv;
         hv.x = h[0] | ((unsigned)h[1] << 16); hv.y = h[2] | ((unsigned)h[3] << 16);
@@ -324,63 +325,61 @@ contract_fp32_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int n
 // b2_s = [Im | -Re], so conj(a)*b = Cr + i Ci.  Results are staged in LDS as
 // [pair][bin] and written as 128-byte rows.
 // Operand fragments of one lane for one bin and one split part (hi or lo):
-// K' = 2*kh = 32*NK32 with kh a multiple of 16, so every K step is one
-// 16x16x32 MFMA: a dependent accumulate chain never mixes MFMA shapes.
-// Source side: A; receiver side: B1 = [Re|Im] and B2 = [Im|-Re].
-template <int NK32>
+// K' = 2*kh = 16*NK16 with kh a multiple of 8: NK16/2 K steps of one
+// 16x16x32 MFMA each, and for odd NK16 a last step of one 16x16x16 MFMA, so
+// a window count such as 24 is not padded to 32.
+// Source side: A = (Re, Im) per window; receiver side B1 = (Re, Im) and
+// B2 = (Im, -Re) per window, so A.B1 = Re(conj(a) b), A.B2 = Im(conj(a) b).
+// B2 is formed in registers from B1: each 32-bit word holds one window's
+// (Re, Im), and B2's word is that word rotated by 16 bits with the new high
+// half negated.
+template <int NK16>
 struct SrcFrag {
-  uint4 a32[NK32];
+  static constexpr int kFull = NK16 / 2;
+  static constexpr bool kHalf = (NK16 & 1) != 0;
+  uint4 a32[kFull > 0 ? kFull : 1];
+  uint2 a16;  // the 16-deep last step (kHalf)
 };
-// B2 = [Im | -Re] is B1 with its halves swapped and one half negated.  When
-// kh is a multiple of 32 (NK32 even) the swap moves whole K steps, so B2 is
-// formed in registers from B1; otherwise it is loaded separately.
-template <int NK32>
+template <int NK16>
 struct RecFrag {
-  static constexpr bool kDerived = (NK32 % 2) == 0;
-  uint4 b32[NK32];
-  uint4 c32[kDerived ? 1 : NK32];
+  static constexpr int kFull = NK16 / 2;
+  static constexpr bool kHalf = (NK16 & 1) != 0;
+  uint4 b32[kFull > 0 ? kFull : 1];
+  uint2 b16;
 };
 
-__device__ __forceinline__ uint4 neg_bits16(uint4 v) {
-  v.x ^= 0x80008000u; v.y ^= 0x80008000u; v.z ^= 0x80008000u; v.w ^= 0x80008000u;
-  return v;
+// (Re, Im) -> (Im, -Re) in every 32-bit word of 16-bit values.
+__device__ __forceinline__ unsigned swap_neg(unsigned d) {
+  return ((d >> 16) | (d << 16)) ^ 0x80000000u;
 }
+__device__ __forceinline__ uint4 swap_neg(uint4 v) {
+  return make_uint4(swap_neg(v.x), swap_neg(v.y), swap_neg(v.z), swap_neg(v.w));
+}
+__device__ __forceinline__ uint2 swap_neg(uint2 v) { return make_uint2(swap_neg(v.x), swap_neg(v.y)); }
 
 // a / b point at K group 0 of one (bin, station) in the split layout; K
-// group g sits gstride = 8*nsta elements further.  K step s of lane group
-// lane_g is K group 4s + lane_g.
-template <int NK32>
+// group g (8 values) sits gstride = 8*nsta elements further.  Full K step s
+// of lane group lane_g is K group 4s + lane_g; in the 16-deep last step lane
+// group lane_g holds 4 values, the half (lane_g & 1) of K group
+// 4*kFull + lane_g/2.
+template <int NK16>
 __device__ __forceinline__ void load_src(const unsigned short *a, size_t gstride, int lane_g,
-                                         SrcFrag<NK32> &o) {
+                                         SrcFrag<NK16> &o) {
+  constexpr int kFull = SrcFrag<NK16>::kFull;
 #pragma unroll
-  for (int s = 0; s < NK32; ++s) o.a32[s] = *(const uint4 *)(a + (4 * s + lane_g) * gstride);
+  for (int s = 0; s < kFull; ++s) o.a32[s] = *(const uint4 *)(a + (4 * s + lane_g) * gstride);
+  if constexpr (SrcFrag<NK16>::kHalf)
+    o.a16 = *(const uint2 *)(a + (4 * kFull + (lane_g >> 1)) * gstride + (lane_g & 1) * 4);
 }
 
-template <int NK32>
+template <int NK16>
 __device__ __forceinline__ void load_rec(const unsigned short *b, size_t gstride, int lane_g,
-                                         RecFrag<NK32> &o) {
-  constexpr int kgh = 2 * NK32;  // K groups per half (kh / 8)
+                                         RecFrag<NK16> &o) {
+  constexpr int kFull = RecFrag<NK16>::kFull;
 #pragma unroll
-  for (int s = 0; s < NK32; ++s) {
-    const int g = 4 * s + lane_g;
-    o.b32[s] = *(const uint4 *)(b + g * gstride);
-    if constexpr (!RecFrag<NK32>::kDerived) {
-      const bool hi_half = g >= kgh;
-      const uint4 c = *(const uint4 *)(b + (hi_half ? g - kgh : g + kgh) * gstride);
-      o.c32[s] = hi_half ? neg_bits16(c) : c;
-    }
-  }
-}
-
-// K step s of B2.
-template <int NK32>
-__device__ __forceinline__ uint4 rec_b2(const RecFrag<NK32> &y, int s) {
-  if constexpr (RecFrag<NK32>::kDerived) {
-    constexpr int h = NK32 / 2;
-    return s < h ? y.b32[s + h] : neg_bits16(y.b32[s - h]);
-  } else {
-    return y.c32[s];
-  }
+  for (int s = 0; s < kFull; ++s) o.b32[s] = *(const uint4 *)(b + (4 * s + lane_g) * gstride);
+  if constexpr (RecFrag<NK16>::kHalf)
+    o.b16 = *(const uint2 *)(b + (4 * kFull + (lane_g >> 1)) * gstride + (lane_g & 1) * 4);
 }
 
 template <int kMode>
@@ -394,14 +393,33 @@ __device__ __forceinline__ void mfma32(uint4 a, uint4 b, f32x4_t &acc) {
   }
 }
 
-// cr += A.B1, ci += A.B2 over all K steps of one bin.
-template <int kMode, int NK32>
-__device__ __forceinline__ void mfma_bin(const SrcFrag<NK32> &x, const RecFrag<NK32> &y,
-                                         f32x4_t &cr, f32x4_t &ci) {
+template <int kMode>
+__device__ __forceinline__ void mfma16(uint2 a, uint2 b, f32x4_t &acc) {
+  if constexpr (kMode == FXC_PREC_FP16) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4_t, a),
+                                                __builtin_bit_cast(f16x4_t, b), acc, 0, 0, 0);
+  } else {
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4_t, a),
+                                                    __builtin_bit_cast(short4_t, b), acc, 0, 0, 0);
+  }
+}
+
+// cr += A.B1, ci += A.B2 over the full K steps of one bin; the 16-deep last
+// step accumulates into hr, hi.  The two shapes never share an accumulator
+// chain: a 16x16x16 MFMA that took its accumulator straight from a 16x16x32
+// MFMA (issued back to back) gave wrong sums, so the 16-deep sums are kept
+// apart and added on the VALU.
+template <int kMode, int NK16>
+__device__ __forceinline__ void mfma_bin(const SrcFrag<NK16> &x, const RecFrag<NK16> &y,
+                                         f32x4_t &cr, f32x4_t &ci, f32x4_t &hr, f32x4_t &hi) {
 #pragma unroll
-  for (int s = 0; s < NK32; ++s) {
+  for (int s = 0; s < SrcFrag<NK16>::kFull; ++s) {
     mfma32<kMode>(x.a32[s], y.b32[s], cr);
-    mfma32<kMode>(x.a32[s], rec_b2(y, s), ci);
+    mfma32<kMode>(x.a32[s], swap_neg(y.b32[s]), ci);
+  }
+  if constexpr (SrcFrag<NK16>::kHalf) {
+    mfma16<kMode>(x.a16, y.b16, hr);
+    mfma16<kMode>(x.a16, swap_neg(y.b16), hi);
   }
 }
 
@@ -415,10 +433,10 @@ __device__ __forceinline__ int stage_index(int p, int b) { return p * kBinChunk 
 
 // Operands of one bin for the whole 32x32 tile: two 16-station source blocks
 // and two 16-station receiver blocks, hi (and lo for the split form).
-template <int NK32, bool kSplit>
+template <int NK16, bool kSplit>
 struct TileOps {
-  SrcFrag<NK32> sh[2], sl[kSplit ? 2 : 1];
-  RecFrag<NK32> rh[2], rl[kSplit ? 2 : 1];
+  SrcFrag<NK16> sh[2], sl[kSplit ? 2 : 1];
+  RecFrag<NK16> rh[2], rl[kSplit ? 2 : 1];
 };
 
 // Per-lane addressing of one work item (tile x 16-bin chunk).
@@ -455,12 +473,12 @@ __device__ __forceinline__ void item_addr(const TileDesc &td, int chunk, int nst
   ia.k0 = chunk * kBinChunk;
 }
 
-template <int kMode, int NK32, bool kSplit>
+template <int kMode, int NK16, bool kSplit>
 __device__ __forceinline__ void load_tile_ops(const unsigned short *__restrict__ hi,
                                               const unsigned short *__restrict__ lo,
                                               const ItemAddr &ia, int b, int nsta, int nspec,
-                                              int lane_g, TileOps<NK32, kSplit> &o) {
-  constexpr int kp = 32 * NK32;
+                                              int lane_g, TileOps<NK16, kSplit> &o) {
+  constexpr int kp = 16 * NK16;
   const size_t gstride = (size_t)nsta * 8;
   int k = ia.k0 + b;
   k = k < nspec ? k : nspec - 1;  // tail bins compute a valid bin, never stored
@@ -476,8 +494,8 @@ __device__ __forceinline__ void load_tile_ops(const unsigned short *__restrict__
   }
 }
 
-template <int kMode, int NK32, bool kSplit>
-__device__ __forceinline__ void compute_tile_bin(const TileOps<NK32, kSplit> &o, const ItemAddr &ia,
+template <int kMode, int NK16, bool kSplit>
+__device__ __forceinline__ void compute_tile_bin(const TileOps<NK16, kSplit> &o, const ItemAddr &ia,
                                                  int b, int lane_r, int lane_g, float2 *stage) {
 #pragma unroll
   for (int si = 0; si < 2; ++si)
@@ -485,10 +503,21 @@ __device__ __forceinline__ void compute_tile_bin(const TileOps<NK32, kSplit> &o,
     for (int sj = 0; sj < 2; ++sj) {
       f32x4_t cr = {0.f, 0.f, 0.f, 0.f};
       f32x4_t ci = {0.f, 0.f, 0.f, 0.f};
-      mfma_bin<kMode>(o.sh[si], o.rh[sj], cr, ci);  // hi.hi
+      f32x4_t hr = {0.f, 0.f, 0.f, 0.f};
+      f32x4_t hi = {0.f, 0.f, 0.f, 0.f};
+      mfma_bin<kMode>(o.sh[si], o.rh[sj], cr, ci, hr, hi);  // hi.hi
       if constexpr (kSplit) {
-        mfma_bin<kMode>(o.sl[si], o.rh[sj], cr, ci);  // lo.hi
-        mfma_bin<kMode>(o.sh[si], o.rl[sj], cr, ci);  // hi.lo
+        mfma_bin<kMode>(o.sl[si], o.rh[sj], cr, ci, hr, hi);  // lo.hi
+        mfma_bin<kMode>(o.sh[si], o.rl[sj], cr, ci, hr, hi);  // hi.lo
+      }
+      if constexpr (SrcFrag<NK16>::kHalf) {
+        if constexpr (SrcFrag<NK16>::kFull > 0) {
+          cr += hr;
+          ci += hi;
+        } else {
+          cr = hr;
+          ci = hi;
+        }
       }
       // D layout: lane holds rows 4*lane_g + r, column lane_r of the 16x16 block.
 #pragma unroll
@@ -508,7 +537,7 @@ __device__ __forceinline__ void compute_tile_bin(const TileOps<NK32, kSplit> &o,
 // pipelines its bins: the operands of the next bin -- of this item or of the
 // next one -- are loaded before the current bin's MFMAs, so load latency
 // overlaps the MFMAs, the LDS staging and the previous item's stores.
-template <int kMode, int NK32>
+template <int kMode, int NK16>
 __global__ void __launch_bounds__(512)
 contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short *__restrict__ lo,
                      int nsta, int nspec, const float *__restrict__ sta_scale,
@@ -521,7 +550,7 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
   const int lane = tid & 63;
   const int lane_r = lane & 15, lane_g = lane >> 4;
   constexpr bool kSplit = kMode == FXC_PREC_BF16X3;
-  constexpr bool kPipe = NK32 * (kSplit ? 2 : 1) <= 4;
+  constexpr bool kPipe = NK16 * (kSplit ? 2 : 1) <= 8;
   constexpr int kBins = kBinChunk / 8;  // bins per wave and item
   const int nchunks = (nspec + kBinChunk - 1) / kBinChunk;
   const long long nitems = (long long)ntiles * ((nchunks + kChunkGroup - 1) / kChunkGroup * kChunkGroup);
@@ -536,7 +565,7 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
   if (item >= nitems) return;
   ItemAddr ia;
   item_addr<kMode>(tiles[tile], chunk, nsta, lane_r, lane_g, scale, sta_scale, ia);
-  TileOps<NK32, kSplit> cur;
+  TileOps<NK16, kSplit> cur;
   if constexpr (kPipe) load_tile_ops<kMode>(hi, lo, ia, wave, nsta, nspec, lane_g, cur);
   // workspace rows of the tile's 1024 pairs, two per thread, loaded one item
   // ahead so the LDS fill after the barrier does not wait on memory
@@ -558,7 +587,7 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
     if constexpr (kPipe) {
 #pragma unroll
       for (int bi = 0; bi < kBins; ++bi) {
-        TileOps<NK32, kSplit> nxt;
+        TileOps<NK16, kSplit> nxt;
         if (bi + 1 < kBins)
           load_tile_ops<kMode>(hi, lo, ia, wave + 8 * (bi + 1), nsta, nspec, lane_g, nxt);
         else if (nitem < nitems)
@@ -571,7 +600,7 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
     } else {
 #pragma unroll 1
       for (int bi = 0; bi < kBins; ++bi) {
-        TileOps<NK32, kSplit> o;
+        TileOps<NK16, kSplit> o;
         load_tile_ops<kMode>(hi, lo, ia, wave + 8 * bi, nsta, nspec, lane_g, o);
         compute_tile_bin<kMode>(o, ia, wave + 8 * bi, lane_r, lane_g, stage);
       }
@@ -954,6 +983,18 @@ __global__ void scatter_rows_kernel(const float2 *__restrict__ src, long long sr
 
 using namespace fxc;
 
+// Operand depth kh of the MFMA forms for nstep windows (K' = 2*kh).  The
+// contraction's time follows K' (operand loads), so kh pads as little as
+// the compiled depths allow: 8 and 16, then multiples of 8 up to 48, then
+// 64, 96 and 128 (depths 80 and 112 would spill registers).  At depths
+// that are not multiples of 32 the receiver's [Im|-Re] operand is loaded
+// rather than formed in registers, which costs far less than the padding
+// (47 windows, 1000 stations: contraction 29.9 ms at kh 48, 45.3 ms at 64).
+static int xc_operand_depth(int nstep) {
+  if (nstep <= 48) return nstep <= 8 ? 8 : (nstep + 7) / 8 * 8;
+  return nstep <= 64 ? 64 : nstep <= 96 ? 96 : 128;
+}
+
 // ---------------------------------------------------------------------------
 // Plan object behind the C ABI.
 // ---------------------------------------------------------------------------
@@ -1076,12 +1117,7 @@ extern "C" int fxc_xc_plan_create(fxc_xc_plan **out, int nspec, int nstep, float
   p->dt = dt;
   p->precision = precision;
   p->xpitch = (nspec + 15) / 16 * 16;
-  // K' = 2*kh is a whole number of 16x16x32 steps.  kh is the window count
-  // rounded up to 16: at an odd multiple of 16 the receiver's [Im|-Re]
-  // operand is loaded rather than formed in registers, which costs far less
-  // than zero padding to the next multiple of 32 (47 windows: 29.9 ms at
-  // kh 48 against 45.3 ms at kh 64 for the contraction of 1000 stations)
-  p->kh = (nstep + 15) / 16 * 16;
+  p->kh = xc_operand_depth(nstep);
   p->chunk_rows_max = 0;
   p->lag_max_radix = lag_max_radix;
   const char *fixed_env = getenv("FXC_XC_LAG_FIXED");
@@ -1353,7 +1389,7 @@ static int launch_mfma_kh(dim3 grid, size_t lds, hipStream_t st, const unsigned 
                           const unsigned short *lo, int nsta, int nspec, const float *sc,
                           const TileDesc *tiles, const int *trows, float2 *dst, long long xpitch,
                           float scale) {
-  auto kern = contract_mfma_kernel<kMode, KH / 16>;
+  auto kern = contract_mfma_kernel<kMode, KH / 8>;
   // persistent grid: a multiple of 8 workgroups (one per CU, LDS-bound),
   // capped by the number of work items
   int dev = 0, ncu = 0;
@@ -1377,7 +1413,8 @@ static int launch_mfma_mode(int kh, dim3 grid, size_t lds, hipStream_t st,
 #define FXC_KH(V) \
   case V: return launch_mfma_kh<kMode, V>(grid, lds, st, hi, lo, nsta, nspec, sc, tiles, trows, dst, xpitch, scale);
   switch (kh) {
-    FXC_KH(16) FXC_KH(32) FXC_KH(48) FXC_KH(64) FXC_KH(80) FXC_KH(96) FXC_KH(112) FXC_KH(128)
+    FXC_KH(8) FXC_KH(16) FXC_KH(24) FXC_KH(32) FXC_KH(40) FXC_KH(48) FXC_KH(64) FXC_KH(96)
+    FXC_KH(128)
     default: break;
   }
 #undef FXC_KH
@@ -1609,9 +1646,7 @@ struct fxc_xc_batch {
   hipGraphExec_t exec = nullptr;
 };
 
-// Operand depth of a plan: the window count rounded up to 16 (the plan's own
-// rule, fxc_xc_plan_create).
-static int batch_depth(int nstep) { return (nstep + 15) / 16 * 16; }
+static int batch_depth(int nstep) { return xc_operand_depth(nstep); }
 
 static void batch_free_groups(fxc_xc_batch *b) {
   for (auto &g : b->groups) {
