@@ -347,6 +347,12 @@ struct RecFrag {
   uint4 b32[kFull > 0 ? kFull : 1];
   uint2 b16;
 };
+// B2 of one receiver fragment.
+template <int NK16>
+struct RecB2 {
+  uint4 c32[RecFrag<NK16>::kFull > 0 ? RecFrag<NK16>::kFull : 1];
+  uint2 c16;
+};
 
 // (Re, Im) -> (Im, -Re) in every 32-bit word of 16-bit values.
 __device__ __forceinline__ unsigned swap_neg(unsigned d) {
@@ -356,6 +362,13 @@ __device__ __forceinline__ uint4 swap_neg(uint4 v) {
   return make_uint4(swap_neg(v.x), swap_neg(v.y), swap_neg(v.z), swap_neg(v.w));
 }
 __device__ __forceinline__ uint2 swap_neg(uint2 v) { return make_uint2(swap_neg(v.x), swap_neg(v.y)); }
+
+template <int NK16>
+__device__ __forceinline__ void derive_b2(const RecFrag<NK16> &y, RecB2<NK16> &z) {
+#pragma unroll
+  for (int s = 0; s < RecFrag<NK16>::kFull; ++s) z.c32[s] = swap_neg(y.b32[s]);
+  if constexpr (RecFrag<NK16>::kHalf) z.c16 = swap_neg(y.b16);
+}
 
 // a / b point at K group 0 of one (bin, station) in the split layout; K
 // group g (8 values) sits gstride = 8*nsta elements further.  Full K step s
@@ -411,15 +424,16 @@ __device__ __forceinline__ void mfma16(uint2 a, uint2 b, f32x4_t &acc) {
 // apart and added on the VALU.
 template <int kMode, int NK16>
 __device__ __forceinline__ void mfma_bin(const SrcFrag<NK16> &x, const RecFrag<NK16> &y,
-                                         f32x4_t &cr, f32x4_t &ci, f32x4_t &hr, f32x4_t &hi) {
+                                         const RecB2<NK16> &y2, f32x4_t &cr, f32x4_t &ci,
+                                         f32x4_t &hr, f32x4_t &hi) {
 #pragma unroll
   for (int s = 0; s < SrcFrag<NK16>::kFull; ++s) {
     mfma32<kMode>(x.a32[s], y.b32[s], cr);
-    mfma32<kMode>(x.a32[s], swap_neg(y.b32[s]), ci);
+    mfma32<kMode>(x.a32[s], y2.c32[s], ci);
   }
   if constexpr (SrcFrag<NK16>::kHalf) {
     mfma16<kMode>(x.a16, y.b16, hr);
-    mfma16<kMode>(x.a16, swap_neg(y.b16), hi);
+    mfma16<kMode>(x.a16, y2.c16, hi);
   }
 }
 
@@ -497,6 +511,14 @@ __device__ __forceinline__ void load_tile_ops(const unsigned short *__restrict__
 template <int kMode, int NK16, bool kSplit>
 __device__ __forceinline__ void compute_tile_bin(const TileOps<NK16, kSplit> &o, const ItemAddr &ia,
                                                  int b, int lane_r, int lane_g, float2 *stage) {
+  // every B2 of the bin is formed before the first MFMA: a VALU result that
+  // an MFMA reads right away costs wait states
+  RecB2<NK16> b2h[2], b2l[kSplit ? 2 : 1];
+#pragma unroll
+  for (int sj = 0; sj < 2; ++sj) {
+    derive_b2(o.rh[sj], b2h[sj]);
+    if constexpr (kSplit) derive_b2(o.rl[sj], b2l[sj]);
+  }
 #pragma unroll
   for (int si = 0; si < 2; ++si)
 #pragma unroll
@@ -505,10 +527,10 @@ __device__ __forceinline__ void compute_tile_bin(const TileOps<NK16, kSplit> &o,
       f32x4_t ci = {0.f, 0.f, 0.f, 0.f};
       f32x4_t hr = {0.f, 0.f, 0.f, 0.f};
       f32x4_t hi = {0.f, 0.f, 0.f, 0.f};
-      mfma_bin<kMode>(o.sh[si], o.rh[sj], cr, ci, hr, hi);  // hi.hi
+      mfma_bin<kMode>(o.sh[si], o.rh[sj], b2h[sj], cr, ci, hr, hi);  // hi.hi
       if constexpr (kSplit) {
-        mfma_bin<kMode>(o.sl[si], o.rh[sj], cr, ci, hr, hi);  // lo.hi
-        mfma_bin<kMode>(o.sh[si], o.rl[sj], cr, ci, hr, hi);  // hi.lo
+        mfma_bin<kMode>(o.sl[si], o.rh[sj], b2h[sj], cr, ci, hr, hi);  // lo.hi
+        mfma_bin<kMode>(o.sh[si], o.rl[sj], b2l[sj], cr, ci, hr, hi);  // hi.lo
       }
       if constexpr (SrcFrag<NK16>::kHalf) {
         if constexpr (SrcFrag<NK16>::kFull > 0) {
@@ -537,8 +559,12 @@ __device__ __forceinline__ void compute_tile_bin(const TileOps<NK16, kSplit> &o,
 // pipelines its bins: the operands of the next bin -- of this item or of the
 // next one -- are loaded before the current bin's MFMAs, so load latency
 // overlaps the MFMAs, the LDS staging and the previous item's stores.
+// The LDS staging tile admits one workgroup (8 waves) per CU, 2 waves per
+// SIMD; waves_per_eu tells the scheduler so, and it then keeps a bin's
+// operand loads in flight together instead of trimming registers for an
+// occupancy the LDS never allows.
 template <int kMode, int NK16>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
 contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short *__restrict__ lo,
                      int nsta, int nspec, const float *__restrict__ sta_scale,
                      const TileDesc *__restrict__ tiles, const int *__restrict__ tile_rows,
@@ -602,6 +628,10 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
       for (int bi = 0; bi < kBins; ++bi) {
         TileOps<NK16, kSplit> o;
         load_tile_ops<kMode>(hi, lo, ia, wave + 8 * bi, nsta, nspec, lane_g, o);
+        // every operand load of the bin is issued before the first MFMA, so
+        // their latencies overlap instead of being waited for one by one
+        // (up to kh 64; deeper operands would not fit the registers)
+        if constexpr (NK16 <= 8) __builtin_amdgcn_sched_barrier(0);
         compute_tile_bin<kMode>(o, ia, wave + 8 * bi, lane_r, lane_g, stage);
       }
     }
