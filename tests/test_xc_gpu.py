@@ -28,7 +28,7 @@ def _all_pairs(nsta):
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16", "plain"])
 @pytest.mark.parametrize("nsta,nstep,seg,nfft,H", [(40, 5, 800, 882 * 2, 100), (9, 24, 1500, 1764, 300),
-                                                    (33, 47, 600, 1764, 200)])
+                                                    (33, 47, 600, 1764, 200), (17, 100, 600, 1764, 200)])
 def test_correlate_matches_oracle(gpu, precision, nsta, nstep, seg, nfft, H):
     import torch
     from fastxc_amd.ops import XcPlan
