@@ -1794,7 +1794,7 @@ extern "C" int fxc_xc_batch_create(fxc_xc_batch **out, int nspec, float dt, int 
   b->max_rows = max_rows;
   long long soff = 0, voff = 0, t = 0;
   for (int s = 0; s < nseg; ++s) {
-    if (nsta[s] < 1 || nstep[s] < 1 || batch_depth(nstep[s]) > 128) {
+    if (nsta[s] < 1 || nstep[s] < 1 || nstep[s] > 128) {
       set_error("fxc_xc_batch_create: segment %d has %d stations and %d windows (need >= 1 and "
                 "<= 128 windows)", s, nsta[s], nstep[s]);
       delete b;
