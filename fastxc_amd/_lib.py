@@ -32,6 +32,11 @@ class S2sConfig(ctypes.Structure):
                 ("responses", c_void_p), ("freq_lows", c_void_p)]
 
 
+class StackSpec(ctypes.Structure):
+    """fxc_stack_spec: |W| exponent and TF-PWS voice count (0 = every voice)."""
+    _fields_ = [("power", c_float), ("nvoices", c_int)]
+
+
 class FxcError(RuntimeError):
     pass
 
@@ -71,6 +76,10 @@ _SIGNATURES = {
     "fxc_pws": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "fxc_tfpws": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_float,
                           c_float, c_float, c_float, c_void_p, c_void_p]),
+    "fxc_pws_spec": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p,
+                             c_void_p, c_void_p]),
+    "fxc_tfpws_spec": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                               c_float, c_float, c_float, c_float, c_void_p, c_void_p, c_void_p]),
     "fxc_linear_stack": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "fxc_rotate_rtz": (c_int, [c_void_p, c_void_p, c_longlong, c_int, c_void_p, c_void_p]),
     "fxc_plan_paths": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,

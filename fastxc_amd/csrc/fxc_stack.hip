@@ -140,6 +140,14 @@ analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, StackFf
   }
 }
 
+// |W|^power from |W|^2: power 1 (the reference) is the square root, power 2
+// the square itself, other powers powf (spec mode).
+__device__ __forceinline__ float mag_pow(float mag2, float power) {
+  if (power == 1.f) return sqrtf(mag2);
+  if (power == 2.f) return mag2;
+  return mag2 > 0.f ? powf(mag2, 0.5f * power) : 0.f;
+}
+
 // PWS: one workgroup per pair.  Thread t owns transform positions
 // p = t + k kThreads (k < kPer), so W stays in registers and the LDS holds
 // only the group rows and the twiddles.
@@ -148,7 +156,7 @@ __global__ void __launch_bounds__(kThreads)
 pws_kernel(const float2 *__restrict__ A, const float *__restrict__ weights,
            const long long *__restrict__ offsets, const float *__restrict__ lin, StackFftDev F,
            const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i, int nbatch,
-           float *__restrict__ out) {
+           float power, float *__restrict__ out) {
   extern __shared__ float2 lds[];
   const int n = F.n, ld = F.ld;
   float2 *buf = lds;                     // [nbatch][ld]
@@ -201,7 +209,7 @@ pws_kernel(const float2 *__restrict__ A, const float *__restrict__ weights,
   float *o = out + (size_t)pair * n;
   for (int t = tid; t < n; t += kThreads) {
     const float2 w = Ws[stack_pos(F, t)];
-    o[t] = l[t] * sqrtf(w.x * w.x + w.y * w.y);
+    o[t] = l[t] * mag_pow(w.x * w.x + w.y * w.y, power);
   }
 }
 
@@ -220,7 +228,7 @@ __global__ void __launch_bounds__(kThreads)
 tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Alin,
                    const float *__restrict__ weights, const long long *__restrict__ offsets,
                    StackFftDev F, const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i,
-                   int nrows, int nbatch, int nfreq,
+                   int nrows, int nbatch, int nfreq, float power,
                    const unsigned char *__restrict__ chunk_active, float2 *__restrict__ out_spec) {
   extern __shared__ float2 lds[];
   const int n = F.n, ld = F.ld;
@@ -249,8 +257,10 @@ tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Ali
       } else if (c == 0) {
         g = 0.f;
       } else {
+        // even n: the shift c = n/2 meets neither branch of the reference
+        // (c < half, or a mirror n - c < half) and gets 0
         const int cm = c < half ? c : n - c;
-        g = expf(pi2 * (float)cm * (float)cm * 0.1f / ((float)f * (float)f));
+        g = cm < half ? expf(pi2 * (float)cm * (float)cm * 0.1f / ((float)f * (float)f)) : 0.f;
       }
       gauss[c] = g;
     }
@@ -303,9 +313,175 @@ tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Ali
     for (int k = 0; k < kPer; ++k) {
       const int p = tid + k * kThreads;
       if (p < n) {
-        const float mag = sqrtf(W[k].x * W[k].x + W[k].y * W[k].y);
+        const float mag = mag_pow(W[k].x * W[k].x + W[k].y * W[k].y, power);
         sx += L[p].x * mag;
         sy += L[p].y * mag;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      sx += __shfl_down(sx, off);
+      sy += __shfl_down(sy, off);
+    }
+    if ((tid & 63) == 0) {
+      red_x[tid >> 6] = sx;
+      red_y[tid >> 6] = sy;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      float tx = 0.f, ty = 0.f;
+      for (int i = 0; i < kThreads / 64; ++i) {
+        tx += red_x[i];
+        ty += red_y[i];
+      }
+      out_spec[(size_t)pair * n + f] = make_float2(tx, ty);
+    }
+  }
+}
+
+// Gaussian window of voice f (gaussianModulateSub, scale 0.1) into gauss[0, n).
+__device__ __forceinline__ void tfpws_gaussian(float *gauss, int n, int f) {
+  const int half = (n + 1) / 2;
+  const float pi2 = -2.0f * (float)M_PI * (float)M_PI;
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    float g;
+    if (f == 0) {
+      g = 1.f;
+    } else if (c == 0) {
+      g = 0.f;
+    } else {
+      const int cm = c < half ? c : n - c;
+      g = cm < half ? expf(pi2 * (float)cm * (float)cm * 0.1f / ((float)f * (float)f)) : 0.f;
+    }
+    gauss[c] = g;
+  }
+}
+
+// c[k] = |W_v(p)|^power at the positions p = tid + k kThreads this thread
+// owns, W_v = sum_g w_g ST_g,v / (|ST_g,v|' n), the groups transformed up to
+// nrows at a time.  Starts and ends with a barrier.
+template <int kThreads, int kPer>
+__device__ __forceinline__ void tfpws_voice_weight(float (&c)[kPer], int v, const float2 *Ag,
+                                                   const float *weights, long long g0,
+                                                   long long g1, const StackFftDev &F,
+                                                   const float2 *twl, float *gauss, float2 *buf,
+                                                   int nrows, float power) {
+  const int n = F.n, ld = F.ld, tid = threadIdx.x;
+  const float nf = (float)n;
+  __syncthreads();
+  tfpws_gaussian(gauss, n, v);
+  float2 W[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) W[k] = make_float2(0.f, 0.f);
+  for (long long gb = g0; gb < g1; gb += nrows) {
+    const int nb = (int)std::min<long long>(nrows, g1 - gb);
+    __syncthreads();
+    for (int i = tid; i < nb * n; i += kThreads) {
+      const int row = i / n, cc = i - row * n;
+      int src = v + cc;
+      src = src >= n ? src - n : src;
+      buf[row * ld + cc] = cscale(Ag[(gb + row) * n + src], gauss[cc]);
+    }
+    __syncthreads();
+    stack_fft(buf, nb, F, twl, 1.f);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int p = tid + k * kThreads;
+      if (p < n) {
+        float2 acc = W[k];
+        for (int b = 0; b < nb; ++b) {
+          const float2 z = buf[b * ld + p];
+          float fa = sqrtf(z.x * z.x + z.y * z.y);
+          if (fa <= 1e-7f) fa = 1.f;
+          const float wg = weights[gb + b];
+          acc.x += wg * z.x / (fa * nf);
+          acc.y += wg * z.y / (fa * nf);
+        }
+        W[k] = acc;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) c[k] = mag_pow(W[k].x * W[k].x + W[k].y * W[k].y, power);
+  __syncthreads();
+}
+
+// Spec-mode TF-PWS (no reference counterpart; the reference transforms the
+// groups at every one of the n/2+1 voices, tfpws_compute.cu:224).  The
+// groups are transformed only at nvoices voice bins
+// v_j = floor(j (nfreq-1) / (nvoices-1)); every bin f still takes the linear
+// stack's voice, weighted by c_f = (1-t) |W_vj|^p + t |W_vj+1|^p with
+// t = (f - v_j) / (v_j+1 - v_j) and v_j <= f < v_j+1.  One workgroup per
+// (pair, 32 consecutive bins) walks its bins in order and keeps the weights
+// of the current pair of voices in registers, so each voice is transformed
+// once per chunk that needs it.
+template <int kThreads, int kPer>
+__global__ void __launch_bounds__(kThreads)
+tfpws_interp_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Alin,
+                    const float *__restrict__ weights, const long long *__restrict__ offsets,
+                    StackFftDev F, const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i,
+                    int nrows, int nfreq, int nvoices, float power,
+                    const unsigned char *__restrict__ chunk_active, float2 *__restrict__ out_spec) {
+  extern __shared__ float2 lds[];
+  const int n = F.n, ld = F.ld;
+  float *gauss = (float *)lds;                       // [n] floats
+  float2 *buf = lds + (n + 1) / 2;                   // [nrows][ld]
+  float2 *twl = buf + (size_t)nrows * ld;
+  __shared__ float red_x[kThreads / 64], red_y[kThreads / 64];
+  const int pair = blockIdx.y;
+  const int chunk = blockIdx.x;
+  if (chunk_active && !chunk_active[chunk]) return;  // uniform per workgroup
+  stage_stack_tw(twl, F, tw_f, tw_i);
+  const int f0 = chunk * kVoiceChunk;
+  const int f1 = std::min(nfreq, f0 + kVoiceChunk);
+  const long long g0 = offsets[pair], g1 = offsets[pair + 1];
+  const float2 *Al = Alin + (size_t)pair * n;
+  const int tid = threadIdx.x;
+  const long long va = nfreq - 1, vb = nvoices - 1;
+  float cL[kPer], cR[kPer];
+  int jL = -1, jR = -1;  // voices whose weights cL / cR hold
+  for (int f = f0; f < f1; ++f) {
+    // largest j with v_j <= f: floor(j a / b) <= f  <=>  j <= ((f+1) b - 1) / a
+    const int j = (int)std::min<long long>(vb, ((long long)(f + 1) * vb - 1) / va);
+    const int vj = (int)((long long)j * va / vb);
+    if (j != jL) {
+      if (j == jR) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) cL[k] = cR[k];
+      } else {
+        tfpws_voice_weight<kThreads, kPer>(cL, vj, Ag, weights, g0, g1, F, twl, gauss, buf,
+                                           nrows, power);
+      }
+      jL = j;
+    }
+    float t = 0.f;
+    if (vj != f) {
+      const int vj1 = (int)((long long)(j + 1) * va / vb);
+      if (jR != j + 1) {
+        tfpws_voice_weight<kThreads, kPer>(cR, vj1, Ag, weights, g0, g1, F, twl, gauss, buf,
+                                           nrows, power);
+        jR = j + 1;
+      }
+      t = (float)(f - vj) / (float)(vj1 - vj);
+    }
+    // the linear stack's voice f
+    __syncthreads();
+    tfpws_gaussian(gauss, n, f);
+    __syncthreads();
+    for (int cc = tid; cc < n; cc += kThreads) {
+      int src = f + cc;
+      src = src >= n ? src - n : src;
+      buf[cc] = cscale(Al[src], gauss[cc]);
+    }
+    __syncthreads();
+    stack_fft(buf, 1, F, twl, 1.f);
+    float sx = 0.f, sy = 0.f;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int p = tid + k * kThreads;
+      if (p < n) {
+        const float c = t == 0.f ? cL[k] : (1.f - t) * cL[k] + t * cR[k];
+        sx += buf[p].x * c;
+        sy += buf[p].y * c;
       }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -478,16 +654,24 @@ static int stack_common(const float *d_pre, const float *d_lin, const long long 
 // The voice kernel's per-thread W registers come in steps of 4 positions.
 static int launch_voice(int n, size_t lds, dim3 grid, hipStream_t st, const float2 *Ag,
                         const float2 *Alin, const float *wts, const long long *off,
-                        const StackFft &S, int nrows, int nbatch, int nfreq,
-                        const unsigned char *active, float2 *spec) {
+                        const StackFft &S, int nrows, int nbatch, int nfreq, float power,
+                        int nvoices, const unsigned char *active, float2 *spec) {
   const int per = (n + kVoiceThreads - 1) / kVoiceThreads;
 #define FXC_VOICE(P)                                                                        \
   if (per <= P) {                                                                          \
-    auto *k = tfpws_voice_kernel<kVoiceThreads, P>;                                        \
-    FXC_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                (int)lds));                                                \
-    hipLaunchKernelGGL(k, grid, dim3(kVoiceThreads), lds, st, Ag, Alin, wts, off, S.dev,   \
-                       S.tw_f, S.tw_i, nrows, nbatch, nfreq, active, spec);                \
+    if (nvoices > 0) {                                                                     \
+      auto *k = tfpws_interp_kernel<kVoiceThreads, P>;                                     \
+      FXC_HIP(hipFuncSetAttribute((const void *)k,                                         \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));  \
+      hipLaunchKernelGGL(k, grid, dim3(kVoiceThreads), lds, st, Ag, Alin, wts, off, S.dev, \
+                         S.tw_f, S.tw_i, nbatch, nfreq, nvoices, power, active, spec);     \
+    } else {                                                                               \
+      auto *k = tfpws_voice_kernel<kVoiceThreads, P>;                                      \
+      FXC_HIP(hipFuncSetAttribute((const void *)k,                                         \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));  \
+      hipLaunchKernelGGL(k, grid, dim3(kVoiceThreads), lds, st, Ag, Alin, wts, off, S.dev, \
+                         S.tw_f, S.tw_i, nrows, nbatch, nfreq, power, active, spec);       \
+    }                                                                                      \
     FXC_HIP(hipGetLastError());                                                            \
     return FXC_OK;                                                                         \
   }
@@ -513,9 +697,31 @@ static int check_stack_args(const float *pre, const float *wts, const long long 
 
 using namespace fxc;
 
+static int check_spec(const fxc_stack_spec *spec, float *power, int *nvoices, int n) {
+  *power = 1.f;
+  *nvoices = 0;
+  if (!spec) return FXC_OK;
+  FXC_CHECK_ARG(std::isfinite(spec->power) && spec->power > 0.f,
+                "stack: power must be finite and positive");
+  FXC_CHECK_ARG(spec->nvoices == 0 || spec->nvoices >= 2, "stack: nvoices must be 0 or >= 2");
+  *power = spec->power;
+  *nvoices = spec->nvoices >= n / 2 + 1 ? 0 : spec->nvoices;
+  return FXC_OK;
+}
+
 extern "C" int fxc_pws(const float *d_prestack, const float *d_weights, const long long *h_offsets,
                        const float *d_linear, int npairs, int n, float *d_out, void *stream) {
+  return fxc_pws_spec(d_prestack, d_weights, h_offsets, d_linear, npairs, n, nullptr, d_out,
+                      stream);
+}
+
+extern "C" int fxc_pws_spec(const float *d_prestack, const float *d_weights,
+                            const long long *h_offsets, const float *d_linear, int npairs, int n,
+                            const fxc_stack_spec *spec, float *d_out, void *stream) {
   FXC_TRY(check_stack_args(d_prestack, d_weights, h_offsets, d_linear, npairs, n, d_out));
+  float power = 1.f;
+  int nvoices = 0;
+  FXC_TRY(check_spec(spec, &power, &nvoices, n));
   StackFft S;
   FXC_TRY(stack_fft_plan(n, &S));
   // the rows must also hold W (n values) after the last batch
@@ -535,7 +741,8 @@ extern "C" int fxc_pws(const float *d_prestack, const float *d_weights, const lo
     FXC_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,   \
                                 (int)lds));                                                  \
     hipLaunchKernelGGL(k, dim3((unsigned)npairs), dim3(kVoiceThreads), lds, st, w.A,         \
-                       d_weights, w.offsets, d_linear, S.dev, S.tw_f, S.tw_i, nbatch, d_out); \
+                       d_weights, w.offsets, d_linear, S.dev, S.tw_f, S.tw_i, nbatch, power, \
+                       d_out);                                                                \
     FXC_HIP(hipGetLastError());                                                              \
   } else
   FXC_PWS(4)
@@ -550,7 +757,19 @@ extern "C" int fxc_tfpws(const float *d_prestack, const float *d_weights,
                          const long long *h_offsets, const float *d_linear, int npairs, int n,
                          int band_limited, float delta, float fmin, float fmax, float taper_hz,
                          float *d_out, void *stream) {
+  return fxc_tfpws_spec(d_prestack, d_weights, h_offsets, d_linear, npairs, n, band_limited, delta,
+                        fmin, fmax, taper_hz, nullptr, d_out, stream);
+}
+
+extern "C" int fxc_tfpws_spec(const float *d_prestack, const float *d_weights,
+                              const long long *h_offsets, const float *d_linear, int npairs, int n,
+                              int band_limited, float delta, float fmin, float fmax,
+                              float taper_hz, const fxc_stack_spec *spec, float *d_out,
+                              void *stream) {
   FXC_TRY(check_stack_args(d_prestack, d_weights, h_offsets, d_linear, npairs, n, d_out));
+  float power = 1.f;
+  int nvoices = 0;
+  FXC_TRY(check_spec(spec, &power, &nvoices, n));
   FXC_CHECK_ARG(!band_limited || (delta > 0.f && fmax > fmin), "fxc_tfpws: invalid band");
   StackFft S;
   FXC_TRY(stack_fft_plan(n, &S));
@@ -587,7 +806,7 @@ extern "C" int fxc_tfpws(const float *d_prestack, const float *d_weights,
   size_t lds = ((size_t)(n + 1) / 2 + (size_t)nrows * S.dev.ld + S.twc) * sizeof(float2);
   FXC_TRY(launch_voice(n, lds, dim3((unsigned)nchunks, (unsigned)npairs), st, w.A,
                        w.A + (size_t)ng * n, d_weights, w.offsets, S, nrows, nbatch, nfreq,
-                       (const unsigned char *)w.active, w.spec));
+                       power, nvoices, (const unsigned char *)w.active, w.spec));
   lds = ((size_t)S.dev.ld + S.twc) * sizeof(float2);
   FXC_HIP(hipFuncSetAttribute((const void *)tfpws_final_kernel,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

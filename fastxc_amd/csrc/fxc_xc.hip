@@ -1623,9 +1623,10 @@ extern "C" int fxc_xc_plan_info(const fxc_xc_plan *p, int *nfft, int *half_cc, i
 // group's operand depth K is its plan's kh, fixed by its largest window
 // count; segments with fewer windows carry zeros in the rest.  Grouping 0
 // puts every segment in one group (fewest launches, deepest K); grouping 1
-// makes one group per K depth in steps of 16 windows (16, 32, 48, ...), so
-// short segments are not padded to the longest.  fxc_xc_batch_tune times both on the caller's data
-// and keeps the faster.
+// makes one group per compiled operand depth (the window count rounded up
+// to 8, 16, 24, 32, 40, 48, 64, 96 or 128, as xc_operand_depth does), so
+// short segments are not padded to the longest.  fxc_xc_batch_tune times
+// both on the caller's data and keeps the faster.
 //
 // The contraction of a group divides by the group's window count; each row's
 // output scale then carries group_nstep / divisor, where the divisor is the

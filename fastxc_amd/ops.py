@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import PRECISIONS, S2sConfig, check, lib
+from ._lib import PRECISIONS, S2sConfig, StackSpec, check, lib
 
 
 def _stream_ptr(device) -> ctypes.c_void_p:
@@ -368,25 +368,38 @@ def _stack_args(prestack, weights, offsets, linear, out):
     return off, npairs, n, out
 
 
-def pws(prestack, weights, offsets, linear, out=None):
-    """Phase-weighted stack (power 1) of every pair; prestack rows are group means."""
+def pws(prestack, weights, offsets, linear, out=None, power=1.0):
+    """Phase-weighted stack of every pair; prestack rows are group means.
+    power 1 is the reference (fxc_pws); other powers go through fxc_pws_spec."""
     off, npairs, n, out = _stack_args(prestack, weights, offsets, linear, out)
-    check(lib().fxc_pws(prestack.data_ptr(), weights.data_ptr(), off.ctypes.data,
-                        linear.data_ptr(), npairs, n, out.data_ptr(),
-                        _stream_ptr(linear.device)), "fxc_pws")
+    if power == 1.0:
+        check(lib().fxc_pws(prestack.data_ptr(), weights.data_ptr(), off.ctypes.data,
+                            linear.data_ptr(), npairs, n, out.data_ptr(),
+                            _stream_ptr(linear.device)), "fxc_pws")
+        return out
+    spec = StackSpec(float(power), 0)
+    check(lib().fxc_pws_spec(prestack.data_ptr(), weights.data_ptr(), off.ctypes.data,
+                             linear.data_ptr(), npairs, n, ctypes.byref(spec), out.data_ptr(),
+                             _stream_ptr(linear.device)), "fxc_pws_spec")
     return out
 
 
-def tfpws(prestack, weights, offsets, linear, band=None, delta=1.0, out=None):
-    """Time-frequency PWS (S-transform, scale 0.1, power 1); prestack rows are group sums.
-    band: None or (fmin, fmax, taper_hz)."""
+def tfpws(prestack, weights, offsets, linear, band=None, delta=1.0, out=None, power=1.0,
+          nvoices=0):
+    """Time-frequency PWS (S-transform, scale 0.1); prestack rows are group sums.
+    band: None or (fmin, fmax, taper_hz).  power 1 over every voice (nvoices 0)
+    is the reference (fxc_tfpws); otherwise fxc_tfpws_spec."""
     off, npairs, n, out = _stack_args(prestack, weights, offsets, linear, out)
     bl = band is not None
     fmin, fmax, taper = band if bl else (0.0, 0.0, 0.0)
-    check(lib().fxc_tfpws(prestack.data_ptr(), weights.data_ptr(), off.ctypes.data,
-                          linear.data_ptr(), npairs, n, int(bl), float(delta), float(fmin),
-                          float(fmax), float(taper), out.data_ptr(), _stream_ptr(linear.device)),
-          "fxc_tfpws")
+    args = (prestack.data_ptr(), weights.data_ptr(), off.ctypes.data, linear.data_ptr(), npairs,
+            n, int(bl), float(delta), float(fmin), float(fmax), float(taper))
+    if power == 1.0 and nvoices == 0:
+        check(lib().fxc_tfpws(*args, out.data_ptr(), _stream_ptr(linear.device)), "fxc_tfpws")
+        return out
+    spec = StackSpec(float(power), int(nvoices))
+    check(lib().fxc_tfpws_spec(*args, ctypes.byref(spec), out.data_ptr(),
+                               _stream_ptr(linear.device)), "fxc_tfpws_spec")
     return out
 
 

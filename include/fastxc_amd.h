@@ -118,8 +118,8 @@ int fxc_xc_batch_destroy(fxc_xc_batch *batch);
 int fxc_xc_batch_info(const fxc_xc_batch *batch, long long *ntasks, long long *spec_elems,
                       long long *valid_elems, int *ngroups, int *grouping);
 /* grouping 0: every segment in one plan (operand depth of the longest
- * segment); 1: one plan per operand depth, the window count rounded up to 16
- * (16, 32, 48, ..., 128). */
+ * segment); 1: one plan per operand depth, the window count rounded up to
+ * the next compiled depth (8, 16, 24, 32, 40, 48, 64, 96, 128). */
 int fxc_xc_batch_set_grouping(fxc_xc_batch *batch, int grouping);
 int fxc_xc_batch_correlate(fxc_xc_batch *batch, const float *d_spec, const unsigned char *d_valid,
                            float *d_cc, void *stream);
@@ -186,6 +186,25 @@ int fxc_pws(const float *d_prestack, const float *d_weights, const long long *h_
 int fxc_tfpws(const float *d_prestack, const float *d_weights, const long long *h_offsets,
               const float *d_linear, int npairs, int n, int band_limited, float delta, float fmin,
               float fmax, float taper_hz, float *d_out, void *stream);
+/* Spec mode of both stacks (no reference counterpart; the reference is
+ * power 1 over every voice, native/pws/kernels/cuda.pws.cu:79-95 and
+ * native/tfpws/core/tfpws_compute.cu:224).  power: the exponent of |W|
+ * (> 0; 1 = the reference).  nvoices (TF-PWS only): 0 or >= n/2+1 = every
+ * voice; else the groups are transformed only at the voice bins
+ * v_j = floor(j (n/2) / (nvoices-1)), j < nvoices, and each bin's weight is
+ * interpolated linearly between its two voices.  A null spec is the
+ * reference. */
+typedef struct fxc_stack_spec {
+  float power;
+  int nvoices;
+} fxc_stack_spec;
+int fxc_pws_spec(const float *d_prestack, const float *d_weights, const long long *h_offsets,
+                 const float *d_linear, int npairs, int n, const fxc_stack_spec *spec,
+                 float *d_out, void *stream);
+int fxc_tfpws_spec(const float *d_prestack, const float *d_weights, const long long *h_offsets,
+                   const float *d_linear, int npairs, int n, int band_limited, float delta,
+                   float fmin, float fmax, float taper_hz, const fxc_stack_spec *spec,
+                   float *d_out, void *stream);
 
 /* Linear stack: d_out[p][j] = (float)(sum of (double)d_records[r][j] over
  * rows r = h_offsets[p] .. h_offsets[p+1]-1, divided in double by their

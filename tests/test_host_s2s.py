@@ -145,3 +145,63 @@ def test_file_to_file_and_xc_chain(gpu, tmp_path, nch, whiten, normalize):
     err = xc.rel_l2(cc, ref)
     print(f"chain rel_l2={err:.3e}")
     assert err < 1e-5
+
+
+@pytest.mark.gpu
+def test_sac_lengths_and_delta_tolerance(gpu, tmp_path):
+    """read_sac_buffer (native/sac2spec/io/sacio.c:158-211): a file longer than
+    npts = llround(sac_len/dt) is truncated, a shorter one zero-padded; the
+    delta check (io/parallel_io.c:12-73) accepts |delta - dt| <= max(1e-5 dt,
+    1e-6) and rejects the run otherwise."""
+    delta, npts = 0.1, 4000
+    rng = np.random.default_rng(21)
+    lens = [npts, npts + 517, npts - 1234, 100, npts + 1]
+    deltas = [delta, delta * (1 + 8e-6), delta, delta * (1 - 9e-6), delta]
+    os.makedirs(tmp_path / "sac", exist_ok=True)
+    rows, traces = [], []
+    for s, (n, d) in enumerate(zip(lens, deltas), start=1):
+        x = rng.standard_normal(n).astype(np.float32)
+        p = str(tmp_path / "sac" / ("S%02d.BHZ.sac" % s))
+        ff.write_sac(p, x, d, stla=40.0 + s, stlo=120.0 - s, kstnm="S%02d" % s, knetwk="XX",
+                     kcmpnm="BHZ")
+        rows.append(("2020.123.0400", str(s), "XX", "S%02d" % s, "--", "BHZ", p))
+        t = np.zeros(npts, np.float32)
+        t[: min(n, npts)] = x[:npts]
+        traces.append(t)
+
+    def write_index(path, rws):
+        with open(path, "w") as f:
+            f.write("timestamp\tnsl_id\tnetwork\tstation\tlocation\tcomponent\tsac_path\n")
+            for r_ in rws:
+                f.write("\t".join(r_) + "\n")
+
+    index = str(tmp_path / "sac_index.tsv")
+    write_index(index, rows)
+    filt = str(tmp_path / "filter.txt")
+    s2s.write_filter_file(filt, delta, "0.2/2")
+    L = "400/100/100/20"
+    ws = tmp_path / "ws"
+    r = run(S2S, ["-I", index, "-O", str(ws), "-C", "1", "-B", filt, "-L", L, "-W", "2/0",
+                  "-N", "3", "-T", "2"])
+    print(r.stderr[-3000:])
+    assert r.returncode == 0
+    assert "truncated" in r.stderr and "zero_padded" in r.stderr
+    sp = ws / "stepack"
+    packs = sorted(p for p in os.listdir(sp) if p.endswith(".stepack"))
+    _, _, spec = ff.read_stepack(str(sp / packs[0]))
+    plan = s2s.plan_lengths(400, delta, 100, 100, 20, s2s.read_filters(filt))
+    assert plan.npts == npts
+    want = _oracle_spectra(traces, plan, 1, 2, 3)
+    err = xc.rel_l2(spec, want)
+    print(f"sac lengths rel_l2={err:.3e}")
+    assert err < 1e-5
+    # one file 2e-5 off in delta: the run fails and names the file
+    bad = str(tmp_path / "sac" / "BAD.BHZ.sac")
+    ff.write_sac(bad, rng.standard_normal(npts).astype(np.float32), delta * (1 + 2e-5),
+                 kstnm="BAD", knetwk="XX", kcmpnm="BHZ")
+    index2 = str(tmp_path / "sac_index2.tsv")
+    write_index(index2, rows[:2] + [("2020.123.0400", "9", "XX", "BAD", "--", "BHZ", bad)])
+    r = run(S2S, ["-I", index2, "-O", str(tmp_path / "ws2"), "-C", "1", "-B", filt, "-L", L,
+                  "-W", "2/0", "-N", "3", "-T", "2"])
+    assert r.returncode != 0
+    assert "sac_delta_mismatch" in r.stderr and "BAD.BHZ.sac" in r.stderr

@@ -122,3 +122,81 @@ def test_stack_rejects_length_beyond_lds(gpu, kind):
     w = torch.full((2,), 0.5, dtype=torch.float32, device=gpu)
     with pytest.raises(FxcError, match="too long"):
         getattr(ops, kind)(z, w, np.array([0, 2]), z[:1].contiguous())
+
+
+@pytest.mark.parametrize("N", [2001, 4001])
+def test_pws_hilbert_rotation_closed_form(gpu, N):
+    """Two equal-weight groups, the second the first Hilbert-rotated by theta:
+    the reference's PWS (power 1) is lin |cos(theta/2)| exactly; the spec
+    mode's power 2 gives lin cos^2(theta/2)."""
+    import torch
+    from fastxc_amd import ops
+
+    rng = np.random.default_rng(40 + N)
+    theta = 2 * np.pi / 3
+    half = (N + 1) // 2
+    X1 = np.zeros(N, complex)
+    X1[1:half] = rng.standard_normal(half - 1) + 1j * rng.standard_normal(half - 1)
+    X1[N - half + 1:] = np.conj(X1[1:half][::-1])
+    X2 = X1.copy()
+    X2[1:half] *= np.exp(-1j * theta)
+    X2[N - half + 1:] = np.conj(X2[1:half][::-1])
+    x1, x2 = np.fft.ifft(X1).real, np.fft.ifft(X2).real
+    pre = np.stack([x1, x2]).astype(np.float32)
+    w = np.array([0.5, 0.5], np.float32)
+    lin = (0.5 * (pre[0] + pre[1]))[None].astype(np.float32)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(gpu)
+    c = abs(np.cos(theta / 2))
+    for power in (1.0, 2.0):
+        out = ops.pws(d(pre), d(w), np.array([0, 2]), d(lin), power=power).cpu().numpy()[0]
+        want = lin[0].astype(np.float64) * c ** power
+        err = rel_l2(out, want)
+        print(f"pws hilbert N={N} power={power} rel_l2={err:.3e}")
+        assert err < 1e-5
+        assert rel_l2(out, stack.pws(pre.astype(np.float64), w.astype(np.float64), lin[0],
+                                     power=power)) < 1e-5
+
+
+@pytest.mark.parametrize("N,power,nvoices,band", [(2001, 2.0, 0, None), (2001, 2.0, 256, None),
+                                                  (2001, 1.0, 256, (0.1, 0.3, 0.02)),
+                                                  (401, 1.5, 64, None), (4001, 2.0, 256, None)])
+def test_stack_spec_mode_matches_oracle(gpu, N, power, nvoices, band):
+    """Spec mode (power and voice count) of both stacks against the FP64 oracle."""
+    import torch
+    from fastxc_amd import ops
+
+    rng = np.random.default_rng(N + 7)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(gpu)
+    recs, pre, w, lin, off = _batch(rng, 2, 6, N, 2, "pws")
+    out = ops.pws(d(pre), d(w), off, d(lin), power=power).cpu().numpy()
+    for p, r in enumerate(recs):
+        g, wg, l = stack.pws_prepare(r, 2)
+        err = rel_l2(out[p], stack.pws(g, wg, l, power=power))
+        print(f"pws spec N={N} power={power} pair={p} rel_l2={err:.3e}")
+        assert err < 1e-5
+    recs, pre, w, lin, off = _batch(rng, 2, 6, N, 2, "tfpws")
+    out = ops.tfpws(d(pre), d(w), off, d(lin), band=band, delta=1.0, power=power,
+                    nvoices=nvoices).cpu().numpy()
+    for p, r in enumerate(recs):
+        s, wg, l = stack.tfpws_prepare(r, 2)
+        want = stack.tfpws(s, wg, l, band=band, delta=1.0, power=power, nvoices=nvoices)
+        err = rel_l2(out[p], want)
+        print(f"tfpws spec N={N} power={power} nvoices={nvoices} pair={p} rel_l2={err:.3e}")
+        assert err < 1e-4
+
+
+@pytest.mark.parametrize("N", [200, 2000])
+def test_tfpws_even_length_matches_oracle(gpu, N):
+    """Even n: the Gaussian is 0 at the shift c = n/2 (cuda.stransform.cu:119-141)."""
+    import torch
+    from fastxc_amd import ops
+
+    rng = np.random.default_rng(N + 3)
+    recs, pre, w, lin, off = _batch(rng, 2, 6, N, 3, "tfpws")
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(gpu)
+    out = ops.tfpws(d(pre), d(w), off, d(lin), delta=1.0).cpu().numpy()
+    for p, r in enumerate(recs):
+        s, wg, l = stack.tfpws_prepare(r, 3)
+        err = rel_l2(out[p], stack.tfpws(s, wg, l, delta=1.0))
+        print(f"tfpws even N={N} pair={p} rel_l2={err:.3e}")
+        assert err < 1e-4

@@ -19,6 +19,14 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-3
 
 
+def _operand_depth(nstep):
+    """The MFMA operand depth a window count compiles to: 8, multiples of 8
+    up to 48, then 64, 96, 128 (fxc_xc.hip xc_operand_depth)."""
+    if nstep <= 48:
+        return 8 if nstep <= 8 else (nstep + 7) // 8 * 8
+    return 64 if nstep <= 64 else 96 if nstep <= 96 else 128
+
+
 def _spectra(rng, nstep, nsta, seg, nfft, dt, common=0.5):
     base = rng.standard_normal((nstep, 1, seg))
     x = common * base + rng.standard_normal((nstep, nsta, seg))
@@ -70,7 +78,7 @@ def test_batch_matches_oracle(gpu, gaps, grouping):
     cc = b.correlate(spec, vmask).cpu().numpy()
     info = b.info()
     assert info["grouping"] == grouping
-    assert info["groups"] == (1 if grouping == 0 else len({(w + 15) // 16 for w in nstep}))
+    assert info["groups"] == (1 if grouping == 0 else len({_operand_depth(w) for w in nstep}))
     _check_segments(b, cc, specs, valid, gaps, nfft, H, dt)
     b.close()
 

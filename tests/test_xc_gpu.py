@@ -54,6 +54,33 @@ def test_correlate_matches_oracle(gpu, precision, nsta, nstep, seg, nfft, H):
     assert np.all(row_err < 10 * TOL[precision])
 
 
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16"])
+@pytest.mark.parametrize("nstep", [1, 12, 20, 28, 36, 44, 60, 90, 128])
+def test_every_operand_depth(gpu, precision, nstep):
+    """One window count per compiled MFMA operand depth (kh 8, 16, 24, 32, 40,
+    48, 64, 96, 128), two station tiles, against FP64."""
+    import torch
+    from fastxc_amd.ops import XcPlan
+
+    rng = np.random.default_rng(500 + nstep)
+    nsta, seg, nfft, H, dt = 37, 600, 1764, 200, 0.1
+    spec = _spectra(rng, nstep, nsta, seg, nfft, dt)
+    src, rec = _all_pairs(nsta)
+    plan = XcPlan(nfft // 2 + 1, nstep, dt, H, precision)
+    plan.set_pairs(nsta, src, rec)
+    sd = torch.from_numpy(spec).to(gpu)
+    xs = plan.cross_spectra(sd).cpu().numpy()
+    want_xs = xc.cross_spectrum(spec, src, rec)
+    err_xs = xc.rel_l2(xs, want_xs)
+    cc = plan.correlate(sd).cpu().numpy()
+    want = xc.lags(want_xs, nfft, H, dt)
+    err = xc.rel_l2(cc, want)
+    print(f"depth test {precision} nstep={nstep} xspec rel_l2={err_xs:.3e} cc rel_l2={err:.3e}")
+    assert err_xs < TOL[precision] and err < TOL[precision]
+    row_err = np.linalg.norm(cc - want, axis=1) / np.linalg.norm(want, axis=1)
+    assert np.all(row_err < 10 * TOL[precision])
+
+
 def test_production_length_fp32(gpu):
     """nfft = 37044 (2^2 3^3 7^3), H = 1000 at dt = 0.1, a handful of pairs."""
     import torch
