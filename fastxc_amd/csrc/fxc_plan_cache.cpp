@@ -90,18 +90,19 @@ int bluestein_length(int n) {
   return 0;
 }
 
-int plan_cache_bluestein(int n, const BluesteinPlan **out) {
+int plan_cache_bluestein(int n, const BluesteinPlan **out, bool create) {
   const int m = bluestein_length(n);
   FXC_CHECK_ARG(m > 0, "bluestein: no convolution length for n=%d", n);
   const FftPlan *fwd = nullptr, *inv = nullptr;
-  FXC_TRY(plan_cache_complex(m, -1, &fwd));
-  FXC_TRY(plan_cache_complex(m, 1, &inv));
+  FXC_TRY(plan_cache_complex(m, -1, &fwd, 32, create));
+  FXC_TRY(plan_cache_complex(m, 1, &inv, 32, create));
   int dev = 0;
   FXC_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_mu);
   auto key = std::make_tuple(dev, n);
   auto it = g_blue.find(key);
   if (it == g_blue.end()) {
+    FXC_CHECK_ARG(create, "bluestein plan of length %d is not prepared on device %d", n, dev);
     std::unique_ptr<BluesteinPlan> p(new BluesteinPlan());
     p->fwd = fwd;
     p->inv = inv;
@@ -116,13 +117,14 @@ int plan_cache_bluestein(int n, const BluesteinPlan **out) {
   return FXC_OK;
 }
 
-int plan_cache_complex(int n, int sign, const FftPlan **out, int max_radix) {
+int plan_cache_complex(int n, int sign, const FftPlan **out, int max_radix, bool create) {
   int dev = 0;
   FXC_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_mu);
   auto key = std::make_tuple(dev, n, sign, max_radix);
   auto it = g_cplx.find(key);
   if (it == g_cplx.end()) {
+    FXC_CHECK_ARG(create, "FFT plan of length %d is not prepared on device %d", n, dev);
     std::unique_ptr<FftPlan> p(new FftPlan());
     FXC_TRY(fft_plan_create(p.get(), n, sign, max_radix));
     it = g_cplx.emplace(key, std::move(p)).first;

@@ -11,8 +11,11 @@ namespace fxc {
 // including the FFT twiddle table.
 constexpr int kLdsComplexCapacity = 20480;
 
-// max_radix bounds the butterfly size (part of the cache key).
-int plan_cache_complex(int n, int sign, const FftPlan **out, int max_radix = 32);
+// max_radix bounds the butterfly size (part of the cache key).  With
+// create = false a plan that is not cached yet is an argument error
+// instead of being built (no device allocation).
+int plan_cache_complex(int n, int sign, const FftPlan **out, int max_radix = 32,
+                       bool create = true);
 int plan_cache_real(int n, int sign, const RealFftPlan **out, int max_radix = 32,
                     bool largest_last = false);
 void plan_cache_release();
@@ -33,6 +36,6 @@ struct BluesteinPlan {
 };
 // Smallest m >= 2n-1 that factor_radices accepts (0 when none up to 2^16).
 int bluestein_length(int n);
-int plan_cache_bluestein(int n, const BluesteinPlan **out);
+int plan_cache_bluestein(int n, const BluesteinPlan **out, bool create = true);
 
 }  // namespace fxc

@@ -582,20 +582,21 @@ struct StackWork {
   }
 };
 
-// Direct plan when n factors, else the Bluestein plan.
-static int stack_fft_plan(int n, StackFft *S) {
+// Direct plan when n factors, else the Bluestein plan.  create = false
+// only looks the plans up (no device allocation).
+static int stack_fft_plan(int n, StackFft *S, bool create = true) {
   int radix[kMaxPasses], np = 0;
   const FftPlan *pf = nullptr, *pi = nullptr;
   StackFftDev &d = S->dev;
   if (factor_radices(n, radix, &np)) {
-    FXC_TRY(plan_cache_complex(n, -1, &pf));
-    FXC_TRY(plan_cache_complex(n, 1, &pi));
+    FXC_TRY(plan_cache_complex(n, -1, &pf, 32, create));
+    FXC_TRY(plan_cache_complex(n, 1, &pi, 32, create));
     d.blue = 0;
     d.ld = n;
     d.chirp = d.bhat_f = d.bhat_i = nullptr;
   } else {
     const BluesteinPlan *bp = nullptr;
-    FXC_TRY(plan_cache_bluestein(n, &bp));
+    FXC_TRY(plan_cache_bluestein(n, &bp, create));
     pf = bp->fwd;
     pi = bp->inv;
     d.blue = 1;
@@ -626,13 +627,13 @@ static int stack_batch(const StackFft &S, size_t fixed, int extra_rows, const lo
   return (int)std::min<long long>(std::min<long long>(kStkBatch, gmax), room / S.dev.ld);
 }
 
-static int stack_common(const float *d_pre, const float *d_lin, const long long *h_off, int npairs,
-                        int n, StackWork &w, const StackFft &S, hipStream_t st) {
+// Device offsets and the analytic spectra of every group and linear stack,
+// into A [groups + pairs][n] and offsets [npairs + 1] (caller's buffers).
+static int stack_analytic(const float *d_pre, const float *d_lin, const long long *h_off,
+                          int npairs, int n, float2 *A, long long *offsets, const StackFft &S,
+                          hipStream_t st) {
   const long long ng = h_off[npairs];
-  const long long rows = ng + npairs;
-  FXC_TRY(dmalloc((void **)&w.A, (size_t)rows * n * sizeof(float2)));
-  FXC_TRY(dmalloc((void **)&w.offsets, (size_t)(npairs + 1) * sizeof(long long)));
-  FXC_HIP(hipMemcpyAsync(w.offsets, h_off, (size_t)(npairs + 1) * sizeof(long long),
+  FXC_HIP(hipMemcpyAsync(offsets, h_off, (size_t)(npairs + 1) * sizeof(long long),
                          hipMemcpyHostToDevice, st));
   // at most 3 rows per workgroup: three workgroups share a CU at n = 2001
   const int per = std::max(1, std::min(3, (kLdsComplexCapacity - S.twc) / S.dev.ld));
@@ -641,14 +642,22 @@ static int stack_common(const float *d_pre, const float *d_lin, const long long 
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (ng > 0) {
     hipLaunchKernelGGL(analytic_kernel, dim3((unsigned)ceil_div((size_t)ng, (size_t)per)),
-                       dim3(kStkThreads), lds, st, d_pre, ng, per, S.dev, S.tw_f, S.tw_i, w.A);
+                       dim3(kStkThreads), lds, st, d_pre, ng, per, S.dev, S.tw_f, S.tw_i, A);
     FXC_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(analytic_kernel, dim3((unsigned)ceil_div((size_t)npairs, (size_t)per)),
                      dim3(kStkThreads), lds, st, d_lin, (long long)npairs, per, S.dev, S.tw_f,
-                     S.tw_i, w.A + (size_t)ng * n);
+                     S.tw_i, A + (size_t)ng * n);
   FXC_HIP(hipGetLastError());
   return FXC_OK;
+}
+
+static int stack_common(const float *d_pre, const float *d_lin, const long long *h_off, int npairs,
+                        int n, StackWork &w, const StackFft &S, hipStream_t st) {
+  const long long rows = h_off[npairs] + npairs;
+  FXC_TRY(dmalloc((void **)&w.A, (size_t)rows * n * sizeof(float2)));
+  FXC_TRY(dmalloc((void **)&w.offsets, (size_t)(npairs + 1) * sizeof(long long)));
+  return stack_analytic(d_pre, d_lin, h_off, npairs, n, w.A, w.offsets, S, st);
 }
 
 // The voice kernel's per-thread W registers come in steps of 4 positions.
@@ -697,6 +706,31 @@ static int check_stack_args(const float *pre, const float *wts, const long long 
 
 using namespace fxc;
 
+// PWS proper over analytic spectra A and device offsets already in place.
+static int pws_launch(const StackFft &S, int nbatch, int npairs, const float2 *A,
+                      const float *d_weights, const long long *offsets, const float *d_linear,
+                      float power, float *d_out, hipStream_t st) {
+  const int n = S.dev.n;
+  const int per = (n + kVoiceThreads - 1) / kVoiceThreads;
+  const size_t lds = ((size_t)nbatch * S.dev.ld + S.twc) * sizeof(float2);
+#define FXC_PWS(P)                                                                            \
+  if (per <= P) {                                                                            \
+    auto *k = pws_kernel<kVoiceThreads, P>;                                                  \
+    FXC_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                (int)lds));                                                  \
+    hipLaunchKernelGGL(k, dim3((unsigned)npairs), dim3(kVoiceThreads), lds, st, A, d_weights, \
+                       offsets, d_linear, S.dev, S.tw_f, S.tw_i, nbatch, power, d_out);      \
+    FXC_HIP(hipGetLastError());                                                              \
+    return FXC_OK;                                                                           \
+  }
+  FXC_PWS(4)
+  FXC_PWS(8)
+  FXC_PWS(16)
+#undef FXC_PWS
+  FXC_CHECK_ARG(false, "fxc_pws: length %d too long", n);
+  return FXC_ERR_ARG;
+}
+
 static int check_spec(const fxc_stack_spec *spec, float *power, int *nvoices, int n) {
   *power = 1.f;
   *nvoices = 0;
@@ -734,21 +768,7 @@ extern "C" int fxc_pws_spec(const float *d_prestack, const float *d_weights,
   hipStream_t st = (hipStream_t)stream;
   StackWork w;
   FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, S, st));
-  const size_t lds = ((size_t)nbatch * S.dev.ld + S.twc) * sizeof(float2);
-#define FXC_PWS(P)                                                                            \
-  if (per <= P) {                                                                            \
-    auto *k = pws_kernel<kVoiceThreads, P>;                                                  \
-    FXC_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,   \
-                                (int)lds));                                                  \
-    hipLaunchKernelGGL(k, dim3((unsigned)npairs), dim3(kVoiceThreads), lds, st, w.A,         \
-                       d_weights, w.offsets, d_linear, S.dev, S.tw_f, S.tw_i, nbatch, power, \
-                       d_out);                                                                \
-    FXC_HIP(hipGetLastError());                                                              \
-  } else
-  FXC_PWS(4)
-  FXC_PWS(8)
-  FXC_PWS(16) {}
-#undef FXC_PWS
+  FXC_TRY(pws_launch(S, nbatch, npairs, w.A, d_weights, w.offsets, d_linear, power, d_out, st));
   FXC_HIP(hipStreamSynchronize(st));  // workspace is released on return
   return FXC_OK;
 }
