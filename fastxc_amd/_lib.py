@@ -80,6 +80,9 @@ _SIGNATURES = {
                              c_void_p, c_void_p]),
     "fxc_tfpws_spec": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                                c_float, c_float, c_float, c_float, c_void_p, c_void_p, c_void_p]),
+    "fxc_stack_prepare": (c_int, [c_int]),
+    "fxc_pws_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p,
+                                c_void_p]),
     "fxc_linear_stack": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "fxc_rotate_rtz": (c_int, [c_void_p, c_void_p, c_longlong, c_int, c_void_p, c_void_p]),
     "fxc_plan_paths": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,

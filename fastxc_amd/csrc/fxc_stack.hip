@@ -28,6 +28,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <vector>
 
 namespace fxc {
@@ -770,6 +772,84 @@ extern "C" int fxc_pws_spec(const float *d_prestack, const float *d_weights,
   FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, S, st));
   FXC_TRY(pws_launch(S, nbatch, npairs, w.A, d_weights, w.offsets, d_linear, power, d_out, st));
   FXC_HIP(hipStreamSynchronize(st));  // workspace is released on return
+  return FXC_OK;
+}
+
+// ------------------------------------------------------------------------
+// PWS without allocation, for callers that cannot call hipMalloc (code
+// under HIP graph capture, tight serving loops).  The workspace of
+// fxc_pws (offsets and analytic spectra) lives in a file-scope device array
+// of kNoallocScratchBytes per GPU, resolved to a device address once per
+// device by fxc_stack_prepare, which also builds the length's FFT plans.
+// Every call on a GPU shares that GPU's array: calls that overlap on one
+// GPU race.  Nothing in it has to start at zero (offsets and spectra are
+// written in full before they are read), so nothing is cleared.
+// ------------------------------------------------------------------------
+namespace fxc {
+constexpr size_t kNoallocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char *> g_scratch_ptr;  // device -> address of `scratch` there
+
+// Bytes of scratch fxc_pws_noalloc needs: offsets, then the spectra on a
+// 256-byte boundary.
+static size_t noalloc_bytes(long long groups, int npairs, int n, size_t *spec_off) {
+  *spec_off = ((size_t)(npairs + 1) * sizeof(long long) + 255) / 256 * 256;
+  return *spec_off + (size_t)(groups + npairs) * n * sizeof(float2);
+}
+}  // namespace fxc
+
+extern "C" int fxc_stack_prepare(int n) {
+  FXC_CHECK_ARG(n >= 2, "fxc_stack_prepare: invalid length %d", n);
+  int dev = 0;
+  FXC_HIP(hipGetDevice(&dev));
+  StackFft S;
+  FXC_TRY(stack_fft_plan(n, &S, true));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr.count(dev)) {
+    void *p = nullptr;
+    FXC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    FXC_CHECK_ARG(p, "fxc_stack_prepare: no device address for the scratch array");
+    g_scratch_ptr[dev] = (unsigned char *)p;
+  }
+  return FXC_OK;
+}
+
+extern "C" int fxc_pws_noalloc(const float *d_prestack, const float *d_weights,
+                               const long long *h_offsets, const float *d_linear, int npairs,
+                               int n, float *d_out, void *stream) {
+  FXC_TRY(check_stack_args(d_prestack, d_weights, h_offsets, d_linear, npairs, n, d_out));
+  int dev = 0;
+  FXC_HIP(hipGetDevice(&dev));
+  unsigned char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) base = it->second;
+  }
+  FXC_CHECK_ARG(base, "fxc_pws_noalloc: call fxc_stack_prepare(n) on device %d first", dev);
+  StackFft S;
+  FXC_TRY(stack_fft_plan(n, &S, false));
+  size_t spec_off = 0;
+  const size_t need = noalloc_bytes(h_offsets[npairs], npairs, n, &spec_off);
+  FXC_CHECK_ARG(need <= kNoallocScratchBytes,
+                "fxc_pws_noalloc: %d pairs with %lld groups of length %d need %zu bytes of "
+                "workspace, more than the %zu of the fixed scratch array",
+                npairs, h_offsets[npairs], n, need, kNoallocScratchBytes);
+  const int nbatch = std::max(stack_batch(S, 0, 0, h_offsets, npairs),
+                              (int)ceil_div((size_t)n, (size_t)S.dev.ld));
+  FXC_CHECK_ARG((size_t)nbatch * S.dev.ld + S.twc + kLdsReserve <= (size_t)kLdsComplexCapacity,
+                "fxc_pws_noalloc: length %d too long for the LDS transforms", n);
+  FXC_CHECK_ARG((n + kVoiceThreads - 1) / kVoiceThreads <= 16, "fxc_pws_noalloc: length %d too long",
+                n);
+  hipStream_t st = (hipStream_t)stream;
+  long long *offsets = (long long *)base;
+  float2 *A = (float2 *)(base + spec_off);
+  FXC_TRY(stack_analytic(d_prestack, d_linear, h_offsets, npairs, n, A, offsets, S, st));
+  FXC_TRY(pws_launch(S, nbatch, npairs, A, d_weights, offsets, d_linear, 1.f, d_out, st));
+  // complete on return, as fxc_pws (the host offsets are read by the copy)
+  FXC_HIP(hipStreamSynchronize(st));
   return FXC_OK;
 }
 

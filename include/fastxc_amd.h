@@ -206,6 +206,20 @@ int fxc_tfpws_spec(const float *d_prestack, const float *d_weights, const long l
                    float fmin, float fmax, float taper_hz, const fxc_stack_spec *spec,
                    float *d_out, void *stream);
 
+/* PWS without device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops).  Same arguments and result as fxc_pws (power 1); its
+ * workspace (offsets and analytic spectra, 8 (npairs+1) + 8 n (groups +
+ * npairs) bytes) lives in a fixed 64 MiB device array per GPU, held until
+ * the process ends; larger calls are FXC_ERR_ARG.  fxc_stack_prepare(n)
+ * must run first on the device (once per device and length): it builds
+ * the length-n FFT plans and resolves the array's device address (it
+ * allocates; fxc_pws_noalloc then calls no hipMalloc/hipFree).  Calls on
+ * one GPU share the array: calls that overlap on one GPU race. */
+int fxc_stack_prepare(int n);
+int fxc_pws_noalloc(const float *d_prestack, const float *d_weights, const long long *h_offsets,
+                    const float *d_linear, int npairs, int n, float *d_out, void *stream);
+
 /* Linear stack: d_out[p][j] = (float)(sum of (double)d_records[r][j] over
  * rows r = h_offsets[p] .. h_offsets[p+1]-1, divided in double by their
  * count).  Rows of a pair are summed in row order, so the result equals the
