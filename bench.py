@@ -57,8 +57,10 @@ def parse():
     ap.add_argument("--nstep", type=int, default=24)
     ap.add_argument("--nspec", type=int, default=18523)
     ap.add_argument("--half-cc", type=int, default=1000)
-    ap.add_argument("--precision", default="bf16x3,fp32,plain",
+    ap.add_argument("--precision", default="bf16x3,fp16,fp32,plain",
                     help="comma list; the first is the headline value")
+    ap.add_argument("--check-pairs", type=int, default=300,
+                    help="pairs of every form checked against an FP64 evaluation")
     ap.add_argument("--chunk-rows", type=int, default=0)
     ap.add_argument("--cpu-sample-sta", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -91,7 +93,48 @@ def parse():
     return ap.parse_args()
 
 
-def run_mode(precision, spec, src, rec, args, out):
+def sample_pairs(nsta, count, tile=32, seed=5):
+    """Row indices (into the i < j row-major pair list) checked against FP64:
+    the first and last pairs, pairs inside a diagonal and an off-diagonal
+    32-station tile and near the end of the array, then random pairs."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    pick = {(0, 1), (nsta - 2, nsta - 1), (0, nsta - 1)}
+    for a in range(3):
+        for b in range(a + 1, 4):
+            for p in ((tile + a, tile + b), (a, 5 * tile + b), (nsta - 40 + a, nsta - 5 + b)):
+                if 0 <= p[0] < p[1] < nsta:
+                    pick.add(p)
+    while len(pick) < min(count, nsta * (nsta - 1) // 2):
+        i, j = sorted(rng.choice(nsta, 2, replace=False))
+        pick.add((int(i), int(j)))
+    pairs = sorted(pick)
+    row = lambda i, j: i * nsta - i * (i + 1) // 2 + (j - i - 1)
+    return pairs, [row(i, j) for i, j in pairs]
+
+
+def fp64_rows(spec, pairs, half_cc, dt=0.1):
+    """FP64 host evaluation of the sampled pairs, in the reference's step
+    order: (1/nstep) sum_w conj(S_src) S_rec, x(-1)^k, inverse real FFT of
+    length 2(nspec-1), x 1/(nfft dt), lags [nfft/2-H, nfft/2+H]."""
+    import numpy as np
+
+    stations = sorted({s for p in pairs for s in p})
+    loc = {s: i for i, s in enumerate(stations)}
+    sub = spec[:, stations, :].cpu().numpy().astype(np.complex128)
+    nstep, _, nspec = sub.shape
+    nfft = 2 * (nspec - 1)
+    sign = np.where(np.arange(nspec) % 2 == 0, 1.0, -1.0)
+    out = np.empty((len(pairs), 2 * half_cc + 1))
+    for r, (i, j) in enumerate(pairs):
+        X = (np.conj(sub[:, loc[i]]) * sub[:, loc[j]]).mean(axis=0) * sign
+        y = np.fft.irfft(X, n=nfft) * nfft / (float(np.float32(nfft)) * dt)
+        out[r] = y[nfft // 2 - half_cc: nfft // 2 + half_cc + 1]
+    return out
+
+
+def run_mode(precision, spec, src, rec, args, out, check_rows):
     import torch
     from fastxc_amd.ops import XcPlan
 
@@ -111,8 +154,7 @@ def run_mode(precision, spec, src, rec, args, out):
     info = plan.pairs_info()
     plan.close()
     npairs = len(src)
-    sample = torch.linspace(0, npairs - 1, 257, device=out.device).round().long()
-    rows = out[sample].double().cpu()
+    rows = out[torch.tensor(check_rows, device=out.device)].double().cpu().numpy()
     ms = wall * 1e3 / args.steps
     return {
         "precision": precision,
@@ -174,9 +216,30 @@ def roofline(mode, args, npairs, default_workload):
     byts = algorithmic_bytes(args, npairs) / launches
     achieved = byts / (t / 1e3) / 1e9
     traffic = read_traffic(args.traffic_file, mode["precision"], name) if default_workload else None
-    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "launch_ms": round(t, 3), "algorithmic_bytes_per_launch": int(byts)}
+    rec = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+           "launch_ms": round(t, 3), "algorithmic_bytes_per_launch": int(byts)}
+    if traffic:
+        # the bandwidth the kernel actually moves (measured bytes / launch time)
+        rec["traffic_gbs"] = round(traffic / (t / 1e3) / 1e9, 1)
+    lim = read_limiter(args.traffic_file, name) if default_workload else None
+    if lim:
+        rec["limiter"] = lim
+    return rec
+
+
+def read_limiter(path, kernel):
+    """What the kernel's hardware counters say limits it (a 'limiter <kernel>
+    <what>' line of the committed traffic summary), e.g. VALU issue."""
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split(None, 2)
+                if len(parts) == 3 and parts[0] == "limiter" and parts[1] == kernel:
+                    return parts[2].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_threads(args):
@@ -776,15 +839,21 @@ def main():
     src = src.astype(np.int32)
     rec = rec.astype(np.int32)
     out = torch.empty((len(src), 2 * args.half_cc + 1), dtype=torch.float32, device=dev)
-    modes = [run_mode(p, spec, src, rec, args, out) for p in args.precision.split(",")]
-    # every form is checked against the tiled fp32 form on 257 rows spread over
-    # all pairs (first and last included): a cheap guard against a form that
-    # silently skips work
-    ref_rows = next((m["_rows"] for m in modes if m["precision"] == "fp32"), None)
+    check_pairs, check_rows = sample_pairs(args.nsta, args.check_pairs)
+    modes = [run_mode(p, spec, src, rec, args, out, check_rows) for p in args.precision.split(",")]
+    # every form against an FP64 host evaluation of the sampled pairs (first,
+    # last, tile-diagonal and off-diagonal ones included)
+    want = fp64_rows(spec, check_pairs, args.half_cc)
     for m in modes:
         rows = m.pop("_rows")
-        if ref_rows is not None:
-            m["rel_l2_vs_fp32_rows"] = float(((rows - ref_rows).norm() / ref_rows.norm()).item())
+        d = rows - want
+        m["rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(want))
+        m["max_abs_vs_fp64"] = float(np.abs(d).max())
+        m["max_row_rel_l2_vs_fp64"] = float(
+            (np.linalg.norm(d, axis=1) / np.linalg.norm(want, axis=1)).max())
+    fp64_check = {"pairs": len(check_pairs), "peak_abs": float(np.abs(want).max()),
+                  "sample": "first and last pairs, pairs inside a diagonal and an off-diagonal "
+                            "32-station tile, random pairs; FP64 numpy evaluation on the host"}
     default_workload = (args.nsta, args.nstep, args.nspec, args.half_cc, args.chunk_rows) == \
         (1000, 24, 18523, 1000, 0)
     head = modes[0]
@@ -808,6 +877,7 @@ def main():
                    "stations": args.nsta, "windows": args.nstep, "nspec": args.nspec,
                    "pairs": len(src), "lags": 2 * args.half_cc + 1, "chunk_rows": args.chunk_rows},
         "modes": modes,
+        "fp64_check": fp64_check,
         "roofline": roofline(head, args, len(src), default_workload),
     }
     for m in modes[1:]:

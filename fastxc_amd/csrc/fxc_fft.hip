@@ -8,9 +8,11 @@
 // native/tfpws/core/tfpws_workspace.cu:190-235).  Conventions follow the
 // DFT definition: forward exp(-i...), inverse exp(+i...), both unnormalised,
 // R2C returns bins 0..n/2.
+#include "fxc_fft_large.hpp"
 #include "fxc_fft_real.hpp"
 #include "fxc_plan_cache.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -126,6 +128,24 @@ extern "C" int fxc_fft_c2c(const float *d_in, long long in_dist, float *d_out, l
                            int n, long long batch, int inverse, float scale, void *stream) {
   FXC_CHECK_ARG(d_in && d_out && n > 0 && batch >= 0, "fxc_fft_c2c: invalid arguments");
   if (batch == 0) return FXC_OK;
+  if (n + tw_lds_count(n) > kLdsComplexCapacity) {
+    // beyond one workgroup's LDS: the four-step transform, per call scratch
+    const LargeFft *lp = nullptr;
+    FXC_TRY(large_plan(n, inverse ? 1 : -1, &lp));
+    const long long chunk = std::min<long long>(batch, 65535);
+    float2 *tmp = nullptr;
+    FXC_TRY(dmalloc((void **)&tmp, (size_t)chunk * n * sizeof(float2)));
+    int rc = FXC_OK;
+    for (long long r = 0; r < batch && rc == FXC_OK; r += chunk)
+      rc = large_c2c(lp, (const float2 *)d_in + r * in_dist, in_dist, (float2 *)d_out + r * out_dist,
+                     out_dist, std::min(chunk, batch - r), tmp, scale, (hipStream_t)stream);
+    if (rc == FXC_OK && hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
+      set_error("fxc_fft_c2c: stream synchronisation failed");
+      rc = FXC_ERR_HIP;
+    }
+    dfree(tmp);
+    return rc;
+  }
   const FftPlan *p = nullptr;
   FXC_TRY(plan_cache_complex(n, inverse ? 1 : -1, &p));
   const bool smooth = plan_in_mask(p->dev, kRadixSmooth);
@@ -154,6 +174,21 @@ extern "C" int fxc_fft_r2c(const float *d_in, long long in_dist, int in_len, flo
   FXC_CHECK_ARG(d_in && d_out && n >= 2 && (n % 2) == 0 && batch >= 0 && in_len >= 0,
                 "fxc_fft_r2c: invalid arguments");
   if (batch == 0) return FXC_OK;
+  if (real_lds_count(n / 2) > kLdsComplexCapacity) {
+    const long long chunk = std::min<long long>(batch, 65535);
+    float2 *work = nullptr;
+    FXC_TRY(dmalloc((void **)&work, (size_t)chunk * large_real_work(n) * sizeof(float2)));
+    int rc = FXC_OK;
+    for (long long r = 0; r < batch && rc == FXC_OK; r += chunk)
+      rc = large_r2c(n, d_in + r * in_dist, in_dist, in_len, (float2 *)d_out + r * out_dist,
+                     out_dist, std::min(chunk, batch - r), work, scale, (hipStream_t)stream);
+    if (rc == FXC_OK && hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
+      set_error("fxc_fft_r2c: stream synchronisation failed");
+      rc = FXC_ERR_HIP;
+    }
+    dfree(work);
+    return rc;
+  }
   const RealFftPlan *p = nullptr;
   FXC_TRY(plan_cache_real(n, -1, &p));
   FXC_CHECK_ARG(max_transforms_per_wg(p->half.dev, kFftThreads, kLdsComplexCapacity) >= 1,
@@ -174,6 +209,22 @@ extern "C" int fxc_fft_c2r(const float *d_in, long long in_dist, float *d_out, l
                     out_len <= n,
                 "fxc_fft_c2r: invalid arguments");
   if (batch == 0) return FXC_OK;
+  if (real_lds_count(n / 2) > kLdsComplexCapacity) {
+    const long long chunk = std::min<long long>(batch, 65535);
+    float2 *work = nullptr;
+    FXC_TRY(dmalloc((void **)&work, (size_t)chunk * large_real_work(n) * sizeof(float2)));
+    int rc = FXC_OK;
+    for (long long r = 0; r < batch && rc == FXC_OK; r += chunk)
+      rc = large_c2r(n, (const float2 *)d_in + r * in_dist, in_dist, d_out + r * out_dist,
+                     out_dist, out_len, std::min(chunk, batch - r), work, scale,
+                     (hipStream_t)stream);
+    if (rc == FXC_OK && hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
+      set_error("fxc_fft_c2r: stream synchronisation failed");
+      rc = FXC_ERR_HIP;
+    }
+    dfree(work);
+    return rc;
+  }
   const RealFftPlan *p = nullptr;
   FXC_TRY(plan_cache_real(n, 1, &p));
   FXC_CHECK_ARG(max_transforms_per_wg(p->half.dev, kFftThreads, kLdsComplexCapacity) >= 1,

@@ -1,6 +1,7 @@
 #include "fxc_plan_cache.hpp"
 
 #include "fxc_fft_device.hpp"
+#include "fxc_fft_large.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -160,4 +161,7 @@ void plan_cache_release() {
 
 }  // namespace fxc
 
-extern "C" void fxc_release_plans(void) { fxc::plan_cache_release(); }
+extern "C" void fxc_release_plans(void) {
+  fxc::large_release();
+  fxc::plan_cache_release();
+}

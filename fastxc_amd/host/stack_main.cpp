@@ -58,6 +58,8 @@ struct Args {
   size_t staged_limit = 200000;
   bool band = false;
   double fmin = 0, fmax = 0, taper = 0;
+  float power = 1.f;  // spec mode: exponent of |W| (the reference: 1)
+  int voices = 0;     // spec mode (TF-PWS): voice count (0: every voice, the reference)
 };
 
 void usage() {
@@ -70,7 +72,10 @@ void usage() {
             "  -S  output selection, must end in 1 (e.g. 001)\n"
             "  -B  sub-stack size (default 1)\n"
             "  -F  fmin/fmax band for the band-limited blend   -T taper (Hz)\n"
-            "  --progress <file>\n");
+            "  --progress <file>\n"
+            "  --power <p>   spec mode: |W|^p (default 1, the reference)\n"
+            "  --voices <v>  spec mode: v voices, weights interpolated between them\n"
+            "                (default 0: every voice, the reference)\n");
   else
     fprintf(stderr,
             "Usage: ncf_pws --sourcepack-list f --output-sourcepack dir [options]\n"
@@ -78,7 +83,8 @@ void usage() {
             "  -M, --gpu-memory-mib <MiB>     device memory cap\n"
             "  -B, --substack-size <n>        records per sub-stack (default 1)\n"
             "  -H, --staged-group-limit <n>   host staging bound in groups\n"
-            "  --progress <file>\n");
+            "  --progress <file>\n"
+            "  --power <p>                    spec mode: |W|^p (default 1, the reference)\n");
 }
 
 bool parse_args(int argc, char **argv, Args *a) {
@@ -97,6 +103,8 @@ bool parse_args(int argc, char **argv, Args *a) {
                                   {"substack-size", required_argument, 0, 'B'},
                                   {"staged-group-limit", required_argument, 0, 'H'},
                                   {"host-group-limit", required_argument, 0, 'H'},
+                                  {"power", required_argument, 0, 1003},
+                                  {"voices", required_argument, 0, 1004},
                                   {0, 0, 0, 0}};
   const char *shorts = kTf ? "I:O:G:S:B:M:F:T:" : "G:B:H:M:";
   int c, idx = 0;
@@ -116,6 +124,22 @@ bool parse_args(int argc, char **argv, Args *a) {
         }
         break;
       case 'M': a->mem = optarg; break;
+      case 1003: {
+        char *end = nullptr;
+        a->power = strtof(optarg, &end);
+        if (!end || *end || !(a->power > 0.f) || !std::isfinite(a->power)) {
+          FXH_LOG_ERROR("invalid_power", "value=\"%s\"", optarg);
+          return false;
+        }
+        break;
+      }
+      case 1004:
+        a->voices = atoi(optarg);
+        if (!kTf || a->voices < 0 || a->voices == 1) {
+          FXH_LOG_ERROR("invalid_voices", "value=\"%s\" (ncf_tfpws only; 0 or >= 2)", optarg);
+          return false;
+        }
+        break;
       case 'B':
         a->substack = atoi(optarg);
         if (a->substack < 2) a->substack = 1;
@@ -466,12 +490,18 @@ class Runner {
     HIPX(hipMemcpyAsync(d_w_, h_w.data(), rows * 4, hipMemcpyHostToDevice, st_));
     HIPX(hipMemcpyAsync(d_lin_, h_lin.data(), (size_t)np * n * 4, hipMemcpyHostToDevice, st_));
     int rc;
+    const fxc_stack_spec spec{a_.power, a_.voices};
+    const bool parity = a_.power == 1.f && a_.voices == 0;
     if (kTf)
-      rc = fxc_tfpws(d_rows_, d_w_, offs.data(), d_lin_, np, n, a_.band ? 1 : 0,
-                     batch_[0].hdr.delta, (float)a_.fmin, (float)a_.fmax, (float)a_.taper,
-                     d_out_, st_);
+      rc = parity ? fxc_tfpws(d_rows_, d_w_, offs.data(), d_lin_, np, n, a_.band ? 1 : 0,
+                              batch_[0].hdr.delta, (float)a_.fmin, (float)a_.fmax,
+                              (float)a_.taper, d_out_, st_)
+                  : fxc_tfpws_spec(d_rows_, d_w_, offs.data(), d_lin_, np, n, a_.band ? 1 : 0,
+                                   batch_[0].hdr.delta, (float)a_.fmin, (float)a_.fmax,
+                                   (float)a_.taper, &spec, d_out_, st_);
     else
-      rc = fxc_pws(d_rows_, d_w_, offs.data(), d_lin_, np, n, d_out_, st_);
+      rc = parity ? fxc_pws(d_rows_, d_w_, offs.data(), d_lin_, np, n, d_out_, st_)
+                  : fxc_pws_spec(d_rows_, d_w_, offs.data(), d_lin_, np, n, &spec, d_out_, st_);
     if (rc != FXC_OK) {
       FXH_LOG_ERROR("stack_compute_failed", "error=\"%s\"", fxc_last_error());
       return false;

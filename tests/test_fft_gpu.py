@@ -1,4 +1,6 @@
-"""Workgroup LDS FFT vs numpy (FP64), on the GPU."""
+"""Workgroup LDS FFT vs numpy (FP64), on the GPU; lengths whose transform
+exceeds one workgroup's LDS (complex 36000 and 37044, real 72000, 72576 and
+74088: the dt 0.05 s lengths) run the four-step transform over HBM."""
 import numpy as np
 import pytest
 
@@ -7,7 +9,7 @@ from oracle.xc import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n", [12, 98, 2001, 18000, 18144, 18522])
+@pytest.mark.parametrize("n", [12, 98, 2001, 18000, 18144, 18522, 36000, 37044])
 def test_c2c_forward_inverse(gpu, n):
     import torch
     from fastxc_amd import ops
@@ -21,7 +23,8 @@ def test_c2c_forward_inverse(gpu, n):
     assert rel_l2(b, np.fft.ifft(x.astype(np.complex128)) * n) < 1e-6
 
 
-@pytest.mark.parametrize("n,in_len", [(36000, 36000), (36288, 36000), (37044, 36000), (24, 20)])
+@pytest.mark.parametrize("n,in_len", [(36000, 36000), (36288, 36000), (37044, 36000), (24, 20),
+                                      (72000, 72000), (72576, 72000), (74088, 72000)])
 def test_r2c_c2r(gpu, n, in_len):
     import torch
     from fastxc_amd import ops
