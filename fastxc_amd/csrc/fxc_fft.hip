@@ -216,7 +216,7 @@ extern "C" int fxc_fft_c2r(const float *d_in, long long in_dist, float *d_out, l
     int rc = FXC_OK;
     for (long long r = 0; r < batch && rc == FXC_OK; r += chunk)
       rc = large_c2r(n, (const float2 *)d_in + r * in_dist, in_dist, d_out + r * out_dist,
-                     out_dist, out_len, std::min(chunk, batch - r), work, scale,
+                     out_dist, out_len, std::min(chunk, batch - r), work, (double)scale,
                      (hipStream_t)stream);
     if (rc == FXC_OK && hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
       set_error("fxc_fft_c2r: stream synchronisation failed");

@@ -124,16 +124,17 @@ __global__ void large_fold_kernel(const float2 *__restrict__ X, long long in_dis
   Z[row * M + k] = v;
 }
 
-// y[2j] = Re z[j], y[2j+1] = Im z[j], times scale, first out_len samples.
+// y[2j] = Re z[j], y[2j+1] = Im z[j], times scale (product in double), first
+// out_len samples.
 __global__ void large_unpack_kernel(const float2 *__restrict__ z, int M, float *__restrict__ y,
-                                    long long out_dist, int out_len, float scale) {
+                                    long long out_dist, int out_len, double scale) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= M) return;
   const long long row = blockIdx.y;
   const float2 v = z[row * M + j];
   float *yr = y + row * out_dist;
-  if (2 * j < out_len) yr[2 * j] = v.x * scale;
-  if (2 * j + 1 < out_len) yr[2 * j + 1] = v.y * scale;
+  if (2 * j < out_len) yr[2 * j] = (float)((double)v.x * scale);
+  if (2 * j + 1 < out_len) yr[2 * j + 1] = (float)((double)v.y * scale);
 }
 
 namespace {
@@ -259,7 +260,7 @@ int large_r2c(int n, const float *in, long long in_dist, int in_len, float2 *out
 }
 
 int large_c2r(int n, const float2 *in, long long in_dist, float *out, long long out_dist,
-              int out_len, long long rows, float2 *work, float scale, hipStream_t st) {
+              int out_len, long long rows, float2 *work, double scale, hipStream_t st) {
   FXC_CHECK_ARG(n >= 8 && n % 2 == 0, "fft: real length %d must be even", n);
   const int M = n / 2;
   const LargeFft *p = nullptr;

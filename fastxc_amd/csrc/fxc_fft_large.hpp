@@ -44,12 +44,18 @@ int large_c2c(const LargeFft *p, const float2 *in, long long in_dist, float2 *ou
 int large_r2c(int n, const float *in, long long in_dist, int in_len, float2 *out,
               long long out_dist, long long rows, float2 *work, float scale, hipStream_t st);
 // Real inverse transform of bins 0..n/2 (imaginary parts of 0 and n/2
-// ignored), the first out_len samples out.  work: 2 x rows x M complex.
+// ignored), the first out_len samples out, each (float)(value * scale) with
+// the product in double.  work: 2 x rows x M complex.
 int large_c2r(int n, const float2 *in, long long in_dist, float *out, long long out_dist,
-              int out_len, long long rows, float2 *work, float scale, hipStream_t st);
+              int out_len, long long rows, float2 *work, double scale, hipStream_t st);
 // Complex elements of work the real transforms of length n need per row.
 inline long long large_real_work(int n) { return 2LL * (n / 2); }
 // Frees every cached four-step plan's tables (fxc_release_plans).
 void large_release();
+
+// Fold of the real inverse: Z[k] = (X[k] + conj X[M-k]) + i w^k (X[k] - conj X[M-k]),
+// k < M, w^k = half[k] of the inverse plan; grid (ceil(M/256), rows), 256 threads.
+__global__ void large_fold_kernel(const float2 *__restrict__ X, long long in_dist, int M,
+                                  const float2 *__restrict__ half, float2 *__restrict__ Z);
 
 }  // namespace fxc

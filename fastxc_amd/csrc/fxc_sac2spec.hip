@@ -17,6 +17,7 @@
 // the normalisation divide and clip, whitening divide and taper, and the
 // phase-only option are all folded into those passes.  Sums the reference
 // takes in FP64 (mean, trend, running means) stay FP64.
+#include "fxc_fft_large.hpp"
 #include "fxc_plan_cache.hpp"
 #include "fastxc_amd.h"
 
@@ -584,6 +585,131 @@ __global__ void s2s_mf_accum_kernel(const float *__restrict__ xb, const float *_
   total[i] = total[i] + v;
 }
 
+// ---------------------------------------------------------------------------
+// Long windows (transforms beyond one workgroup's LDS, e.g. dt 0.05 s: seg
+// 72000, filter FFT 72576, output FFT 74088).  The same steps in the same
+// order as the fused kernels above, as element-wise passes over HBM rows
+// around the four-step transforms of fxc_fft_large.
+// ---------------------------------------------------------------------------
+// Window cut, non-finite -> 0, demean, detrend (FP64 sums) and edge taper of
+// one row into x[row][seg] (the front of s2s_prefilter_kernel).
+__global__ void __launch_bounds__(kS2sThreads)
+s2sL_pre_kernel(const float *__restrict__ traces, int npts, int nfiles,
+                const int *__restrict__ steps, int shift, int seg, int taper_size,
+                float *__restrict__ xout) {
+  const int row = blockIdx.x;
+  const int step = row / nfiles, file = row - step * nfiles;
+  const long long off = (long long)steps[step] * shift;
+  const float *src = traces + (size_t)file * npts;
+  float *y = xout + (size_t)row * seg;
+  double s = 0.0, unused1 = 0.0, unused2 = 0.0;
+  for (int i = threadIdx.x; i < seg; i += blockDim.x) {
+    float v = off + i < npts ? src[off + i] : 0.f;
+    if (!isfinite(v)) v = 0.f;
+    y[i] = v;
+    s += v;
+  }
+  block_sum3(s, unused1, unused2);
+  const double mean = s / seg;
+  double s1 = 0.0, si = 0.0;
+  for (int i = threadIdx.x; i < seg; i += blockDim.x) {
+    const float v = (float)((double)y[i] - mean);
+    y[i] = v;
+    s1 += v;
+    si += (double)v * i;
+  }
+  unused1 = 0.0;
+  block_sum3(s1, si, unused1);
+  const double W = seg;
+  const double a12 = 0.5 * W * (W - 1);
+  const double a11 = a12 * (2 * W - 1) / 3.;
+  const double a22 = W;
+  const double det = a11 * a22 - a12 * a12;
+  const double ca = (a22 * si - a12 * s1) / det;
+  const double cb = (a11 * s1 - a12 * si) / det;
+  for (int i = threadIdx.x; i < seg; i += blockDim.x) {
+    const float v = (float)((double)y[i] - (ca * i + cb));
+    float factor = 1.f;
+    if (i < taper_size)
+      factor = (float)(0.5 * (1.0 - cos(M_PI * i / taper_size)));
+    else if (i >= seg - taper_size)
+      factor = (float)(0.5 * (1.0 - cos(M_PI * (seg - i) / taper_size)));
+    y[i] = v * factor;
+  }
+}
+
+// X[k] = sanitise(X[k] dt) (times resp[k] when given; |X| into amp when
+// given), k < nbin, every row.
+__global__ void s2sL_spec_kernel(float2 *__restrict__ X, long long pitch, int nbin, float dt,
+                                 const float *__restrict__ resp, float *__restrict__ amp,
+                                 long long apitch) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nbin) return;
+  const long long row = blockIdx.y;
+  float2 v = cscale(X[row * pitch + k], dt);
+  if (!cfinite(v)) v = make_float2(0.f, 0.f);
+  if (resp) v = cscale(v, resp[k]);
+  X[row * pitch + k] = v;
+  if (amp) amp[row * apitch + k] = sqrtf(v.x * v.x + v.y * v.y);
+}
+
+// Pending time-domain normalisation applied in place (load_row's rules).
+__global__ void s2sL_norm_kernel(float *__restrict__ x, int seg, int nch, int mode,
+                                 const float *__restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= seg) return;
+  const long long row = blockIdx.y;
+  float v = x[row * seg + i];
+  if (mode == kLoadRunabs) {
+    const float d = w[(row / nch) * seg + i];
+    v = (!isfinite(d) || d < kMinValD) ? 0.f : v / d;
+    if (!isfinite(v)) v = 0.f;
+    v = v > kMaxVal ? kMaxVal : (v < -kMaxVal ? -kMaxVal : v);
+  } else if (mode == kLoadOnebit) {
+    v = v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f);
+  }
+  x[row * seg + i] = v;
+}
+
+// Whitening: X / ww (0 for an invalid weight), spectral taper, sanitise.
+__global__ void s2sL_whiten_kernel(float2 *__restrict__ X, long long fpitch, int nbin,
+                                   const float *__restrict__ ww, int nch, int i1, int i2, int i3,
+                                   int i4) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nbin) return;
+  const long long row = blockIdx.y;
+  float2 v = X[row * fpitch + k];
+  const float d = ww[(row / nch) * fpitch + k];
+  v = (!isfinite(d) || d < kMinValD) ? make_float2(0.f, 0.f) : make_float2(v.x / d, v.y / d);
+  v = cscale(v, spec_taper(k, i1, i2, i3, i4 < nbin ? i4 : nbin));
+  if (!cfinite(v)) v = make_float2(0.f, 0.f);
+  X[row * fpitch + k] = v;
+}
+
+// One band of RUN-ABS-MF: dst[k] = base[k] resp[k].
+__global__ void s2sL_band_kernel(const float2 *__restrict__ base, long long bpitch, int nbin,
+                                 const float *__restrict__ resp, float2 *__restrict__ dst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nbin) return;
+  const long long row = blockIdx.y;
+  dst[row * bpitch + k] = cscale(base[row * bpitch + k], resp[k]);
+}
+
+// Output spectrum: x dt, sanitise, optional phase-only, in place.
+__global__ void s2sL_out_kernel(float2 *__restrict__ out, int nspec, float dt, int phase_only) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nspec) return;
+  const long long row = blockIdx.y;
+  float2 v = cscale(out[row * nspec + k], dt);
+  if (!cfinite(v)) v = make_float2(0.f, 0.f);
+  if (phase_only) {
+    const float a = hypotf(v.x, v.y);
+    v = (!isfinite(a) || a <= kMinVal) ? make_float2(0.f, 0.f) : cscale(v, 1.f / a);
+    if (!cfinite(v)) v = make_float2(0.f, 0.f);
+  }
+  out[row * nspec + k] = v;
+}
+
 }  // namespace fxc
 
 using namespace fxc;
@@ -610,7 +736,16 @@ struct fxc_s2s_plan {
   float *d_amp = nullptr, *d_ww = nullptr;
   int *d_steps = nullptr;
   int cap_steps = 0;
+  // long-window path (transforms beyond one workgroup's LDS)
+  bool large = false;
+  long long lpitch = 0;              // complex row pitch of the long spectra
+  float2 *d_L = nullptr;             // [rows][lpitch]
+  float2 *d_Lw = nullptr;            // four-step work, kLargeChunk rows
+  long long cap_L = 0;
 };
+
+// Rows per four-step call of the long-window path (bounds its work buffer).
+constexpr long long kLargeChunk = 2048;
 
 static int make_tables(int n, RealTables *t) {
   const RealFftPlan *f = nullptr, *i = nullptr;
@@ -651,6 +786,7 @@ extern "C" int fxc_s2s_plan_create(fxc_s2s_plan **out, const fxc_s2s_config *cfg
                 "fxc_s2s_plan_create: at least one filter response is required");
   FXC_CHECK_ARG(c.normalize != 1 || c.filter_count >= 2,
                 "fxc_s2s_plan_create: RUN-ABS-MF needs sub-band filters");
+  bool large = false;
   {
     // exact LDS needs of the per-row kernels (data slots + staged tables)
     const int Mf = c.nfft_filter / 2, Ms = c.seg / 2, Mo = c.nfft_out / 2;
@@ -659,12 +795,21 @@ extern "C" int fxc_s2s_plan_create(fxc_s2s_plan **out, const fxc_s2s_config *cfg
     const int need_o = Mo + 1 + tables_count(Mo);
     const int need_so = std::max(Ms, Mo) + 1 + tables_count(Ms) + tables_count(Mo);
     const int need = std::max(std::max(need_f, need_s), std::max(need_o, need_so));
-    FXC_CHECK_ARG(need <= kLdsComplexCapacity,
-                  "fxc_s2s_plan_create: transforms (seg %d, filter %d, out %d) exceed the "
-                  "single-workgroup LDS FFT (%d > %d complex)", c.seg, c.nfft_filter, c.nfft_out,
-                  need, kLdsComplexCapacity);
+    large = need > kLdsComplexCapacity;
+    if (large) {
+      // every transform of the plan runs as a four-step transform over HBM
+      int a = 0, b = 0;
+      for (int M : {Mf, Ms, Mo})
+        FXC_CHECK_ARG(large_factor(M, &a, &b),
+                      "fxc_s2s_plan_create: transforms (seg %d, filter %d, out %d) exceed one "
+                      "workgroup and %d has no two-level factorisation", c.seg, c.nfft_filter,
+                      c.nfft_out, 2 * M);
+    }
   }
   fxc_s2s_plan *p = new fxc_s2s_plan();
+  p->large = large;
+  p->lpitch = ((long long)std::max(std::max(c.nfft_filter, c.seg), c.nfft_out) / 2 + 1 + 15) /
+              16 * 16;
   p->cfg = c;
   p->cfg.responses = nullptr;
   p->cfg.freq_lows = nullptr;
@@ -694,9 +839,9 @@ extern "C" int fxc_s2s_plan_create(fxc_s2s_plan **out, const fxc_s2s_config *cfg
     delete p;
     return FXC_ERR_HIP;
   }
-  if ((rc = make_tables(c.nfft_filter, &p->tf)) != FXC_OK ||
-      (rc = make_tables(c.seg, &p->ts)) != FXC_OK ||
-      (rc = make_tables(c.nfft_out, &p->to)) != FXC_OK) {
+  if (!large && ((rc = make_tables(c.nfft_filter, &p->tf)) != FXC_OK ||
+                 (rc = make_tables(c.seg, &p->ts)) != FXC_OK ||
+                 (rc = make_tables(c.nfft_out, &p->to)) != FXC_OK)) {
     dfree(p->d_resp);
     delete p;
     return rc;
@@ -709,7 +854,7 @@ extern "C" int fxc_s2s_plan_destroy(fxc_s2s_plan *p) {
   if (!p) return FXC_OK;
   for (void *q : {(void *)p->d_resp, (void *)p->d_x, (void *)p->d_w, (void *)p->d_xb,
                   (void *)p->d_total, (void *)p->d_X, (void *)p->d_base, (void *)p->d_amp,
-                  (void *)p->d_ww, (void *)p->d_steps})
+                  (void *)p->d_ww, (void *)p->d_steps, (void *)p->d_L, (void *)p->d_Lw})
     dfree(q);
   delete p;
   return FXC_OK;
@@ -825,6 +970,125 @@ static int s2s_whiten(fxc_s2s_plan *p, long long rows, long long frames, int mod
   return FXC_OK;
 }
 
+// Long-window path: four-step real transforms in chunks of kLargeChunk rows.
+static int s2sL_r2c(fxc_s2s_plan *p, int n, const float *x, long long xd, int in_len, float2 *X,
+                    long long Xd, long long rows, hipStream_t st) {
+  for (long long r = 0; r < rows; r += kLargeChunk)
+    FXC_TRY(large_r2c(n, x + r * xd, xd, in_len, X + r * Xd, Xd, std::min(kLargeChunk, rows - r),
+                      p->d_Lw, 1.f, st));
+  return FXC_OK;
+}
+
+static int s2sL_c2r(fxc_s2s_plan *p, int n, const float2 *X, long long Xd, float *x, long long xd,
+                    int out_len, double scale, long long rows, hipStream_t st) {
+  for (long long r = 0; r < rows; r += kLargeChunk)
+    FXC_TRY(large_c2r(n, X + r * Xd, Xd, x + r * xd, xd, out_len, std::min(kLargeChunk, rows - r),
+                      p->d_Lw, scale, st));
+  return FXC_OK;
+}
+
+static dim3 s2sL_grid(long long n, long long rows) {
+  return dim3((unsigned)ceil_div((size_t)n, (size_t)256), (unsigned)rows);
+}
+
+// Whitening of d_x rows (pending normalisation `mode`, weights w first).
+static int s2sL_whiten(fxc_s2s_plan *p, long long rows, long long frames, int mode,
+                       const float *w, hipStream_t st) {
+  const fxc_s2s_config &c = p->cfg;
+  if (mode != kLoadPlain) {
+    hipLaunchKernelGGL(s2sL_norm_kernel, s2sL_grid(c.seg, rows), dim3(256), 0, st, p->d_x, c.seg,
+                       c.num_ch, mode, w);
+    FXC_HIP(hipGetLastError());
+  }
+  FXC_TRY(s2sL_r2c(p, c.seg, p->d_x, c.seg, c.seg, p->d_X, p->fpitch, rows, st));
+  hipLaunchKernelGGL(s2sL_spec_kernel, s2sL_grid(p->fwidth, rows), dim3(256), 0, st, p->d_X,
+                     p->fpitch, p->fwidth, c.delta, (const float *)nullptr, p->d_amp, p->fpitch);
+  FXC_HIP(hipGetLastError());
+  FXC_TRY(launch_weight(true, p->d_X, p->fpitch, p->fwidth, c.num_ch, p->whiten_win, frames,
+                        p->d_ww, p->fpitch, st));
+  hipLaunchKernelGGL(s2sL_whiten_kernel, s2sL_grid(p->fwidth, rows), dim3(256), 0, st, p->d_X,
+                     p->fpitch, p->fwidth, p->d_ww, c.num_ch, c.idx[0], c.idx[1], c.idx[2],
+                     c.idx[3]);
+  FXC_HIP(hipGetLastError());
+  const double inv_seg = 1.0 / (double)((float)c.seg * c.delta);
+  return s2sL_c2r(p, c.seg, p->d_X, p->fpitch, p->d_x, c.seg, c.seg, inv_seg, rows, st);
+}
+
+static int s2s_process_large(fxc_s2s_plan *p, const float *d_traces, int nfiles, long long rows,
+                             long long frames, float2 *d_out, hipStream_t st) {
+  const fxc_s2s_config &c = p->cfg;
+  FXC_CHECK_ARG(rows <= 65535, "sac2spec: %lld station-windows exceed one long-window call "
+                "(65535)", rows);
+  if (rows > p->cap_L) {
+    dfree(p->d_L);
+    dfree(p->d_Lw);
+    p->d_L = p->d_Lw = nullptr;
+    p->cap_L = 0;
+    const int Mmax = std::max(std::max(c.nfft_filter, c.seg), c.nfft_out) / 2;
+    FXC_TRY(dmalloc((void **)&p->d_L, (size_t)rows * p->lpitch * sizeof(float2)));
+    FXC_TRY(dmalloc((void **)&p->d_Lw, (size_t)std::min(kLargeChunk, rows) * 2 * Mmax *
+                                           sizeof(float2)));
+    p->cap_L = rows;
+  }
+  // window cut + preprocess, then the broad-band prefilter
+  hipLaunchKernelGGL(s2sL_pre_kernel, dim3((unsigned)rows), dim3(kS2sThreads), 0, st, d_traces,
+                     c.npts, nfiles, p->d_steps, c.shift, c.seg, p->taper_size, p->d_x);
+  FXC_HIP(hipGetLastError());
+  const int nbf = c.nfft_filter / 2 + 1;
+  const double inv_f = 1.0 / (double)((float)c.nfft_filter * c.delta);
+  FXC_TRY(s2sL_r2c(p, c.nfft_filter, p->d_x, c.seg, c.seg, p->d_L, p->lpitch, rows, st));
+  hipLaunchKernelGGL(s2sL_spec_kernel, s2sL_grid(nbf, rows), dim3(256), 0, st, p->d_L, p->lpitch,
+                     nbf, c.delta, p->d_resp, (float *)nullptr, 0LL);
+  FXC_HIP(hipGetLastError());
+  FXC_TRY(s2sL_c2r(p, c.nfft_filter, p->d_L, p->lpitch, p->d_x, c.seg, c.seg, inv_f, rows, st));
+  int mode = kLoadPlain;
+  const float *w = nullptr;
+  if (c.whiten == 1 || c.whiten == 3) FXC_TRY(s2sL_whiten(p, rows, frames, kLoadPlain, nullptr, st));
+  if (c.normalize == 3) {
+    FXC_TRY(launch_weight(false, p->d_x, c.seg, c.seg, c.num_ch, p->runabs_win, frames, p->d_w,
+                          c.seg, st));
+    mode = kLoadRunabs;
+    w = p->d_w;
+  } else if (c.normalize == 2) {
+    mode = kLoadOnebit;
+  } else if (c.normalize == 1) {
+    // base spectrum R2C(nfft_filter) x dt, then per band C2R(base |H_i|^2)
+    FXC_TRY(s2sL_r2c(p, c.nfft_filter, p->d_x, c.seg, c.seg, p->d_base, p->bpitch, rows, st));
+    hipLaunchKernelGGL(s2sL_spec_kernel, s2sL_grid(nbf, rows), dim3(256), 0, st, p->d_base,
+                       p->bpitch, nbf, c.delta, (const float *)nullptr, (float *)nullptr, 0LL);
+    FXC_HIP(hipGetLastError());
+    FXC_HIP(hipMemsetAsync(p->d_total, 0, (size_t)rows * c.seg * sizeof(float), st));
+    for (int i = 1; i < c.filter_count; ++i) {
+      hipLaunchKernelGGL(s2sL_band_kernel, s2sL_grid(nbf, rows), dim3(256), 0, st, p->d_base,
+                         p->bpitch, nbf, p->d_resp + (size_t)i * nbf, p->d_L);
+      FXC_HIP(hipGetLastError());
+      FXC_TRY(s2sL_c2r(p, c.nfft_filter, p->d_L, p->bpitch, p->d_xb, c.seg, c.seg, inv_f, rows,
+                       st));
+      FXC_TRY(launch_weight(false, p->d_xb, c.seg, c.seg, c.num_ch, p->mf_win[i], frames, p->d_w,
+                            c.seg, st));
+      const size_t n = (size_t)rows * c.seg;
+      hipLaunchKernelGGL(s2s_mf_accum_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st,
+                         p->d_xb, p->d_w, c.seg, c.num_ch, rows, p->d_total);
+      FXC_HIP(hipGetLastError());
+    }
+    FXC_HIP(hipMemcpyAsync(p->d_x, p->d_total, (size_t)rows * c.seg * sizeof(float),
+                           hipMemcpyDeviceToDevice, st));
+  }
+  if (c.whiten == 2 || c.whiten == 3) {
+    FXC_TRY(s2sL_whiten(p, rows, frames, mode, w, st));
+  } else if (mode != kLoadPlain) {
+    hipLaunchKernelGGL(s2sL_norm_kernel, s2sL_grid(c.seg, rows), dim3(256), 0, st, p->d_x, c.seg,
+                       c.num_ch, mode, w);
+    FXC_HIP(hipGetLastError());
+  }
+  // output spectrum: zero pad to nfft_out, R2C, x dt, sanitise, phase-only
+  FXC_TRY(s2sL_r2c(p, c.nfft_out, p->d_x, c.seg, c.seg, d_out, p->nspec, rows, st));
+  hipLaunchKernelGGL(s2sL_out_kernel, s2sL_grid(p->nspec, rows), dim3(256), 0, st, d_out,
+                     p->nspec, c.delta, c.phase_only);
+  FXC_HIP(hipGetLastError());
+  return FXC_OK;
+}
+
 // Processes nfiles traces [nfiles][npts] (nfiles = groups * num_ch) for the
 // windows listed in h_steps; writes [nsteps][nfiles][nspec] complex64.
 extern "C" int fxc_s2s_process(fxc_s2s_plan *p, const float *d_traces, int nfiles,
@@ -840,6 +1104,7 @@ extern "C" int fxc_s2s_process(fxc_s2s_plan *p, const float *d_traces, int nfile
   const long long frames = rows / c.num_ch;
   FXC_TRY(s2s_ensure(p, rows, frames, nsteps));
   FXC_HIP(hipMemcpyAsync(p->d_steps, h_steps, nsteps * sizeof(int), hipMemcpyHostToDevice, st));
+  if (p->large) return s2s_process_large(p, d_traces, nfiles, rows, frames, (float2 *)d_out, st);
   // K1 prefilter
   const int Mf = c.nfft_filter / 2;
   size_t lds = lds_bytes(Mf + 1, Mf);

@@ -416,7 +416,8 @@ extern "C" int fxc_stream_create(fxc_stream **out, const fxc_s2s_config *cfg, in
   s->nchan = nchan;
   s->nstep = (cfg->npts - cfg->seg) / cfg->shift + 1;
   s->nspec = cfg->nfft_out / 2 + 1;
-  s->half_cc = std::min(half_cc, s->nspec - 1);
+  // 2H+1 must fit nfft = 2(nspec-1): H <= nspec - 2
+  s->half_cc = std::min(half_cc, s->nspec - 2);
   s->ncc = 2 * s->half_cc + 1;
   s->precision = precision;
   s->history = history;

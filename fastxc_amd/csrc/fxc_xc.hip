@@ -23,6 +23,7 @@
 // kernel folds each row into an 18522-point complex inverse FFT held wholly
 // in LDS and writes only the 2H+1 lags.
 #include "fxc_fft_fixed.hpp"
+#include "fxc_fft_large.hpp"
 #include "fxc_plan_cache.hpp"
 #include "fastxc_amd.h"
 
@@ -667,6 +668,7 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
 // additionally the register-resident composite 9, which takes one LDS pass
 // fewer on 2*3^3*7^3 (radix 14 no longer fits the register budget).
 constexpr int kLagThreads = 1024;
+constexpr long long kLagLargeChunk = 2048;  // rows per four-step lag call
 constexpr int kLagMaxRadixDefault = 9;
 constexpr unsigned kRadixLag8 = (1u << 2) | (1u << 3) | (1u << 4) | (1u << 5) | (1u << 6) | (1u << 7) | (1u << 8);
 constexpr unsigned kRadixLag9 = kRadixLag8 | (1u << 9);
@@ -999,6 +1001,26 @@ plain_accum_kernel(const float2 *__restrict__ spec_w, int nspec, const int *__re
   }
 }
 
+// Lags of rows whose folded inverse transform z (M = nfft/2 complex per row,
+// z[j] = y[2j] + i y[2j+1]) is done: out[m] = y[(m - H) mod nfft] * weight,
+// the product in double (the reference's (-1)^k shift + C2R + extraction).
+__global__ void lag_extract_kernel(const float2 *__restrict__ z, int M, long long nrows,
+                                   const int *__restrict__ row_out, float *__restrict__ cc,
+                                   int half_cc, double weight,
+                                   const float *__restrict__ task_scale) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncc = 2 * half_cc + 1;
+  if (m >= ncc) return;
+  for (long long r = blockIdx.y; r < nrows; r += gridDim.y) {
+    const int out_row = row_out ? row_out[r] : (int)r;
+    const double wr = task_scale ? weight * (double)task_scale[out_row] : weight;
+    int i = m - half_cc;
+    if (i < 0) i += 2 * M;
+    const float2 v = z[r * M + (i >> 1)];
+    cc[(size_t)out_row * ncc + m] = (float)((double)((i & 1) ? v.y : v.x) * wr);
+  }
+}
+
 // Copies workspace rows to their task rows (cross-spectra output order).
 __global__ void scatter_rows_kernel(const float2 *__restrict__ src, long long src_pitch,
                                     const int *__restrict__ row_task, int nspec,
@@ -1038,6 +1060,10 @@ struct fxc_xc_plan {
   // tables of the compile-time lag plan (nfft/2 == LagPlan18522::kM only)
   float2 *d_fix_tw = nullptr, *d_fix_twl = nullptr, *d_fix_fold = nullptr, *d_fix_roots = nullptr;
   long long chunk_rows_max = 0;  // cross-spectrum rows per workspace fill
+  // nfft/2 beyond one workgroup's LDS: lag synthesis by the four-step
+  // inverse transform (work buffer for kLagLargeChunk rows)
+  bool large_lag = false;
+  float2 *d_lwork = nullptr;
   // pairs
   int nsta = 0;
   long long npairs = 0;
@@ -1127,18 +1153,25 @@ extern "C" int fxc_xc_plan_create(fxc_xc_plan **out, int nspec, int nstep, float
                 "fxc_xc_plan_create: unknown precision %d", precision);
   const int nfft = 2 * (nspec - 1);
   FXC_CHECK_ARG(2 * half_cc + 1 <= nfft, "fxc_xc_plan_create: 2*half_cc+1 exceeds nfft");
-  FXC_CHECK_ARG(nfft / 2 <= kLdsComplexCapacity,
-                "fxc_xc_plan_create: nfft %d exceeds the single-workgroup FFT size", nfft);
   if (precision == FXC_PREC_BF16X3 || precision == FXC_PREC_FP16)
     FXC_CHECK_ARG(nstep <= 128, "fxc_xc_plan_create: MFMA forms support up to 128 windows");
   // lag butterfly set: radices up to 9 (default) or 8; FXC_XC_LAG_MAX_RADIX overrides
   int lag_max_radix = kLagMaxRadixDefault;
   if (const char *env = getenv("FXC_XC_LAG_MAX_RADIX")) lag_max_radix = atoi(env) >= 9 ? 9 : 8;
-  const RealFftPlan *rp = nullptr;
-  FXC_TRY(plan_cache_real(nfft, 1, &rp, lag_max_radix, true));
-  FXC_CHECK_ARG(real_lds_count(nfft / 2) <= kLdsComplexCapacity,
-                "fxc_xc_plan_create: nfft %d does not fit one workgroup", nfft);
+  const bool large_lag = real_lds_count(nfft / 2) > kLdsComplexCapacity;
+  if (large_lag) {
+    int a = 0, b = 0;
+    if (!large_factor(nfft / 2, &a, &b)) {
+      set_error("fxc_xc_plan_create: nfft %d exceeds one workgroup and has no two-level "
+                "factorisation", nfft);
+      return FXC_ERR_UNSUPPORTED;
+    }
+  } else {
+    const RealFftPlan *rp = nullptr;
+    FXC_TRY(plan_cache_real(nfft, 1, &rp, lag_max_radix, true));
+  }
   fxc_xc_plan *p = new fxc_xc_plan();
+  p->large_lag = large_lag;
   p->nspec = nspec;
   p->nstep = nstep;
   p->nfft = nfft;
@@ -1235,6 +1268,7 @@ extern "C" int fxc_xc_plan_destroy(fxc_xc_plan *p) {
   dfree(p->d_fix_twl);
   dfree(p->d_fix_fold);
   dfree(p->d_fix_roots);
+  dfree(p->d_lwork);
   delete p;
   return FXC_OK;
 }
@@ -1504,6 +1538,31 @@ static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long
                         const int *row_out, float *d_cc, hipStream_t st,
                         const float *task_scale = nullptr) {
   if (nrows == 0) return FXC_OK;
+  const double weight0 = 1.0 / (double)((float)p->nfft * p->dt);
+  if (p->large_lag) {
+    // four-step inverse of the folded spectrum, kLagLargeChunk rows at a time
+    const int M = p->nfft / 2;
+    const LargeFft *lp = nullptr;
+    FXC_TRY(large_plan(M, 1, &lp));
+    if (!p->d_lwork) FXC_TRY(dmalloc((void **)&p->d_lwork, (size_t)kLagLargeChunk * 2 * M * sizeof(float2)));
+    float2 *w0 = p->d_lwork, *w1 = p->d_lwork + (size_t)kLagLargeChunk * M;
+    FXC_TRY(xc_mark(p, kTimeLag, true, st));
+    for (long long r0 = 0; r0 < nrows; r0 += kLagLargeChunk) {
+      const long long nr = std::min<long long>(kLagLargeChunk, nrows - r0);
+      hipLaunchKernelGGL(large_fold_kernel, dim3((unsigned)ceil_div(M, 256), (unsigned)nr),
+                         dim3(256), 0, st, xs + r0 * xpitch, xpitch, M, lp->d_half, w0);
+      FXC_HIP(hipGetLastError());
+      FXC_TRY(large_c2c(lp, w0, M, w0, M, nr, w1, 1.f, st));
+      hipLaunchKernelGGL(lag_extract_kernel,
+                         dim3((unsigned)ceil_div(p->cc_size, 256), (unsigned)std::min(nr, 65535LL)),
+                         dim3(256), 0, st, w0, M, nr, row_out ? row_out + r0 : nullptr,
+                         row_out ? d_cc : d_cc + (size_t)r0 * p->cc_size, p->half_cc, weight0,
+                         task_scale);
+      FXC_HIP(hipGetLastError());
+    }
+    FXC_TRY(xc_mark(p, kTimeLag, false, st));
+    return FXC_OK;
+  }
   const RealFftPlan *rp = nullptr;
   FXC_TRY(plan_cache_real(p->nfft, 1, &rp, p->lag_max_radix, true));
   const size_t lds = (size_t)real_lds_count(p->nfft / 2) * sizeof(float2);

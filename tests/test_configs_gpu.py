@@ -189,7 +189,7 @@ def test_config1_production_shape(gpu, tmp_path):
     idx = _sourcepack_indexes(spd)
     assert len(idx) == days
     res = linear_stack_sourcepack_indexes(idx, tmp_path / "stack", device=gpu)
-    assert len(res) == npair and all(x.count == days for x in res)
+    assert len(res) == npair and all(x.segment_count == days for x in res)
     lin = ff.read_sourcepack_index(
         str(tmp_path / "stack" / "linearstack_sourcepack" / "STACK" / "sourcepack_index.tsv"))
     _compare("config1 linear", {_key(row): data for row, _, data in lin},

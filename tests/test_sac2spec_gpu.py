@@ -98,3 +98,29 @@ def test_production_lengths_one_station(gpu, tmp_path):
     win = s2s.cut_windows(tr.astype(np.float64), plan, [0, 1, 2])
     want = s2s.process_batch(win.reshape(3, 1, plan.seg), plan, 2, 3, False).reshape(3, 1, -1)
     assert rel_l2(out, want) < 1e-5
+
+
+@pytest.mark.parametrize("whiten,normalize,bands,nch", [(2, 3, "0.1/2", 1), (3, 1, "0.1/0.5 0.5/2", 3),
+                                                        (1, 2, "0.1/2", 1), (0, 0, "0.1/2", 1)])
+def test_long_windows_dt005(gpu, tmp_path, whiten, normalize, bands, nch):
+    """dt 0.05 s, one-hour windows: seg 72000, nfft_filter 72576, nfft_out
+    74088, whose transforms exceed one workgroup's LDS and run as four-step
+    transforms over HBM, against the FP64 oracle."""
+    import torch
+
+    path = str(tmp_path / "filter.txt")
+    s2s.write_filter_file(path, 0.05, bands)
+    plan = s2s.plan_lengths(2 * 3600, 0.05, 3600, 3600, 100, s2s.read_filters(path))
+    # 72576 with the 0.1-2 Hz band; the 0.1-0.5 Hz band pads the filter FFT further
+    assert (plan.seg, plan.nfft_out) == (72000, 74088) and plan.nfft_filter >= 72576
+    rng = np.random.default_rng(10 + whiten + normalize)
+    nsta = 2
+    tr = _traces(rng, nsta * nch, plan.npts)
+    gp = _gpu_plan(plan, nch, whiten, normalize, 0)
+    out = gp.process(torch.from_numpy(tr).to(gpu), [0, 1]).cpu().numpy()
+    win = s2s.cut_windows(tr.astype(np.float64), plan, [0, 1])
+    want = s2s.process_batch(win.reshape(2 * nsta, nch, plan.seg), plan, whiten, normalize,
+                             False).reshape(2, nsta * nch, -1)
+    err = rel_l2(out, want)
+    print(f"dt 0.05 whiten={whiten} normalize={normalize} rel_l2={err:.3e}")
+    assert err < 1e-5

@@ -178,3 +178,25 @@ def test_uneven_segments_with_gaps(gpu, precision):
         want = xc.correlate(spec, src, rec, nfft, H, dt)
         err = xc.rel_l2(cc, want)
         assert err < TOL[precision], (s, nsta, nstep, err)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_long_nfft_dt005(gpu, precision):
+    """nfft 74088 (dt 0.05 s, one-hour windows), 4001 lags: the lag synthesis
+    runs the four-step inverse transform; against FP64, lag order exact."""
+    import torch
+    from fastxc_amd.ops import XcPlan
+
+    rng = np.random.default_rng(74)
+    nsta, nstep, seg, nfft, H, dt = 6, 3, 72000, 74088, 2000, 0.05
+    spec = _spectra(rng, nstep, nsta, seg, nfft, dt)
+    src, rec = _all_pairs(nsta)
+    plan = XcPlan(nfft // 2 + 1, nstep, dt, H, precision)
+    plan.set_pairs(nsta, src, rec)
+    cc = plan.correlate(torch.from_numpy(spec).to(gpu)).cpu().numpy()
+    want = xc.correlate(spec, src, rec, nfft, H, dt)
+    err = xc.rel_l2(cc, want)
+    print(f"nfft 74088 {precision} rel_l2={err:.3e}")
+    assert cc.shape == (len(src), 4001) and err < TOL[precision]
+    row_err = np.linalg.norm(cc - want, axis=1) / np.linalg.norm(want, axis=1)
+    assert np.all(row_err < 10 * TOL[precision])
