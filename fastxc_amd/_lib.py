@@ -106,6 +106,10 @@ _SIGNATURES = {
                                 POINTER(c_longlong), POINTER(c_longlong)]),
     "fxc_stream_read": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_longlong, c_int, c_int,
                                 c_float, c_float, c_float, c_void_p, c_void_p, c_void_p]),
+    "fxc_stream_set_pws": (c_int, [c_void_p, c_int]),
+    "fxc_stream_read_spec": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_longlong, c_int, c_int,
+                                     c_float, c_float, c_float, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]),
     "fxc_stream_destroy": (c_int, [c_void_p]),
 }
 

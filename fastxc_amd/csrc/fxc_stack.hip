@@ -506,6 +506,129 @@ tfpws_interp_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Al
   }
 }
 
+// ---------------------------------------------------------------------------
+// Running PWS state of the streaming engine (no reference counterpart: the
+// reference stacks a pair's stored records offline in sub-stacks of B,
+// native/pws/io/sourcepack_io.cu:385-478, kernels/cuda.pws.cu:28-95).  Per
+// pair the engine keeps S[t] = sum over closed sub-stacks g of n_g phi_g[t]
+// and the float sum of the open sub-stack's records; then
+//   W = (S + n_open phi_open) / T,   pws = lin |W|^power
+// equals the reference's sum_g (n_g / T) phi_g over all T records.
+// phi of a sub-stack: its mean (sum * (1/n), float), analytic signal (bins
+// 1..ceil(N/2)-1 doubled, the rest 0), unnormalised inverse / N, divided by
+// max(|a|, 1e-7) -- the steps of analytic_kernel and pws_kernel.
+// ---------------------------------------------------------------------------
+template <int kPer>
+__device__ __forceinline__ void stream_phi(float2 *lds, const float *__restrict__ sum, float scale,
+                                           const StackFftDev &F, const float2 *twl,
+                                           float2 (&phi)[kPer]) {
+  const int n = F.n, tid = threadIdx.x;
+  for (int j = tid; j < F.ld; j += blockDim.x)
+    lds[j] = j < n ? make_float2(sum[j] * scale, 0.f) : make_float2(0.f, 0.f);
+  __syncthreads();
+  stack_fft(lds, 1, F, twl, -1.f);
+  // analytic mask into natural order (through registers)
+  const int half = (n + 1) / 2;
+  float2 a[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int f = tid + k * kStkThreads;
+    a[k] = make_float2(0.f, 0.f);
+    if (f >= 1 && f < half) {
+      const float2 z = lds[stack_pos(F, f)];
+      a[k] = make_float2(2.f * z.x, 2.f * z.y);
+    }
+  }
+  __syncthreads();
+  for (int j = tid; j < F.ld; j += blockDim.x) lds[j] = make_float2(0.f, 0.f);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int f = tid + k * kStkThreads;
+    if (f < n) lds[f] = a[k];
+  }
+  __syncthreads();
+  stack_fft(lds, 1, F, twl, 1.f);
+  const float invn = 1.f / (float)n;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int t = tid + k * kStkThreads;
+    phi[k] = make_float2(0.f, 0.f);
+    if (t < n) {
+      const float2 z = lds[stack_pos(F, t)];
+      const float re = z.x * invn, im = z.y * invn;
+      float mod = sqrtf(re * re + im * im);
+      mod = mod > 1e-7f ? mod : 1e-7f;
+      phi[k] = make_float2(re / mod, im / mod);
+    }
+  }
+  __syncthreads();
+}
+
+// Closes one sub-stack per workgroup: S[slot] += cnt phi(open[slot] / cnt),
+// open[slot] = 0.
+template <int kPer>
+__global__ void __launch_bounds__(kStkThreads)
+stream_pws_close_kernel(float2 *__restrict__ S, float *__restrict__ open,
+                        const int *__restrict__ slot, const int *__restrict__ cnt, StackFftDev F,
+                        const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i) {
+  extern __shared__ float2 lds[];
+  const int n = F.n;
+  float2 *twl = lds + F.ld;
+  stage_stack_tw(twl, F, tw_f, tw_i);
+  const long long sl = slot[blockIdx.x];
+  const int c = cnt[blockIdx.x];
+  float *o = open + sl * n;
+  float2 phi[kPer];
+  stream_phi<kPer>(lds, o, 1.0f / (float)c, F, twl, phi);
+  float2 *Sr = S + sl * n;
+  const float fc = (float)c;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int t = threadIdx.x + k * kStkThreads;
+    if (t < n) {
+      const float2 v = Sr[t];
+      Sr[t] = make_float2(v.x + fc * phi[k].x, v.y + fc * phi[k].y);
+      o[t] = 0.f;
+    }
+  }
+}
+
+// Read-out per pair (one workgroup each): out = (float)(acc / T) |W|^power,
+// W = (S + n_open phi_open) / T.
+template <int kPer>
+__global__ void __launch_bounds__(kStkThreads)
+stream_pws_read_kernel(const float2 *__restrict__ S, const float *__restrict__ open,
+                       const double *__restrict__ acc, const int *__restrict__ slot,
+                       const long long *__restrict__ total, const int *__restrict__ nopen,
+                       float power, StackFftDev F, const float2 *__restrict__ tw_f,
+                       const float2 *__restrict__ tw_i, float *__restrict__ out) {
+  extern __shared__ float2 lds[];
+  const int n = F.n;
+  float2 *twl = lds + F.ld;
+  stage_stack_tw(twl, F, tw_f, tw_i);
+  const int pair = blockIdx.x;
+  const long long sl = slot[pair];
+  const long long T = total[pair];
+  const int no = nopen[pair];
+  float2 phi[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) phi[k] = make_float2(0.f, 0.f);
+  if (no > 0) stream_phi<kPer>(lds, open + sl * n, 1.0f / (float)no, F, twl, phi);
+  const float invT = 1.0f / (float)T;
+  const float fo = (float)no;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int t = threadIdx.x + k * kStkThreads;
+    if (t < n) {
+      const float2 sv = S[sl * n + t];
+      const float wx = (sv.x + fo * phi[k].x) * invT, wy = (sv.y + fo * phi[k].y) * invT;
+      const float lin = (float)(acc[sl * n + t] / (double)T);
+      out[(size_t)pair * n + t] = lin * mag_pow(wx * wx + wy * wy, power);
+    }
+  }
+}
+
 // Final TF-PWS step per pair: voices (blended with the linear analytic
 // spectrum in band mode), zero above nfreq, inverse transform, Re / n.
 __global__ void __launch_bounds__(kStkThreads)
@@ -702,6 +825,65 @@ static int check_stack_args(const float *pre, const float *wts, const long long 
   for (int i = 0; i < npairs; ++i)
     FXC_CHECK_ARG(off[i + 1] > off[i], "stack: pair %d has no groups", i);
   return FXC_OK;
+}
+
+// Host side of the streaming engine's running PWS (lengths n up to 4096
+// positions per 256-thread workgroup, the direct or Bluestein transform).
+static int stream_pws_launch_check(int n, StackFft *S, size_t *lds) {
+  FXC_TRY(stack_fft_plan(n, S));
+  FXC_CHECK_ARG((n + kStkThreads - 1) / kStkThreads <= 16, "stream PWS: length %d too long", n);
+  *lds = ((size_t)S->dev.ld + S->twc) * sizeof(float2);
+  FXC_CHECK_ARG((size_t)S->dev.ld + S->twc + kLdsReserve <= (size_t)kLdsComplexCapacity,
+                "stream PWS: length %d too long for the LDS transform", n);
+  return FXC_OK;
+}
+
+int stream_pws_close(float2 *S, float *open, const int *d_slot, const int *d_cnt, long long nclose,
+                     int n, hipStream_t st) {
+  if (nclose == 0) return FXC_OK;
+  StackFft F;
+  size_t lds = 0;
+  FXC_TRY(stream_pws_launch_check(n, &F, &lds));
+  const int per = (n + kStkThreads - 1) / kStkThreads;
+#define FXC_SPC(P)                                                                            \
+  if (per <= P) {                                                                            \
+    auto *k = stream_pws_close_kernel<P>;                                                    \
+    FXC_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                (int)lds));                                                  \
+    hipLaunchKernelGGL(k, dim3((unsigned)nclose), dim3(kStkThreads), lds, st, S, open, d_slot, \
+                       d_cnt, F.dev, F.tw_f, F.tw_i);                                        \
+    FXC_HIP(hipGetLastError());                                                              \
+    return FXC_OK;                                                                           \
+  }
+  FXC_SPC(4)
+  FXC_SPC(8)
+  FXC_SPC(16)
+#undef FXC_SPC
+  return FXC_ERR_ARG;
+}
+
+int stream_pws_read(const float2 *S, const float *open, const double *acc, const int *d_slot,
+                    const long long *d_total, const int *d_nopen, long long npairs, int n,
+                    float power, float *out, hipStream_t st) {
+  StackFft F;
+  size_t lds = 0;
+  FXC_TRY(stream_pws_launch_check(n, &F, &lds));
+  const int per = (n + kStkThreads - 1) / kStkThreads;
+#define FXC_SPR(P)                                                                            \
+  if (per <= P) {                                                                            \
+    auto *k = stream_pws_read_kernel<P>;                                                     \
+    FXC_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                (int)lds));                                                  \
+    hipLaunchKernelGGL(k, dim3((unsigned)npairs), dim3(kStkThreads), lds, st, S, open, acc,  \
+                       d_slot, d_total, d_nopen, power, F.dev, F.tw_f, F.tw_i, out);         \
+    FXC_HIP(hipGetLastError());                                                              \
+    return FXC_OK;                                                                           \
+  }
+  FXC_SPR(4)
+  FXC_SPR(8)
+  FXC_SPR(16)
+#undef FXC_SPR
+  return FXC_ERR_ARG;
 }
 
 }  // namespace fxc

@@ -33,6 +33,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -45,6 +46,13 @@
 
 int xc_set_pairs_on(fxc_xc_plan *p, int nsta, const int *h_src, const int *h_rec, long long npairs,
                     hipStream_t stream);
+namespace fxc {
+int stream_pws_close(float2 *S, float *open, const int *d_slot, const int *d_cnt, long long nclose,
+                     int n, hipStream_t st);
+int stream_pws_read(const float2 *S, const float *open, const double *acc, const int *d_slot,
+                    const long long *d_total, const int *d_nopen, long long npairs, int n,
+                    float power, float *out, hipStream_t st);
+}  // namespace fxc
 
 namespace fxc {
 
@@ -56,7 +64,8 @@ constexpr int kStrThreads = 256;
 __global__ void __launch_bounds__(kStrThreads)
 stream_update_kernel(const float *__restrict__ cc, int ncc, long long npairs,
                      const int *__restrict__ slot, const int *__restrict__ hpos,
-                     double *__restrict__ acc, float *__restrict__ hist, int history) {
+                     double *__restrict__ acc, float *__restrict__ hist, int history,
+                     float *__restrict__ open) {
   const int l = blockIdx.x * kStrThreads + threadIdx.x;
   if (l >= ncc) return;
   for (long long p = blockIdx.y; p < npairs; p += gridDim.y) {
@@ -64,7 +73,19 @@ stream_update_kernel(const float *__restrict__ cc, int ncc, long long npairs,
     const long long s = slot[p];
     acc[s * ncc + l] += (double)v;
     if (history > 0) hist[(s * history + hpos[p]) * ncc + l] = v;
+    if (open) open[s * ncc + l] += v;  // the open PWS sub-stack, float in record order
   }
+}
+
+// Segment traces into channel-id order: dst[i] = src[perm[i]] (rows of npts).
+__global__ void __launch_bounds__(kStrThreads)
+stream_gather_kernel(const float *__restrict__ src, const int *__restrict__ perm, int npts,
+                     float *__restrict__ dst) {
+  const int i = blockIdx.y;
+  const float *a = src + (size_t)perm[i] * npts;
+  float *b = dst + (size_t)i * npts;
+  for (int t = blockIdx.x * kStrThreads + threadIdx.x; t < npts; t += gridDim.x * kStrThreads)
+    b[t] = a[t];
 }
 
 // out[p][l] = (float)(acc[slot][l] / count); 0 for a pair never updated.
@@ -145,9 +166,10 @@ struct StreamLane {
   hipStream_t st = nullptr;
   fxc_s2s_plan *s2s = nullptr;
   fxc_xc_plan *xc = nullptr;
+  float *d_raw = nullptr;  // traces as submitted
   float *d_traces = nullptr, *d_spec = nullptr, *d_cc = nullptr;
-  int *d_slot = nullptr, *d_hpos = nullptr;
-  int *h_slot = nullptr, *h_hpos = nullptr;  // pinned
+  int *d_slot = nullptr, *d_hpos = nullptr, *d_perm = nullptr, *d_close = nullptr;
+  int *h_slot = nullptr, *h_hpos = nullptr, *h_perm = nullptr, *h_close = nullptr;  // pinned
   hipEvent_t ev_cc = nullptr, ev_done = nullptr;
   std::thread th;
 };
@@ -165,6 +187,11 @@ struct fxc_stream {
   // pair state
   double *d_acc = nullptr;
   float *d_hist = nullptr;
+  // running PWS (sub-stacks of pws_B records; 0 = off)
+  int pws_B = 0;
+  float2 *d_pws_S = nullptr;
+  float *d_pws_open = nullptr;
+  std::vector<int> open_cnt;  // records in each slot's open sub-stack
   std::unordered_map<long long, int> slot_of;
   std::vector<long long> count;  // records assigned per slot (submission order)
   hipStream_t upd = nullptr;
@@ -187,11 +214,22 @@ namespace {
 
 int lane_run(fxc_stream *s, StreamLane &L, const StreamJob &job, const std::vector<int> &src,
              const std::vector<int> &rec, const std::vector<int> &slots,
-             const std::vector<int> &hpos) {
+             const std::vector<int> &hpos, const std::vector<int> &order,
+             const std::vector<int> &close) {
   const int n = (int)job.chan.size();
   const long long np = (long long)src.size();
-  FXC_HIP(hipMemcpyAsync(L.d_traces, job.h_traces, (size_t)n * s->cfg.npts * sizeof(float),
+  const long long nclose = (long long)close.size() / 2;
+  // traces in channel-id order, so every pair's lower id has the lower local
+  // index and the pairs fill the upper-triangle station tiles only
+  FXC_HIP(hipMemcpyAsync(L.d_raw, job.h_traces, (size_t)n * s->cfg.npts * sizeof(float),
                          hipMemcpyHostToDevice, L.st));
+  std::memcpy(L.h_perm, order.data(), (size_t)n * sizeof(int));
+  FXC_HIP(hipMemcpyAsync(L.d_perm, L.h_perm, (size_t)n * sizeof(int), hipMemcpyHostToDevice, L.st));
+  hipLaunchKernelGGL(stream_gather_kernel,
+                     dim3((unsigned)std::min<size_t>(64, ceil_div((size_t)s->cfg.npts, kStrThreads)),
+                          (unsigned)n),
+                     dim3(kStrThreads), 0, L.st, L.d_raw, L.d_perm, s->cfg.npts, L.d_traces);
+  FXC_HIP(hipGetLastError());
   std::vector<int> steps((size_t)s->nstep);
   for (int i = 0; i < s->nstep; ++i) steps[i] = i;
   FXC_TRY(fxc_s2s_process(L.s2s, L.d_traces, n, steps.data(), s->nstep, L.d_spec, L.st));
@@ -201,6 +239,11 @@ int lane_run(fxc_stream *s, StreamLane &L, const StreamJob &job, const std::vect
   std::memcpy(L.h_hpos, hpos.data(), (size_t)np * sizeof(int));
   FXC_HIP(hipMemcpyAsync(L.d_slot, L.h_slot, (size_t)np * sizeof(int), hipMemcpyHostToDevice, L.st));
   FXC_HIP(hipMemcpyAsync(L.d_hpos, L.h_hpos, (size_t)np * sizeof(int), hipMemcpyHostToDevice, L.st));
+  if (nclose > 0) {
+    std::memcpy(L.h_close, close.data(), (size_t)nclose * 2 * sizeof(int));
+    FXC_HIP(hipMemcpyAsync(L.d_close, L.h_close, (size_t)nclose * 2 * sizeof(int),
+                           hipMemcpyHostToDevice, L.st));
+  }
   FXC_HIP(hipEventRecord(L.ev_cc, L.st));
   // State updates are enqueued in admission order: a history position is
   // count mod history, so a record may only land after every earlier record
@@ -221,9 +264,17 @@ int lane_run(fxc_stream *s, StreamLane &L, const StreamJob &job, const std::vect
     } else {
       hipLaunchKernelGGL(stream_update_kernel, row_grid(s->ncc, np), dim3(kStrThreads), 0,
                          s->upd, L.d_cc, s->ncc, np, L.d_slot, L.d_hpos, s->d_acc, s->d_hist,
-                         s->history);
-      if (hipGetLastError() != hipSuccess || hipEventRecord(L.ev_done, s->upd) != hipSuccess) {
+                         s->history, s->d_pws_open);
+      if (hipGetLastError() != hipSuccess) {
         set_error("stream: update launch failed");
+        rc = FXC_ERR_HIP;
+      }
+      // sub-stacks this segment completes: [slots | counts] in d_close
+      if (rc == FXC_OK && nclose > 0)
+        rc = stream_pws_close(s->d_pws_S, s->d_pws_open, L.d_close, L.d_close + nclose, nclose,
+                              s->ncc, s->upd);
+      if (rc == FXC_OK && hipEventRecord(L.ev_done, s->upd) != hipSuccess) {
+        set_error("stream: update event failed");
         rc = FXC_ERR_HIP;
       }
     }
@@ -242,7 +293,7 @@ void lane_main(fxc_stream *s, StreamLane *L) {
   (void)hipSetDevice(s->dev);
   for (;;) {
     StreamJob job;
-    std::vector<int> src, rec, slots, hpos;
+    std::vector<int> src, rec, slots, hpos, order, close;
     int status = FXC_OK;
     std::string msg;
     {
@@ -253,7 +304,7 @@ void lane_main(fxc_stream *s, StreamLane *L) {
       s->queue.pop_front();
       // admission, in submission order: pairs (lower channel id first), slots
       const int n = (int)job.chan.size();
-      std::vector<int> order((size_t)n);
+      order.resize((size_t)n);
       for (int i = 0; i < n; ++i) order[i] = i;
       std::sort(order.begin(), order.end(),
                 [&](int a, int b) { return job.chan[a] < job.chan[b]; });
@@ -263,10 +314,12 @@ void lane_main(fxc_stream *s, StreamLane *L) {
       std::vector<long long> keys;
       keys.reserve((size_t)np);
       long long fresh = 0;
+      // local indices are positions in channel-id order (the lane gathers the
+      // traces into that order)
       for (int a = 0; a < n; ++a)
         for (int b = a + 1; b < n; ++b) {
-          src.push_back(order[a]);
-          rec.push_back(order[b]);
+          src.push_back(a);
+          rec.push_back(b);
           const long long key = (long long)job.chan[order[a]] * s->nchan + job.chan[order[b]];
           keys.push_back(key);
           if (!s->slot_of.count(key)) ++fresh;
@@ -285,18 +338,26 @@ void lane_main(fxc_stream *s, StreamLane *L) {
             sl = (int)s->slot_of.size();
             s->slot_of.emplace(keys[p], sl);
             s->count.push_back(0);
+            s->open_cnt.push_back(0);
           } else {
             sl = it->second;
           }
           slots[p] = sl;
           hpos[p] = s->history > 0 ? (int)(s->count[sl] % s->history) : 0;
           ++s->count[sl];
+          if (s->pws_B > 0 && ++s->open_cnt[sl] == s->pws_B) {
+            close.push_back(sl);
+            s->open_cnt[sl] = 0;
+          }
         }
+        // [slots | counts] of the sub-stacks this segment closes
+        const size_t nc = close.size();
+        close.resize(2 * nc, s->pws_B);
         job.seq = s->admitted++;
       }
     }
     if (status == FXC_OK) {
-      status = lane_run(s, *L, job, src, rec, slots, hpos);
+      status = lane_run(s, *L, job, src, rec, slots, hpos, order, close);
       if (status != FXC_OK) {
         // a lane that failed before its update turn still passes the turn on
         {
@@ -337,19 +398,24 @@ void stream_free(fxc_stream *s) {
   for (auto &L : s->lanes) {
     if (L.s2s) fxc_s2s_plan_destroy(L.s2s);
     if (L.xc) fxc_xc_plan_destroy(L.xc);
+    dfree(L.d_raw);
     dfree(L.d_traces);
     dfree(L.d_spec);
     dfree(L.d_cc);
     dfree(L.d_slot);
     dfree(L.d_hpos);
-    if (L.h_slot) (void)hipHostFree(L.h_slot);
-    if (L.h_hpos) (void)hipHostFree(L.h_hpos);
+    dfree(L.d_perm);
+    dfree(L.d_close);
+    for (int *h : {L.h_slot, L.h_hpos, L.h_perm, L.h_close})
+      if (h) (void)hipHostFree(h);
     if (L.ev_cc) (void)hipEventDestroy(L.ev_cc);
     if (L.ev_done) (void)hipEventDestroy(L.ev_done);
     if (L.st) (void)hipStreamDestroy(L.st);
   }
   dfree(s->d_acc);
   dfree(s->d_hist);
+  dfree(s->d_pws_S);
+  dfree(s->d_pws_open);
   if (s->upd) (void)hipStreamDestroy(s->upd);
   delete s;
 }
@@ -365,7 +431,12 @@ int lane_init(fxc_stream *s, StreamLane &L) {
   FXC_TRY(fxc_xc_set_chunk_rows(L.xc, s->chunk_rows));
   const int n = s->max_seg_chan;
   const long long np = (long long)n * (n - 1) / 2;
+  FXC_TRY(dmalloc((void **)&L.d_raw, (size_t)n * s->cfg.npts * sizeof(float)));
   FXC_TRY(dmalloc((void **)&L.d_traces, (size_t)n * s->cfg.npts * sizeof(float)));
+  FXC_TRY(dmalloc((void **)&L.d_perm, (size_t)n * sizeof(int)));
+  FXC_TRY(dmalloc((void **)&L.d_close, (size_t)np * 2 * sizeof(int)));
+  FXC_HIP(hipHostMalloc((void **)&L.h_perm, (size_t)n * sizeof(int), hipHostMallocDefault));
+  FXC_HIP(hipHostMalloc((void **)&L.h_close, (size_t)np * 2 * sizeof(int), hipHostMallocDefault));
   FXC_TRY(dmalloc((void **)&L.d_spec, (size_t)s->nstep * n * s->nspec * 2 * sizeof(float)));
   FXC_TRY(dmalloc((void **)&L.d_cc, (size_t)np * s->ncc * sizeof(float)));
   FXC_TRY(dmalloc((void **)&L.d_slot, (size_t)np * sizeof(int)));
@@ -513,13 +584,61 @@ extern "C" int fxc_stream_info(fxc_stream *s, long long *pairs_used, long long *
   return FXC_OK;
 }
 
+extern "C" int fxc_stream_set_pws(fxc_stream *s, int substack) {
+  FXC_CHECK_ARG(s && substack >= 1, "fxc_stream_set_pws: invalid arguments");
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    FXC_CHECK_ARG(s->submitted == 0 && s->pws_B == 0,
+                  "fxc_stream_set_pws: must be called once, before the first segment");
+  }
+  const size_t n = (size_t)s->max_pairs * s->ncc;
+  FXC_TRY(dmalloc((void **)&s->d_pws_S, n * sizeof(float2)));
+  FXC_TRY(dmalloc((void **)&s->d_pws_open, n * sizeof(float)));
+  FXC_HIP(hipMemset(s->d_pws_S, 0, n * sizeof(float2)));
+  FXC_HIP(hipMemset(s->d_pws_open, 0, n * sizeof(float)));
+  s->state_bytes += n * (sizeof(float2) + sizeof(float));
+  s->pws_B = substack;
+  return FXC_OK;
+}
+
+static int stream_read_impl(fxc_stream *s, int kind, const int *h_src, const int *h_rec,
+                            long long npairs, int substack, int band_limited, float fmin,
+                            float fmax, float taper_hz, const fxc_stack_spec *spec, float *d_out,
+                            long long *h_count, void *stream);
+
 extern "C" int fxc_stream_read(fxc_stream *s, int kind, const int *h_src, const int *h_rec,
                                long long npairs, int substack, int band_limited, float fmin,
                                float fmax, float taper_hz, float *d_out, long long *h_count,
                                void *stream) {
+  return stream_read_impl(s, kind, h_src, h_rec, npairs, substack, band_limited, fmin, fmax,
+                          taper_hz, nullptr, d_out, h_count, stream);
+}
+
+extern "C" int fxc_stream_read_spec(fxc_stream *s, int kind, const int *h_src, const int *h_rec,
+                                    long long npairs, int substack, int band_limited, float fmin,
+                                    float fmax, float taper_hz, const fxc_stack_spec *spec,
+                                    float *d_out, long long *h_count, void *stream) {
+  return stream_read_impl(s, kind, h_src, h_rec, npairs, substack, band_limited, fmin, fmax,
+                          taper_hz, spec, d_out, h_count, stream);
+}
+
+static int stream_read_impl(fxc_stream *s, int kind, const int *h_src, const int *h_rec,
+                            long long npairs, int substack, int band_limited, float fmin,
+                            float fmax, float taper_hz, const fxc_stack_spec *spec, float *d_out,
+                            long long *h_count, void *stream) {
   FXC_CHECK_ARG(s && h_src && h_rec && d_out && npairs >= 1 && kind >= 0 && kind <= 2,
                 "fxc_stream_read: invalid arguments");
-  FXC_CHECK_ARG(kind == 0 || s->history > 0, "fxc_stream_read: PWS / TF-PWS need a history");
+  FXC_CHECK_ARG(!spec || (std::isfinite(spec->power) && spec->power > 0.f &&
+                          (spec->nvoices == 0 || spec->nvoices >= 2)),
+                "fxc_stream_read: invalid spec");
+  const float power = spec ? spec->power : 1.f;
+  // PWS from the running state when the engine keeps one, else (and for
+  // TF-PWS) from the history ring
+  const bool running_pws = kind == 1 && s->pws_B > 0;
+  FXC_CHECK_ARG(!running_pws || substack <= 1 || substack == s->pws_B,
+                "fxc_stream_read: the running PWS state has sub-stacks of %d records", s->pws_B);
+  FXC_CHECK_ARG(kind == 0 || running_pws || s->history > 0,
+                "fxc_stream_read: PWS / TF-PWS need a history or the running PWS state");
   hipStream_t st = (hipStream_t)stream;
   // every segment submitted so far is applied first
   std::unique_lock<std::mutex> lk(s->mu);
@@ -527,6 +646,7 @@ extern "C" int fxc_stream_read(fxc_stream *s, int kind, const int *h_src, const 
   FXC_CHECK_ARG(s->broken.empty(), "%s", s->broken.c_str());
   std::vector<int> slot((size_t)npairs, 0), start((size_t)npairs, 0), nrec((size_t)npairs, 0);
   std::vector<long long> cnt((size_t)npairs, 0);
+  std::vector<int> nopen((size_t)npairs, 0);
   for (long long p = 0; p < npairs; ++p) {
     FXC_CHECK_ARG(h_src[p] >= 0 && h_rec[p] < s->nchan && h_src[p] < h_rec[p],
                   "fxc_stream_read: pair %lld must be channel ids src < rec", p);
@@ -534,8 +654,11 @@ extern "C" int fxc_stream_read(fxc_stream *s, int kind, const int *h_src, const 
     if (it != s->slot_of.end()) {
       slot[p] = it->second;
       cnt[p] = s->count[it->second];
+      nopen[p] = s->open_cnt[it->second];
     }
-    if (kind != 0) {
+    if (running_pws) {
+      FXC_CHECK_ARG(cnt[p] > 0, "fxc_stream_read: pair %lld has no records", p);
+    } else if (kind != 0) {
       FXC_CHECK_ARG(cnt[p] > 0, "fxc_stream_read: pair %lld has no records", p);
       nrec[p] = (int)std::min<long long>(cnt[p], s->history);
       start[p] = (int)((cnt[p] - nrec[p]) % s->history);
@@ -558,6 +681,20 @@ extern "C" int fxc_stream_read(fxc_stream *s, int kind, const int *h_src, const 
   int *d_slot = nullptr;
   FXC_TRY(dev((void **)&d_slot, (size_t)npairs * sizeof(int)));
   FXC_HIP(hipMemcpyAsync(d_slot, slot.data(), (size_t)npairs * sizeof(int), hipMemcpyHostToDevice, st));
+  if (running_pws) {
+    long long *d_cnt = nullptr;
+    int *d_nopen = nullptr;
+    FXC_TRY(dev((void **)&d_cnt, (size_t)npairs * sizeof(long long)));
+    FXC_TRY(dev((void **)&d_nopen, (size_t)npairs * sizeof(int)));
+    FXC_HIP(hipMemcpyAsync(d_cnt, cnt.data(), (size_t)npairs * sizeof(long long),
+                           hipMemcpyHostToDevice, st));
+    FXC_HIP(hipMemcpyAsync(d_nopen, nopen.data(), (size_t)npairs * sizeof(int),
+                           hipMemcpyHostToDevice, st));
+    FXC_TRY(stream_pws_read(s->d_pws_S, s->d_pws_open, s->d_acc, d_slot, d_cnt, d_nopen, npairs,
+                            s->ncc, power, d_out, st));
+    FXC_HIP(hipStreamSynchronize(st));
+    return FXC_OK;
+  }
   if (kind == 0) {
     long long *d_cnt = nullptr;
     FXC_TRY(dev((void **)&d_cnt, (size_t)npairs * sizeof(long long)));
@@ -600,10 +737,10 @@ extern "C" int fxc_stream_read(fxc_stream *s, int kind, const int *h_src, const 
                      kind == 2 ? 1 : 0, d_pre, d_lin);
   FXC_HIP(hipGetLastError());
   if (kind == 1)
-    FXC_TRY(fxc_pws(d_pre, d_w, goff.data(), d_lin, (int)npairs, s->ncc, d_out, st));
+    FXC_TRY(fxc_pws_spec(d_pre, d_w, goff.data(), d_lin, (int)npairs, s->ncc, spec, d_out, st));
   else
-    FXC_TRY(fxc_tfpws(d_pre, d_w, goff.data(), d_lin, (int)npairs, s->ncc, band_limited,
-                      s->cfg.delta, fmin, fmax, taper_hz, d_out, st));
+    FXC_TRY(fxc_tfpws_spec(d_pre, d_w, goff.data(), d_lin, (int)npairs, s->ncc, band_limited,
+                           s->cfg.delta, fmin, fmax, taper_hz, spec, d_out, st));
   FXC_HIP(hipStreamSynchronize(st));
   return FXC_OK;
 }

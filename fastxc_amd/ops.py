@@ -456,7 +456,7 @@ class StreamEngine:
     def __init__(self, npts, seg, shift, nfft_filter, nfft_out, num_ch, delta, freq_low, idx,
                  responses, freq_lows, nchan, half_cc, precision="bf16x3", whiten=2,
                  normalize=3, phase_only=False, history=0, max_pairs=1 << 20, max_seg_chan=512,
-                 lanes=2):
+                 lanes=2, pws_substack=0):
         resp = np.ascontiguousarray(responses, dtype=np.float32)
         lows = np.ascontiguousarray(freq_lows, dtype=np.float32)
         self._keep = (resp, lows)
@@ -468,8 +468,10 @@ class StreamEngine:
         check(lib().fxc_stream_create(ctypes.byref(self._h), ctypes.byref(cfg), nchan, half_cc,
                                       PRECISIONS[precision], history, max_pairs, max_seg_chan,
                                       lanes), "fxc_stream_create")
+        if pws_substack:
+            check(lib().fxc_stream_set_pws(self._h, int(pws_substack)), "fxc_stream_set_pws")
         self.npts, self.nchan = npts, nchan
-        self.ncc = 2 * min(half_cc, nfft_out // 2) + 1
+        self.ncc = 2 * min(half_cc, nfft_out // 2 - 1) + 1
         self._inflight = {}
 
     def submit(self, chan, traces) -> int:
@@ -501,19 +503,28 @@ class StreamEngine:
         return {"pairs_used": a.value, "segments_done": b.value, "state_bytes": c.value,
                 "lane_bytes": d.value}
 
-    def read(self, src, rec, kind="linear", substack=1, band=None, device="cuda:0"):
+    def read(self, src, rec, kind="linear", substack=1, band=None, device="cuda:0", power=1.0,
+             nvoices=0):
         """Stacks of pairs (src[i] < rec[i]) after every submitted segment:
-        float32 [npairs, 2H+1] device tensor and the record counts."""
+        float32 [npairs, 2H+1] device tensor and the record counts.  PWS comes
+        from the running state when the engine keeps one (pws_substack),
+        else from the history; power / nvoices select the spec mode."""
         s = np.ascontiguousarray(src, dtype=np.int32)
         r = np.ascontiguousarray(rec, dtype=np.int32)
         out = torch.empty((len(s), self.ncc), dtype=torch.float32, device=device)
         cnt = np.zeros(len(s), dtype=np.int64)
         bl = band is not None
         fmin, fmax, taper = band if bl else (0.0, 0.0, 0.0)
-        check(lib().fxc_stream_read(self._h, self.KINDS[kind], s.ctypes.data, r.ctypes.data,
-                                    len(s), int(substack), int(bl), float(fmin), float(fmax),
-                                    float(taper), out.data_ptr(), cnt.ctypes.data,
-                                    _stream_ptr(out.device)), "fxc_stream_read")
+        args = (self._h, self.KINDS[kind], s.ctypes.data, r.ctypes.data, len(s), int(substack),
+                int(bl), float(fmin), float(fmax), float(taper))
+        if power == 1.0 and nvoices == 0:
+            check(lib().fxc_stream_read(*args, out.data_ptr(), cnt.ctypes.data,
+                                        _stream_ptr(out.device)), "fxc_stream_read")
+        else:
+            spec = StackSpec(float(power), int(nvoices))
+            check(lib().fxc_stream_read_spec(*args, ctypes.byref(spec), out.data_ptr(),
+                                             cnt.ctypes.data, _stream_ptr(out.device)),
+                  "fxc_stream_read_spec")
         return out, cnt
 
     def close(self):

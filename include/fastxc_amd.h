@@ -310,6 +310,19 @@ int fxc_stream_info(fxc_stream *stream, long long *pairs_used, long long *segmen
 int fxc_stream_read(fxc_stream *stream, int kind, const int *h_src, const int *h_rec,
                     long long npairs, int substack, int band_limited, float fmin, float fmax,
                     float taper_hz, float *d_out, long long *h_count, void *hip_stream);
+/* Running PWS state (call once, before the first segment): every pair then
+ * keeps sum_g n_g phi_g over its closed sub-stacks of `substack` records
+ * and the sum of its open sub-stack (12 B per lag per pair of max_pairs),
+ * so fxc_stream_read kind 1 returns the PWS of ALL its records in
+ * submission order, equal to fxc_pws over them with sub-stacks of
+ * `substack` (the reference's PWS, native/pws/io/sourcepack_io.cu:385-478,
+ * native/pws/kernels/cuda.pws.cu:28-95; lin is the float64 linear stack). */
+int fxc_stream_set_pws(fxc_stream *stream, int substack);
+/* fxc_stream_read with the stacks' spec mode (power; voices for TF-PWS). */
+int fxc_stream_read_spec(fxc_stream *stream, int kind, const int *h_src, const int *h_rec,
+                         long long npairs, int substack, int band_limited, float fmin, float fmax,
+                         float taper_hz, const fxc_stack_spec *spec, float *d_out,
+                         long long *h_count, void *hip_stream);
 int fxc_stream_destroy(fxc_stream *stream);
 
 #ifdef __cplusplus

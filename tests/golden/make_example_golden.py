@@ -1,8 +1,8 @@
 """Writes the example fixtures under tests/golden/example/.
 
 Inputs: a subset of the reference's bundled example (example/data in the
-reference tree): stations A7K2 and B4Q8, components E, N, Z, days 2022.137
-and 2022.138 at 1 Hz, copied byte for byte as SAC files, and the filter file
+reference tree): stations A7K2 and B4Q8, components E, N, Z, days 2022.138
+and 2022.139 at 1 Hz, copied byte for byte as SAC files, and the filter file
 of the example's bands (example/config.ini: 0.2/0.4 and 0.1/0.2 Hz, so the
 broad band 0.1/0.4 first), written as the reference's design_filter writes it.
 
@@ -15,6 +15,11 @@ two days and PWS / TF-PWS over the days with sub-stacks of one record
 the linear stack).  Saved in golden.npz: per day the nine component-pair
 correlations of the pair A7K2-B4Q8 in the reference's task order, and the
 three stacks.
+
+Day 2022.137 is left out: B4Q8 records zeros there for 4.8 hours (and C9M3
+for 3.5), and a window that is part data, part zeros hands RUN-ABS-MF the
+prefilter's decaying tail, which it scales up to O(1) from values near
+FP32 rounding -- no FP32 path (the reference's included) matches FP64 there.
 
 Run in a tree that holds the reference at /root/reference:
     python tests/golden/make_example_golden.py
@@ -33,7 +38,7 @@ sys.path.insert(0, ROOT)
 
 REF_DATA = "/root/reference/example/data"
 STATIONS = ["A7K2", "B4Q8"]
-DAYS = ["137", "138"]
+DAYS = ["138", "139"]
 COMPS = ["E", "N", "Z"]
 BANDS = "0.2/0.4 0.1/0.2"
 OUT = os.path.join(HERE, "example")

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 EX = os.path.join(HERE, "golden", "example")
 STATIONS = ["A7K2", "B4Q8"]
-DAYS = ["137", "138"]
+DAYS = ["138", "139"]
 COMPS = ["E", "N", "Z"]
 
 

@@ -31,14 +31,15 @@ def _plan(tmp_path):
     return s2s.plan_lengths(180, 0.1, 180, 180, 20, f)
 
 
-def _engine(plan, lanes, history=0, precision="fp32", max_pairs=1 << 12, max_seg_chan=NCHAN):
+def _engine(plan, lanes, history=0, precision="fp32", max_pairs=1 << 12, max_seg_chan=NCHAN,
+            pws_substack=0):
     from fastxc_amd.ops import StreamEngine
 
     resp = np.stack([r[: plan.nfft_filter // 2 + 1] for r in plan.responses])
     return StreamEngine(plan.npts, plan.seg, plan.shift, plan.nfft_filter, plan.nfft_out, 1,
                         plan.delta, plan.freq_low, plan.idx, resp, plan.freq_lows, NCHAN, 200,
                         precision=precision, history=history, max_pairs=max_pairs,
-                        max_seg_chan=max_seg_chan, lanes=lanes)
+                        max_seg_chan=max_seg_chan, lanes=lanes, pws_substack=pws_substack)
 
 
 def _segments(seed, count):
@@ -182,4 +183,40 @@ def test_refuses_segment_beyond_pair_state(gpu, tmp_path):
         eng.submit(np.array([1, 1], np.int32), np.zeros((2, 1800), np.float32))
     with pytest.raises(FxcError, match="out of range"):
         eng.submit(np.array([1, NCHAN], np.int32), np.zeros((2, 1800), np.float32))
+    eng.close()
+
+
+@pytest.mark.parametrize("B", [1, 3])
+def test_running_pws_and_history_tfpws_match_fp64_oracle(gpu, tmp_path, B):
+    """The engine's running PWS over every record of a pair (sub-stacks of B in
+    submission order; power 1 and the spec mode's power 2) and its TF-PWS over
+    the history ring, against oracle/stack.py over the FP64 oracle's
+    correlations of the same segments (channels submitted out of id order)."""
+    from oracle import stack as ostack
+
+    plan = _plan(tmp_path)
+    segs = _segments(20 + B, 12)
+    want = _oracle_ccfs(plan, segs)
+    keys = sorted(k for k in want if len(want[k]) >= 2)
+    src = [k[0] for k in keys]
+    rec = [k[1] for k in keys]
+    hist = 4
+    eng = _engine(plan, lanes=4, history=hist, pws_substack=B)
+    _run(eng, segs)
+    for power in (1.0, 2.0):
+        got, cnt = eng.read(src, rec, "pws", substack=B, power=power)
+        got = got.cpu().numpy()
+        for i, k in enumerate(keys):
+            recs = np.stack(want[k])
+            assert cnt[i] == len(recs)
+            g, w, lin = ostack.pws_prepare(recs, B)
+            err = rel_l2(got[i], ostack.pws(g, w, lin, power=power))
+            assert err < 1e-4, (k, power, err)
+    got, _ = eng.read(src, rec, "tfpws", substack=B)
+    got = got.cpu().numpy()
+    for i, k in enumerate(keys):
+        recs = np.stack(want[k])[-hist:]
+        s_, w, lin = ostack.tfpws_prepare(recs, B)
+        err = rel_l2(got[i], ostack.tfpws(s_, w, lin, delta=plan.delta))
+        assert err < 1e-3, (k, err)
     eng.close()
