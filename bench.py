@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--nstep", type=int, default=24)
     ap.add_argument("--nspec", type=int, default=18523)
     ap.add_argument("--half-cc", type=int, default=1000)
-    ap.add_argument("--precision", default="bf16x3,fp16,fp32,plain",
+    ap.add_argument("--precision", default="bf16x3,fp16x3,fp16,fp32,plain",
                     help="comma list; the first is the headline value")
     ap.add_argument("--check-pairs", type=int, default=300,
                     help="pairs of every form checked against an FP64 evaluation")
@@ -918,7 +918,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"bf16x3": "bf16x3-split/f32-acc", "fp32": "f32", "fp16": "f16/f32-acc",
-                  "plain": "f32"}[head["precision"]],
+                  "fp16x3": "f16x3-split/f32-acc", "plain": "f32"}[head["precision"]],
         "data": "synthetic",
         "config": {"workload": f"xc_fast one timestamp: {args.nsta} stations x {args.nstep} windows, "
                                f"nspec {args.nspec} (nfft {2 * (args.nspec - 1)}), {len(src)} pairs, "

@@ -17,8 +17,9 @@ FXC_PREC_FP32 = 0
 FXC_PREC_BF16X3 = 1
 FXC_PREC_FP16 = 2
 FXC_PREC_PLAIN = 3
+FXC_PREC_FP16X3 = 4
 PRECISIONS = {"fp32": FXC_PREC_FP32, "bf16x3": FXC_PREC_BF16X3, "fp16": FXC_PREC_FP16,
-              "plain": FXC_PREC_PLAIN}
+              "plain": FXC_PREC_PLAIN, "fp16x3": FXC_PREC_FP16X3}
 
 _lib = None
 

@@ -15,7 +15,8 @@
 //   FP32    plain fp32 FMA on the vector ALU (the parity path);
 //   BF16X3  spectra split into bf16 hi+lo, three MFMA products
 //           (hi*hi + hi*lo + lo*hi) with fp32 accumulation;
-//   FP16    one fp16 MFMA product with a power-of-two scale per station.
+//   FP16    one fp16 MFMA product with a power-of-two scale per station;
+//   FP16X3  the scaled spectra split into fp16 hi+lo, three products as BF16X3.
 // A fourth form, PLAIN, keeps the reference's own step order on the GPU (the
 // stack row of every pair read, updated and written once per window, fp32);
 // it exists to time the reference's method beside the tiled forms.
@@ -140,7 +141,9 @@ __device__ __forceinline__ void split_part(const float2 *__restrict__ spec, int 
       const int kk = i / kSplitSta;
       const int sidx = s0 + ss, k = k0 + kk;
       if (sidx >= nsta || k >= nspec) continue;
-      const float sc = (kMode == FXC_PREC_FP16) ? inv_scale[sta_off + sidx] : 1.f;
+      const float sc = (kMode == FXC_PREC_FP16 || kMode == FXC_PREC_FP16X3)
+                           ? inv_scale[sta_off + sidx]
+                           : 1.f;
 #pragma unroll
       for (int part = 0; part < 2; ++part) {  // windows wb + 4 part .. + 3
         unsigned short h[8], l[8];
@@ -151,6 +154,10 @@ __device__ __forceinline__ void split_part(const float2 *__restrict__ spec, int 
           if (kMode == FXC_PREC_FP16) {
             h[j] = __builtin_bit_cast(unsigned short, (_Float16)v);
             l[j] = 0;
+          } else if (kMode == FXC_PREC_FP16X3) {
+            const _Float16 hh = (_Float16)v;
+            h[j] = __builtin_bit_cast(unsigned short, hh);
+            l[j] = __builtin_bit_cast(unsigned short, (_Float16)(v - (float)hh));
           } else {
             const unsigned short hb = bf16_bits_rne(v);
             h[j] = hb;
@@ -163,7 +170,7 @@ __device__ __forceinline__ void split_part(const float2 *__restrict__ spec, int 
         hv.x = h[0] | ((unsigned)h[1] << 16); hv.y = h[2] | ((unsigned)h[3] << 16);
         hv.z = h[4] | ((unsigned)h[5] << 16); hv.w = h[6] | ((unsigned)h[7] << 16);
         *(uint4 *)(hi + o) = hv;
-        if (kMode == FXC_PREC_BF16X3) {
+        if (kMode == FXC_PREC_BF16X3 || kMode == FXC_PREC_FP16X3) {
           uint4 lv;
           lv.x = l[0] | ((unsigned)l[1] << 16); lv.y = l[2] | ((unsigned)l[3] << 16);
           lv.z = l[4] | ((unsigned)l[5] << 16); lv.w = l[6] | ((unsigned)l[7] << 16);
@@ -398,7 +405,7 @@ __device__ __forceinline__ void load_rec(const unsigned short *b, size_t gstride
 
 template <int kMode>
 __device__ __forceinline__ void mfma32(uint4 a, uint4 b, f32x4_t &acc) {
-  if constexpr (kMode == FXC_PREC_FP16) {
+  if constexpr (kMode == FXC_PREC_FP16 || kMode == FXC_PREC_FP16X3) {
     acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, a),
                                                  __builtin_bit_cast(f16x8_t, b), acc, 0, 0, 0);
   } else {
@@ -409,7 +416,7 @@ __device__ __forceinline__ void mfma32(uint4 a, uint4 b, f32x4_t &acc) {
 
 template <int kMode>
 __device__ __forceinline__ void mfma16(uint2 a, uint2 b, f32x4_t &acc) {
-  if constexpr (kMode == FXC_PREC_FP16) {
+  if constexpr (kMode == FXC_PREC_FP16 || kMode == FXC_PREC_FP16X3) {
     acc = __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4_t, a),
                                                 __builtin_bit_cast(f16x4_t, b), acc, 0, 0, 0);
   } else {
@@ -475,11 +482,12 @@ __device__ __forceinline__ void item_addr(const TileDesc &td, int chunk, int nst
     ia.s_off[h] = (size_t)(s_sta < nsta ? s_sta : nsta - 1) * 8;
     ia.r_off[h] = (size_t)(r_sta < nsta ? r_sta : nsta - 1) * 8;
     ia.col_scale[h] = scale;
-    if constexpr (kMode == FXC_PREC_FP16) ia.col_scale[h] *= (r_sta < nsta ? sta_scale[r_sta] : 0.f);
+    if constexpr (kMode == FXC_PREC_FP16 || kMode == FXC_PREC_FP16X3)
+      ia.col_scale[h] *= (r_sta < nsta ? sta_scale[r_sta] : 0.f);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       ia.row_scale[h][r] = 1.f;
-      if constexpr (kMode == FXC_PREC_FP16) {
+      if constexpr (kMode == FXC_PREC_FP16 || kMode == FXC_PREC_FP16X3) {
         const int sa = td.src_base + h * 16 + 4 * lane_g + r;
         ia.row_scale[h][r] = sa < nsta ? sta_scale[sa] : 0.f;
       }
@@ -488,16 +496,20 @@ __device__ __forceinline__ void item_addr(const TileDesc &td, int chunk, int nst
   ia.k0 = chunk * kBinChunk;
 }
 
+// kp_total: the operand depth K' of one bin in the split layout (16 NK16,
+// or a multiple of it when the windows are taken in chunks of K' = 16 NK16);
+// kc: the chunk loaded.
 template <int kMode, int NK16, bool kSplit>
 __device__ __forceinline__ void load_tile_ops(const unsigned short *__restrict__ hi,
                                               const unsigned short *__restrict__ lo,
                                               const ItemAddr &ia, int b, int nsta, int nspec,
-                                              int lane_g, TileOps<NK16, kSplit> &o) {
-  constexpr int kp = 16 * NK16;
+                                              int lane_g, TileOps<NK16, kSplit> &o,
+                                              int kp_total = 16 * NK16, int kc = 0) {
   const size_t gstride = (size_t)nsta * 8;
   int k = ia.k0 + b;
   k = k < nspec ? k : nspec - 1;  // tail bins compute a valid bin, never stored
-  const size_t kb = (size_t)k * kp * nsta;  // bin k: K' x nsta elements
+  // bin k: K' x nsta elements; chunk kc starts 2 NK16 K groups further per chunk
+  const size_t kb = (size_t)k * kp_total * nsta + (size_t)kc * (2 * NK16) * gstride;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     load_src(hi + kb + ia.s_off[h], gstride, lane_g, o.sh[h]);
@@ -553,6 +565,62 @@ __device__ __forceinline__ void compute_tile_bin(const TileOps<NK16, kSplit> &o,
     }
 }
 
+// More than 128 windows: the operand depth runs in kchunks chunks of
+// K' = 16 NK16 (32 windows at NK16 4), the four blocks' sums held in
+// registers across the chunks, then staged as compute_tile_bin stages them.
+template <int kMode, int NK16, bool kSplit>
+__device__ __forceinline__ void compute_tile_bin_chunked(const unsigned short *__restrict__ hi,
+                                                         const unsigned short *__restrict__ lo,
+                                                         const ItemAddr &ia, int b, int nsta,
+                                                         int nspec, int lane_r, int lane_g,
+                                                         int kchunks, float2 *stage) {
+  static_assert((NK16 & 1) == 0, "chunks are whole 16x16x32 steps");
+  f32x4_t cr[2][2], ci[2][2];
+#pragma unroll
+  for (int si = 0; si < 2; ++si)
+#pragma unroll
+    for (int sj = 0; sj < 2; ++sj) {
+      cr[si][sj] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      ci[si][sj] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    }
+  const int kp_total = 16 * NK16 * kchunks;
+#pragma unroll 1
+  for (int kc = 0; kc < kchunks; ++kc) {
+    TileOps<NK16, kSplit> o;
+    load_tile_ops<kMode>(hi, lo, ia, b, nsta, nspec, lane_g, o, kp_total, kc);
+    __builtin_amdgcn_sched_barrier(0);
+    RecB2<NK16> b2h[2], b2l[kSplit ? 2 : 1];
+#pragma unroll
+    for (int sj = 0; sj < 2; ++sj) {
+      derive_b2(o.rh[sj], b2h[sj]);
+      if constexpr (kSplit) derive_b2(o.rl[sj], b2l[sj]);
+    }
+#pragma unroll
+    for (int si = 0; si < 2; ++si)
+#pragma unroll
+      for (int sj = 0; sj < 2; ++sj) {
+        f32x4_t hr = {0.f, 0.f, 0.f, 0.f}, hi16 = {0.f, 0.f, 0.f, 0.f};  // unused (no 16-deep step)
+        mfma_bin<kMode>(o.sh[si], o.rh[sj], b2h[sj], cr[si][sj], ci[si][sj], hr, hi16);
+        if constexpr (kSplit) {
+          mfma_bin<kMode>(o.sl[si], o.rh[sj], b2h[sj], cr[si][sj], ci[si][sj], hr, hi16);
+          mfma_bin<kMode>(o.sh[si], o.rl[sj], b2l[sj], cr[si][sj], ci[si][sj], hr, hi16);
+        }
+      }
+  }
+#pragma unroll
+  for (int si = 0; si < 2; ++si)
+#pragma unroll
+    for (int sj = 0; sj < 2; ++sj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = si * 16 + 4 * lane_g + r;
+        const int c = sj * 16 + lane_r;
+        const float sc = ia.col_scale[sj] * ia.row_scale[si][r];
+        stage[stage_index(a * kTile + c, b)] =
+            make_float2(cr[si][sj][r] * sc, ci[si][sj][r] * sc);
+      }
+}
+
 // One work item = one 32x32 station tile x 16 bins; 8 waves, each computing
 // the whole tile (four 16x16 MFMA blocks) for two of the 16 bins, so every
 // station fragment is read once per workgroup.  Persistent: a workgroup walks
@@ -564,19 +632,20 @@ __device__ __forceinline__ void compute_tile_bin(const TileOps<NK16, kSplit> &o,
 // SIMD; waves_per_eu tells the scheduler so, and it then keeps a bin's
 // operand loads in flight together instead of trimming registers for an
 // occupancy the LDS never allows.
-template <int kMode, int NK16>
+template <int kMode, int NK16, bool kChunked = false>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
 contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short *__restrict__ lo,
                      int nsta, int nspec, const float *__restrict__ sta_scale,
                      const TileDesc *__restrict__ tiles, const int *__restrict__ tile_rows,
-                     float2 *__restrict__ xspec, long long xpitch, int ntiles, float scale) {
+                     float2 *__restrict__ xspec, long long xpitch, int ntiles, float scale,
+                     int kchunks) {
   extern __shared__ float2 stage[];  // [kTilePairs][kBinChunk], swizzled
   __shared__ int rows[kTilePairs];
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
   const int lane_r = lane & 15, lane_g = lane >> 4;
-  constexpr bool kSplit = kMode == FXC_PREC_BF16X3;
+  constexpr bool kSplit = kMode == FXC_PREC_BF16X3 || kMode == FXC_PREC_FP16X3;
   constexpr bool kPipe = NK16 * (kSplit ? 2 : 1) <= 8;
   constexpr int kBins = kBinChunk / 8;  // bins per wave and item
   const int nchunks = (nspec + kBinChunk - 1) / kBinChunk;
@@ -593,7 +662,9 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
   ItemAddr ia;
   item_addr<kMode>(tiles[tile], chunk, nsta, lane_r, lane_g, scale, sta_scale, ia);
   TileOps<NK16, kSplit> cur;
-  if constexpr (kPipe) load_tile_ops<kMode>(hi, lo, ia, wave, nsta, nspec, lane_g, cur);
+  // (the chunked-depth form takes the unpipelined path)
+  constexpr bool pipe = kPipe && !kChunked;
+  if constexpr (pipe) load_tile_ops<kMode>(hi, lo, ia, wave, nsta, nspec, lane_g, cur);
   // workspace rows of the tile's 1024 pairs, two per thread, loaded one item
   // ahead so the LDS fill after the barrier does not wait on memory
   static_assert(kTilePairs == 2 * 512, "two row entries per thread");
@@ -611,7 +682,7 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
       rreg0 = tile_rows[(size_t)ntile * kTilePairs + tid];
       rreg1 = tile_rows[(size_t)ntile * kTilePairs + tid + 512];
     }
-    if constexpr (kPipe) {
+    if constexpr (pipe) {
 #pragma unroll
       for (int bi = 0; bi < kBins; ++bi) {
         TileOps<NK16, kSplit> nxt;
@@ -624,6 +695,11 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
         compute_tile_bin<kMode>(cur, ia, wave + 8 * bi, lane_r, lane_g, stage);
         cur = nxt;
       }
+    } else if constexpr (kChunked) {
+#pragma unroll 1
+      for (int bi = 0; bi < kBins; ++bi)
+        compute_tile_bin_chunked<kMode, NK16, kSplit>(hi, lo, ia, wave + 8 * bi, nsta, nspec,
+                                                      lane_r, lane_g, kchunks, stage);
     } else {
 #pragma unroll 1
       for (int bi = 0; bi < kBins; ++bi) {
@@ -1044,8 +1120,13 @@ using namespace fxc;
 // (47 windows, 1000 stations: contraction 29.9 ms at kh 48, 45.3 ms at 64).
 static int xc_operand_depth(int nstep) {
   if (nstep <= 48) return nstep <= 8 ? 8 : (nstep + 7) / 8 * 8;
-  return nstep <= 64 ? 64 : nstep <= 96 ? 96 : 128;
+  if (nstep <= 128) return nstep <= 64 ? 64 : nstep <= 96 ? 96 : 128;
+  return (nstep + 31) / 32 * 32;  // chunks of 32 windows (see xc_depth_chunks)
 }
+
+// Chunks of 32 windows the contraction runs over for an operand depth kh
+// beyond the compiled depths (1 up to kh 128).
+static int xc_depth_chunks(int kh) { return kh > 128 ? kh / 32 : 1; }
 
 // ---------------------------------------------------------------------------
 // Plan object behind the C ABI.
@@ -1149,12 +1230,11 @@ extern "C" int fxc_xc_plan_create(fxc_xc_plan **out, int nspec, int nstep, float
                 "fxc_xc_plan_create: invalid shape nspec=%d nstep=%d dt=%g half_cc=%d", nspec,
                 nstep, (double)dt, half_cc);
   FXC_CHECK_ARG(precision == FXC_PREC_FP32 || precision == FXC_PREC_BF16X3 ||
-                    precision == FXC_PREC_FP16 || precision == FXC_PREC_PLAIN,
+                    precision == FXC_PREC_FP16 || precision == FXC_PREC_PLAIN ||
+                    precision == FXC_PREC_FP16X3,
                 "fxc_xc_plan_create: unknown precision %d", precision);
   const int nfft = 2 * (nspec - 1);
   FXC_CHECK_ARG(2 * half_cc + 1 <= nfft, "fxc_xc_plan_create: 2*half_cc+1 exceeds nfft");
-  if (precision == FXC_PREC_BF16X3 || precision == FXC_PREC_FP16)
-    FXC_CHECK_ARG(nstep <= 128, "fxc_xc_plan_create: MFMA forms support up to 128 windows");
   // lag butterfly set: radices up to 9 (default) or 8; FXC_XC_LAG_MAX_RADIX overrides
   int lag_max_radix = kLagMaxRadixDefault;
   if (const char *env = getenv("FXC_XC_LAG_MAX_RADIX")) lag_max_radix = atoi(env) >= 9 ? 9 : 8;
@@ -1401,7 +1481,8 @@ static int xc_ensure_workspace(fxc_xc_plan *p, size_t rows) {
     FXC_TRY(dmalloc((void **)&p->d_xspec, rows * (size_t)p->xpitch * sizeof(float2)));
     p->xspec_rows = rows;
   }
-  if (p->precision == FXC_PREC_BF16X3 || p->precision == FXC_PREC_FP16) {
+  if (p->precision == FXC_PREC_BF16X3 || p->precision == FXC_PREC_FP16 ||
+      p->precision == FXC_PREC_FP16X3) {
     const size_t elems = (size_t)p->nsta * p->nspec * (size_t)(2 * p->kh);
     if (elems > p->split_elems) {
       dfree(p->d_hi);
@@ -1409,7 +1490,7 @@ static int xc_ensure_workspace(fxc_xc_plan *p, size_t rows) {
       p->d_hi = p->d_lo = nullptr;
       p->split_elems = 0;
       FXC_TRY(dmalloc((void **)&p->d_hi, elems * sizeof(unsigned short)));
-      if (p->precision == FXC_PREC_BF16X3)
+      if (p->precision == FXC_PREC_BF16X3 || p->precision == FXC_PREC_FP16X3)
         FXC_TRY(dmalloc((void **)&p->d_lo, elems * sizeof(unsigned short)));
       p->split_elems = elems;
     }
@@ -1431,14 +1512,18 @@ static int xc_prepare_spectra(fxc_xc_plan *p, const float2 *d_spec, hipStream_t 
   if (p->precision == FXC_PREC_FP32 || p->precision == FXC_PREC_PLAIN) return FXC_OK;
   FXC_TRY(xc_mark(p, kTimeSplit, true, st));
   const dim3 grid((unsigned)ceil_div(p->nspec, kSplitBins), (unsigned)ceil_div(p->nsta, kSplitSta));
-  if (p->precision == FXC_PREC_FP16) {
+  if (p->precision == FXC_PREC_FP16 || p->precision == FXC_PREC_FP16X3) {
     FXC_HIP(hipMemsetAsync(p->d_amax, 0, p->nsta * sizeof(unsigned int), st));
     hipLaunchKernelGGL(station_absmax_kernel, dim3(8, p->nsta), dim3(256), 0, st, d_spec,
                        p->nstep, p->nsta, p->nspec, p->d_amax);
     hipLaunchKernelGGL(station_scale_kernel, dim3((unsigned)ceil_div(p->nsta, 256)), dim3(256), 0,
                        st, p->d_amax, p->nsta, p->d_inv_scale, p->d_scale);
-    hipLaunchKernelGGL(split_kernel<FXC_PREC_FP16>, grid, dim3(256), 0, st, d_spec, p->nstep,
-                       p->nsta, p->nspec, p->kh, p->d_inv_scale, p->d_hi, p->d_lo);
+    if (p->precision == FXC_PREC_FP16)
+      hipLaunchKernelGGL(split_kernel<FXC_PREC_FP16>, grid, dim3(256), 0, st, d_spec, p->nstep,
+                         p->nsta, p->nspec, p->kh, p->d_inv_scale, p->d_hi, p->d_lo);
+    else
+      hipLaunchKernelGGL(split_kernel<FXC_PREC_FP16X3>, grid, dim3(256), 0, st, d_spec, p->nstep,
+                         p->nsta, p->nspec, p->kh, p->d_inv_scale, p->d_hi, p->d_lo);
   } else {
     hipLaunchKernelGGL(split_kernel<FXC_PREC_BF16X3>, grid, dim3(256), 0, st, d_spec, p->nstep,
                        p->nsta, p->nspec, p->kh, (const float *)nullptr, p->d_hi, p->d_lo);
@@ -1448,12 +1533,12 @@ static int xc_prepare_spectra(fxc_xc_plan *p, const float2 *d_spec, hipStream_t 
   return FXC_OK;
 }
 
-template <int kMode, int KH>
+template <int kMode, int KH, bool kChunked = false>
 static int launch_mfma_kh(dim3 grid, size_t lds, hipStream_t st, const unsigned short *hi,
                           const unsigned short *lo, int nsta, int nspec, const float *sc,
                           const TileDesc *tiles, const int *trows, float2 *dst, long long xpitch,
-                          float scale) {
-  auto kern = contract_mfma_kernel<kMode, KH / 8>;
+                          float scale, int kchunks) {
+  auto kern = contract_mfma_kernel<kMode, KH / 8, kChunked>;
   // persistent grid: a multiple of 8 workgroups (one per CU, LDS-bound),
   // capped by the number of work items
   int dev = 0, ncu = 0;
@@ -1465,7 +1550,7 @@ static int launch_mfma_kh(dim3 grid, size_t lds, hipStream_t st, const unsigned 
   FXC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
   hipLaunchKernelGGL(kern, grid1, dim3(512), lds, st, hi, lo, nsta, nspec, sc, tiles, trows, dst,
-                     xpitch, (int)(grid.y), scale);
+                     xpitch, (int)(grid.y), scale, kchunks);
   return FXC_OK;
 }
 
@@ -1474,8 +1559,11 @@ static int launch_mfma_mode(int kh, dim3 grid, size_t lds, hipStream_t st,
                             const unsigned short *hi, const unsigned short *lo, int nsta, int nspec,
                             const float *sc, const TileDesc *tiles, const int *trows, float2 *dst,
                             long long xpitch, float scale) {
+  if (xc_depth_chunks(kh) > 1)
+    return launch_mfma_kh<kMode, 32, true>(grid, lds, st, hi, lo, nsta, nspec, sc, tiles, trows,
+                                           dst, xpitch, scale, xc_depth_chunks(kh));
 #define FXC_KH(V) \
-  case V: return launch_mfma_kh<kMode, V>(grid, lds, st, hi, lo, nsta, nspec, sc, tiles, trows, dst, xpitch, scale);
+  case V: return launch_mfma_kh<kMode, V>(grid, lds, st, hi, lo, nsta, nspec, sc, tiles, trows, dst, xpitch, scale, 1);
   switch (kh) {
     FXC_KH(8) FXC_KH(16) FXC_KH(24) FXC_KH(32) FXC_KH(40) FXC_KH(48) FXC_KH(64) FXC_KH(96)
     FXC_KH(128)
@@ -1490,6 +1578,9 @@ static int launch_mfma_contract(int precision, int kh, dim3 grid, size_t lds, hi
                                 const unsigned short *hi, const unsigned short *lo, int nsta,
                                 int nspec, const float *sc, const TileDesc *tiles,
                                 const int *trows, float2 *dst, long long xpitch, float scale) {
+  if (precision == FXC_PREC_FP16X3)
+    return launch_mfma_mode<FXC_PREC_FP16X3>(kh, grid, lds, st, hi, lo, nsta, nspec, sc, tiles,
+                                             trows, dst, xpitch, scale);
   if (precision == FXC_PREC_BF16X3)
     return launch_mfma_mode<FXC_PREC_BF16X3>(kh, grid, lds, st, hi, lo, nsta, nspec, sc, tiles,
                                              trows, dst, xpitch, scale);

@@ -8,7 +8,7 @@
 // writes out/xcpack/<ts>.iAAAAAA.jBBBBBB.wWWW.pPPP.{xcpack,tsv} (records
 // [SAC header][2H+1 float], parts cut at 4 GiB), out/xcpack/<ts>.done after
 // each timestamp and out/xcpack/_SUCCESS at the end, and the progress sidecar.
-// Extensions: --precision fp32|bf16x3|fp16|plain (default fp32, the
+// Extensions: --precision fp32|bf16x3|fp16|fp16x3|plain (default fp32, the
 // reference's arithmetic, or the FASTXC_XC_PRECISION environment value),
 // --block-files n (block size; the reference derives it from device memory,
 // at most 4096), --dry-run (print jobs and tasks, no GPU) and
@@ -75,7 +75,7 @@ void usage() {
           "  -G <ids>             GPU list; the first id is used (one GPU per process)\n"
           "  -M <MiB,...>         Device memory cap for the first GPU; 0 means auto\n"
           "  -J <num>             Write queue depth (default 1, 0 writes synchronously)\n"
-          "  --precision <p>      fp32 | bf16x3 | fp16 | plain (default: FASTXC_XC_PRECISION\n"
+          "  --precision <p>      fp32 | bf16x3 | fp16 | fp16x3 | plain (default: FASTXC_XC_PRECISION\n"
           "                       if set, else fp32)\n"
           "  --block-files <n>    Station rows per block (default min(rows, 4096))\n"
           "  --dry-run            Print the jobs and tasks of every timestamp and exit\n"
@@ -178,7 +178,7 @@ bool parse_args(int argc, char **argv, Args *a, int *exit_code) {
     return false;
   }
   if (a->precision != "fp32" && a->precision != "bf16x3" && a->precision != "fp16" &&
-      a->precision != "plain") {
+      a->precision != "fp16x3" && a->precision != "plain") {
     FXH_LOG_ERROR("argument_invalid", "option=\"--precision\" value=\"%s\"", a->precision.c_str());
     return false;
   }
@@ -190,6 +190,7 @@ bool parse_args(int argc, char **argv, Args *a, int *exit_code) {
 int precision_code(const std::string &p) {
   if (p == "bf16x3") return FXC_PREC_BF16X3;
   if (p == "fp16") return FXC_PREC_FP16;
+  if (p == "fp16x3") return FXC_PREC_FP16X3;
   if (p == "plain") return FXC_PREC_PLAIN;
   return FXC_PREC_FP32;
 }

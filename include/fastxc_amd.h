@@ -58,6 +58,7 @@ int fxc_fft_c2r(const float *d_in, long long in_dist, float *d_out, long long ou
 #define FXC_PREC_BF16X3 1 /* split-bf16 hi+lo MFMA, fp32 accumulation */
 #define FXC_PREC_FP16 2   /* fp16 MFMA with per-station power-of-2 scale */
 #define FXC_PREC_PLAIN 3  /* the reference's step order: per-window fp32 stack RMW */
+#define FXC_PREC_FP16X3 4 /* split-fp16 hi+lo MFMA (per-station power-of-2 scale), fp32 acc. */
 
 typedef struct fxc_xc_plan fxc_xc_plan;
 

@@ -250,7 +250,7 @@ def test_config2_full_size_every_form(gpu):
     ps = np.array([loc[a] for a, _ in pairs])
     pr = np.array([loc[b] for _, b in pairs])
     want = xc.correlate(sub, ps, pr, nfft, H, dt)
-    for precision, tol in (("fp32", 1e-5), ("bf16x3", 1e-3), ("fp16", 1e-3)):
+    for precision, tol in (("fp32", 1e-5), ("bf16x3", 1e-3), ("fp16x3", 1e-3), ("fp16", 1e-3)):
         plan = XcPlan(nspec, nstep, dt, H, precision)
         plan.set_pairs(nsta, src_all.astype(np.int32), rec_all.astype(np.int32))
         cc = plan.correlate(spec)

@@ -11,7 +11,7 @@ from oracle import xc
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp32": 1e-5, "bf16x3": 1e-3, "fp16": 1e-3, "plain": 1e-5}
+TOL = {"fp32": 1e-5, "bf16x3": 1e-3, "fp16": 1e-3, "fp16x3": 1e-3, "plain": 1e-5}
 
 
 def _spectra(rng, nstep, nsta, seg, nfft, dt, common=0.5):
@@ -54,11 +54,12 @@ def test_correlate_matches_oracle(gpu, precision, nsta, nstep, seg, nfft, H):
     assert np.all(row_err < 10 * TOL[precision])
 
 
-@pytest.mark.parametrize("precision", ["bf16x3", "fp16"])
-@pytest.mark.parametrize("nstep", [1, 12, 20, 28, 36, 44, 60, 90, 128])
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16", "fp16x3"])
+@pytest.mark.parametrize("nstep", [1, 12, 20, 28, 36, 44, 60, 90, 128, 200])
 def test_every_operand_depth(gpu, precision, nstep):
     """One window count per compiled MFMA operand depth (kh 8, 16, 24, 32, 40,
-    48, 64, 96, 128), two station tiles, against FP64."""
+    48, 64, 96, 128) and one beyond them (200 windows: seven chunks of 32
+    accumulated in registers), two station tiles, against FP64."""
     import torch
     from fastxc_amd.ops import XcPlan
 
@@ -134,9 +135,10 @@ def test_chunked_matches_unchunked(gpu):
     assert np.array_equal(ca, cb)
 
 
-def test_production_length_bf16x3(gpu):
-    """The headline form at nfft 37044: split-bf16 contraction + the fixed-plan
-    lag kernel, against FP64."""
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3", "fp16"])
+def test_production_length_mfma_forms(gpu, precision):
+    """The MFMA forms at nfft 37044: split contraction + the fixed-plan lag
+    kernel, against FP64."""
     import torch
     from fastxc_amd.ops import XcPlan
 
@@ -144,13 +146,13 @@ def test_production_length_bf16x3(gpu):
     nsta, nstep, seg, nfft, H, dt = 40, 6, 36000, 37044, 1000, 0.1
     spec = _spectra(rng, nstep, nsta, seg, nfft, dt)
     src, rec = _all_pairs(nsta)
-    plan = XcPlan(nfft // 2 + 1, nstep, dt, H, "bf16x3")
+    plan = XcPlan(nfft // 2 + 1, nstep, dt, H, precision)
     plan.set_pairs(nsta, src, rec)
     cc = plan.correlate(torch.from_numpy(spec).to(gpu)).cpu().numpy()
     want = xc.correlate(spec, src, rec, nfft, H, dt)
     err = xc.rel_l2(cc, want)
-    print(f"bf16x3 nfft 37044 rel_l2={err:.3e}")
-    assert err < TOL["bf16x3"]
+    print(f"{precision} nfft 37044 rel_l2={err:.3e}")
+    assert err < TOL[precision]
 
 
 @pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
