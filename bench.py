@@ -765,31 +765,52 @@ def batch_measure(nseg=64, seed=13, reps=3, budget_gb=235.0, max_rows=65536):
     spec = torch.randn(b.spec_elems, dtype=torch.complex64, device=dev, generator=g)
     valid = (torch.rand(b.valid_elems, device=dev, generator=g) >= 0.1).to(torch.uint8)
     out = torch.empty((b.ntasks, b.cc_size), dtype=torch.float32, device=dev)
-    if max(g0, g1) <= budget_gb:
-        tune = b.tune(spec, valid, out, reps=1)
-        tuner = {"ms": [round(x, 2) for x in tune["ms"]], "grouping": tune["grouping"]}
-    else:
-        # the tuner's candidates are the groupings that fit the memory budget
-        b.set_grouping(0 if g0 <= budget_gb else 1)
-        tuner = {"ms": None, "grouping": b.info()["grouping"],
-                 "skipped": f"grouping {1 if g0 <= budget_gb else 0} needs "
-                            f"~{max(g0, g1):.0f} GB"}
+    # the tuner times grouping x row chunk; candidates whose workspaces do not
+    # fit beside the spectra, the outputs and the operands are skipped
+    room = budget_gb - max(g0, g1) + max_rows * 18528 * 8.0 / 1e9
+    tune = b.autotune(spec, valid, out, rows=(32768, 65536, 131072), reps=1, budget_gb=room)
+    tuner = {"key": tune["key"], "grouping": tune["grouping"], "max_rows": tune["max_rows"],
+             "trials": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in t.items()}
+                        for t in tune["trials"]]}
+    ngroups = b.info()["groups"]
     sec = _time_calls(lambda: b.correlate(spec, valid, out), reps)
     b.capture(spec, valid, out)
     torch.cuda.synchronize()
     gsec = _time_calls(lambda: b.replay(), reps)
     pw = float((nsta * (nsta - 1) // 2 * nstep).sum())
+    b.correlate(spec, valid, out)
+    torch.cuda.synchronize()
+    # the PWS leg in FP32: every pair of the shared station array over the
+    # segments containing it (sub-stacks of one record)
+    from fastxc_amd.ops import batch_pws
+    b_nsta, b_base = list(b.nsta), b.task_base
+    b.close()
+    del spec, valid
+    torch.cuda.empty_cache()
+
+    class _Rows:  # the row layout batch_pws reads
+        nsta, task_base = b_nsta, b_base
+
+    batch_pws(_Rows, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    psrc, _, pout = batch_pws(_Rows, out)
+    torch.cuda.synchronize()
+    psec = time.perf_counter() - t0
+    pws_leg = {"pairs": int(len(psrc)), "records": int(out.shape[0]),
+               "ms": round(psec * 1e3, 2), "pairs_per_s": round(len(psrc) / psec, 1),
+               "note": "gather of each pair's records + linear mean (torch) + fxc_pws, FP32"}
+    del pout
     res = {"segments": int(k), "pairs": int(b.ntasks), "pair_windows": pw,
            "stations_total": int(nsta.sum()),
            "windows_min_max": [int(nstep.min()), int(nstep.max())],
            "gaps": "10 % of station-windows, excluded per pair", "nspec": nspec,
-           "lags": 2 * half + 1, "max_rows": max_rows, "tuner": tuner,
-           "groups": b.info()["groups"], "ms_eager": round(sec * 1e3, 2),
-           "ms_graph": round(gsec * 1e3, 2),
+           "lags": 2 * half + 1, "max_rows": tune["max_rows"], "tuner": tuner,
+           "groups": ngroups,
+           "ms_eager": round(sec * 1e3, 2), "ms_graph": round(gsec * 1e3, 2),
            "pair_windows_per_s": round(pw / min(sec, gsec), 1),
-           "est_gb": [round(g0, 1), round(g1, 1)]}
-    b.close()
-    del spec, valid, out
+           "est_gb": [round(g0, 1), round(g1, 1)], "pws_leg": pws_leg}
+    del out
     torch.cuda.empty_cache()
     return res
 

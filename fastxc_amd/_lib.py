@@ -68,6 +68,7 @@ _SIGNATURES = {
     "fxc_xc_batch_info": (c_int, [c_void_p, POINTER(c_longlong), POINTER(c_longlong),
                                   POINTER(c_longlong), POINTER(c_int), POINTER(c_int)]),
     "fxc_xc_batch_set_grouping": (c_int, [c_void_p, c_int]),
+    "fxc_xc_batch_set_rows": (c_int, [c_void_p, c_longlong]),
     "fxc_xc_batch_correlate": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fxc_xc_batch_tune": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
                                   POINTER(ctypes.c_double)]),

@@ -2019,6 +2019,15 @@ extern "C" int fxc_xc_batch_set_grouping(fxc_xc_batch *b, int grouping) {
   return batch_build(b);
 }
 
+// Row cap of each plan's cross-spectrum workspace (0: no cap); rebuilds the
+// groups, so a captured graph is dropped.
+extern "C" int fxc_xc_batch_set_rows(fxc_xc_batch *b, long long max_rows) {
+  FXC_CHECK_ARG(b && max_rows >= 0, "fxc_xc_batch_set_rows: invalid arguments");
+  if (max_rows == b->max_rows && !b->groups.empty()) return FXC_OK;
+  b->max_rows = max_rows;
+  return batch_build(b);
+}
+
 // Every launch of one batch run on `st`; allocates nothing once the plans'
 // workspaces exist (the first run creates them).
 static int batch_run(fxc_xc_batch *b, const float2 *spec, const unsigned char *valid, float *cc,

@@ -196,6 +196,7 @@ class XcBatch:
         self.ntasks, self.spec_elems, self.valid_elems = (info["ntasks"], info["spec_elems"],
                                                           info["valid_elems"])
         self.task_base = np.concatenate([[0], np.cumsum([n * (n - 1) // 2 for n in self.nsta])])
+        self.max_rows = int(max_rows)
         self._stream = None
 
     def close(self):
@@ -262,6 +263,74 @@ class XcBatch:
                                       _stream_ptr(spec.device), ms), "fxc_xc_batch_tune")
         return {"ms": [ms[0], ms[1]], "grouping": self.info()["grouping"]}
 
+    def set_rows(self, max_rows: int) -> None:
+        check(lib().fxc_xc_batch_set_rows(self._h, int(max_rows)), "fxc_xc_batch_set_rows")
+        self.max_rows = int(max_rows)
+
+    def tune_key(self) -> str:
+        """Class of this batch for the tuning cache: segment count, median and
+        largest station count (to 50), largest window count (to 8)."""
+        a = np.asarray(self.nsta)
+        return (f"seg{len(a)}_sta{int(np.median(a)) // 50 * 50}-{int(a.max()) // 50 * 50}"
+                f"_win{(max(self.nstep) + 7) // 8 * 8}_nspec{self.nspec}")
+
+    def autotune(self, spec: torch.Tensor, valid: torch.Tensor | None = None,
+                 out: torch.Tensor | None = None, rows=(32768, 65536, 131072), reps: int = 1,
+                 cache: str | None = None, budget_gb: float | None = None):
+        """Chooses the plan grouping (0: one plan; 1: one per operand depth)
+        and the row chunk of every plan's persistent launches by timing each
+        candidate on this data (HIP events on the current stream), skipping
+        candidates whose cross-spectrum workspaces exceed budget_gb; the
+        choice is kept in the JSON file `cache` under tune_key(), and a batch
+        of a cached class takes it without timing."""
+        import json
+        import os
+
+        key = self.tune_key()
+        table = {}
+        if cache and os.path.exists(cache):
+            with open(cache) as f:
+                table = json.load(f)
+        if key in table:
+            g, r = table[key]["grouping"], table[key]["max_rows"]
+            self.set_grouping(g)
+            self.set_rows(r)
+            return {"key": key, "grouping": g, "max_rows": r, "cached": True, "ms": None}
+        depths = {(w + 7) // 8 * 8 if w <= 48 else (64 if w <= 64 else 96 if w <= 96 else 128)
+                  for w in self.nstep}
+        xpitch = (self.nspec + 15) // 16 * 16
+        trials = []
+        cur = torch.cuda.current_stream(spec.device)
+        for g in (0, 1):
+            ngroups = 1 if g == 0 else len(depths)
+            for r in rows:
+                ws = ngroups * min(r, self.ntasks) * xpitch * 8 / 1e9
+                if budget_gb is not None and ws > budget_gb:
+                    trials.append({"grouping": g, "max_rows": r, "ms": None, "skipped_gb": ws})
+                    continue
+                self.set_grouping(g)
+                self.set_rows(r)
+                self.correlate(spec, valid, out)  # creates the workspaces
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(cur)
+                for _ in range(reps):
+                    self.correlate(spec, valid, out)
+                e1.record(cur)
+                e1.synchronize()
+                trials.append({"grouping": g, "max_rows": r, "ms": e0.elapsed_time(e1) / reps})
+                torch.cuda.empty_cache()
+        best = min((t for t in trials if t["ms"] is not None), key=lambda t: t["ms"])
+        self.set_grouping(best["grouping"])
+        self.set_rows(best["max_rows"])
+        if cache:
+            table[key] = {"grouping": best["grouping"], "max_rows": best["max_rows"],
+                          "ms": round(best["ms"], 3)}
+            os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+            with open(cache, "w") as f:
+                json.dump(table, f, indent=1, sort_keys=True)
+        return {"key": key, "grouping": best["grouping"], "max_rows": best["max_rows"],
+                "cached": False, "trials": trials}
+
     def capture(self, spec: torch.Tensor, valid: torch.Tensor | None = None,
                 out: torch.Tensor | None = None) -> torch.Tensor:
         """Records one run into a HIP graph bound to these tensors (on a side
@@ -287,6 +356,47 @@ class XcBatch:
               "fxc_xc_batch_replay")
         cur.wait_stream(self._stream)
         return self._graph_io[2]
+
+
+def batch_pws(batch, cc, power=1.0):
+    """PWS of every station pair over the batch's segments, in FP32.
+
+    Segment s's stations are taken as stations 0..nsta[s]-1 of one array, so
+    the pair (i, j) recurs in every segment with nsta > j.  Its records are
+    those segments' rows of `cc` (the batch output, segment order = time
+    order); they form sub-stacks of one record each, and the stack is the
+    reference's PWS over them (fxc_pws; native/pws/io/sourcepack_io.cu:385-478
+    with B = 1: weights 1/T, lin the mean).  Returns (src, rec, pws) with pws
+    [npairs, 2H+1] on cc's device, pairs ordered (i, j) row-major."""
+    nsta = np.asarray(batch.nsta, dtype=np.int64)
+    M = int(nsta.max())
+    gids, rows = [], []
+    for s, n in enumerate(nsta):
+        i, j = np.triu_indices(int(n), k=1)
+        gids.append(i * M - i * (i + 1) // 2 + (j - i - 1))
+        rows.append(batch.task_base[s] + np.arange(len(i)))
+    gid = np.concatenate(gids)
+    row = np.concatenate(rows)
+    order = np.argsort(gid, kind="stable")       # segment order within a pair
+    gid, row = gid[order], row[order]
+    npairs_all = M * (M - 1) // 2
+    cnt = np.bincount(gid, minlength=npairs_all)
+    present = np.flatnonzero(cnt)
+    off = np.zeros(len(present) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(cnt[present])
+    dev = cc.device
+    rows_d = torch.from_numpy(row).to(dev)
+    pre = cc.index_select(0, rows_d)               # [groups, n] FP32 records
+    pair_of_group = torch.from_numpy(np.repeat(np.arange(len(present)), cnt[present])).to(dev)
+    counts = torch.from_numpy(cnt[present].astype(np.float32)).to(dev)
+    lin = torch.zeros((len(present), cc.shape[1]), dtype=torch.float32, device=dev)
+    lin.index_add_(0, pair_of_group, pre)
+    lin /= counts[:, None]
+    w = (1.0 / counts).index_select(0, pair_of_group).contiguous()
+    out = torch.empty_like(lin)
+    pws(pre, w, off, lin, out=out, power=power)
+    ii, jj = np.triu_indices(M, k=1)
+    return ii[present], jj[present], out
 
 
 class S2sPlan:

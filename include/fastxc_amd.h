@@ -122,6 +122,9 @@ int fxc_xc_batch_info(const fxc_xc_batch *batch, long long *ntasks, long long *s
  * segment); 1: one plan per operand depth, the window count rounded up to
  * the next compiled depth (8, 16, 24, 32, 40, 48, 64, 96, 128). */
 int fxc_xc_batch_set_grouping(fxc_xc_batch *batch, int grouping);
+/* Row cap of each plan's cross-spectrum workspace (0 = no cap): the row
+ * chunk every persistent contraction and lag launch of the batch covers. */
+int fxc_xc_batch_set_rows(fxc_xc_batch *batch, long long max_rows);
 int fxc_xc_batch_correlate(fxc_xc_batch *batch, const float *d_spec, const unsigned char *d_valid,
                            float *d_cc, void *stream);
 /* Times both groupings on these buffers (reps runs each, after a warm-up)

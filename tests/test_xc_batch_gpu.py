@@ -186,3 +186,68 @@ def test_batch_production_length(gpu):
         err = xc.rel_l2(cc[b.segment_rows(s)][pick], want)
         assert err < TOL, (s, err)
     b.close()
+
+
+@pytest.mark.parametrize("power", [1.0, 2.0])
+def test_batch_pws_leg_matches_oracle(gpu, power):
+    """The batch's PWS leg (ops.batch_pws, FP32): every pair's records over
+    the segments containing it, stacked as the reference's PWS with
+    sub-stacks of one record (and the spec mode's power 2), against
+    oracle/stack.py over the FP64 oracle's correlations of each segment."""
+    import torch
+    from fastxc_amd.ops import XcBatch, batch_pws
+    from oracle import stack
+
+    rng = np.random.default_rng(61)
+    seg, nfft, H, dt = 600, 1764, 200, 0.1
+    nsta = [12, 9, 15, 6, 12, 11]
+    nstep = [6, 17, 24, 6, 40, 9]
+    specs = [_spectra(rng, w, n, seg, nfft, dt) for n, w in zip(nsta, nstep)]
+    b = XcBatch(nfft // 2 + 1, dt, H, nsta, nstep, gaps="keep")
+    spec = XcBatch.pack([torch.from_numpy(x) for x in specs], gpu)
+    cc = b.correlate(spec)
+    src, rec, out = batch_pws(b, cc, power=power)
+    out = out.cpu().numpy()
+    want_seg = [_expected(sp, np.ones(sp.shape[:2], np.uint8), "keep", nfft, H, dt) for sp in specs]
+    assert len(src) == max(nsta) * (max(nsta) - 1) // 2
+    worst = 0.0
+    for p, (i, j) in enumerate(zip(src, rec)):
+        recs = []
+        for s, n in enumerate(nsta):
+            if j < n:
+                r = i * n - i * (i + 1) // 2 + (j - i - 1)
+                recs.append(want_seg[s][r])
+        g, w, lin = stack.pws_prepare(np.stack(recs), 1)
+        err = xc.rel_l2(out[p], stack.pws(g, w, lin, power=power))
+        worst = max(worst, err)
+        assert err < 1e-4, (i, j, len(recs), err)
+    print(f"batch pws power={power} pairs={len(src)} worst rel_l2={worst:.3e}")
+    b.close()
+
+
+def test_batch_autotune_searches_and_caches(gpu, tmp_path):
+    """The tuner times grouping x row chunk on the data, keeps results
+    identical to the untuned batch, and a second batch of the same class
+    takes the cached choice without timing."""
+    import torch
+    from fastxc_amd.ops import XcBatch
+
+    rng = np.random.default_rng(71)
+    seg, nfft, H, dt = 600, 1764, 200, 0.1
+    nsta, nstep, specs, valid = _segments(rng, 6, seg, nfft, dt)
+    b = XcBatch(nfft // 2 + 1, dt, H, nsta, nstep, gaps="exclude")
+    spec = XcBatch.pack([torch.from_numpy(x) for x in specs], gpu)
+    vmask = XcBatch.pack([torch.from_numpy(v) for v in valid], gpu)
+    ref = b.correlate(spec, vmask).clone()
+    cache = str(tmp_path / "tune.json")
+    res = b.autotune(spec, vmask, rows=(64, 4096), cache=cache)
+    assert not res["cached"] and len(res["trials"]) == 4
+    assert (res["grouping"], res["max_rows"]) == (b.info()["grouping"], b.max_rows)
+    cc = b.correlate(spec, vmask)
+    assert xc.rel_l2(cc.cpu().numpy(), ref.cpu().numpy()) < 1e-6
+    _check_segments(b, cc.cpu().numpy(), specs, valid, "exclude", nfft, H, dt)
+    b2 = XcBatch(nfft // 2 + 1, dt, H, nsta, nstep, gaps="exclude")
+    res2 = b2.autotune(spec, vmask, rows=(64, 4096), cache=cache)
+    assert res2["cached"] and (res2["grouping"], res2["max_rows"]) == (res["grouping"], res["max_rows"])
+    b.close()
+    b2.close()
