@@ -346,12 +346,14 @@ def _stage(name, units, unit, sec, work, bound, peak, punit, detail):
 
 
 STAGE_CPU_SECONDS = 3.0
+STAGE_CPU_THREADS = 1
 
 
-def _stage_cpu(fn, per_call, unit, sample, seconds=None):
-    """The stage's FP64 numpy oracle timed on one host thread (BLAS and FFT
-    pools limited to one thread) over a bounded sample: calls of `fn`, each
-    worth `per_call` work units, repeated until `seconds` have passed."""
+def _stage_cpu(fn, per_call, unit, sample, seconds=None, threads=1, kind="port"):
+    """A CPU leg timed over a bounded sample: calls of `fn`, each worth
+    `per_call` work units, repeated until `seconds` have passed.  `threads`
+    is what fn uses (the C restatements take it as an argument; numpy legs
+    run with the BLAS and FFT pools limited to one thread)."""
     from threadpoolctl import threadpool_limits
 
     seconds = STAGE_CPU_SECONDS if seconds is None else seconds
@@ -365,8 +367,10 @@ def _stage_cpu(fn, per_call, unit, sample, seconds=None):
             dt = time.perf_counter() - t0
             if dt >= seconds:
                 break
-    return {"value": round(calls * per_call / dt, 3), "unit": unit, "cores": 1, "kind": "port",
-            "sample": f"{calls} x {sample}, {dt:.1f} s on one thread"}
+    return {"value": round(calls * per_call / dt, 3), "unit": unit, "cores": threads,
+            "kind": kind,
+            "sample": f"{calls} x {sample}, {dt:.1f} s on {threads} thread"
+                      f"{'s' if threads > 1 else ''}"}
 
 
 def _butter_power(nfft, delta, lo, hi):
@@ -407,16 +411,20 @@ def stage_sac2spec(dev, reps, cpu=True):
                   "nfft_out": nfft_o, "normalize": "RUN-ABS", "whiten": "after",
                   "output_finite": ok})
     if cpu:
+        from oracle import cbind
         from oracle import sac2spec as o
 
+        th = STAGE_CPU_THREADS
         oplan = o.Plan(npts=npts, delta=0.1, seg=seg, shift=seg, nstep=nstep, nfft_out=nfft_o,
                        nspec=nfft_o // 2 + 1, nfft_filter=nfft_f, freq_low=0.1, freq_high=2.0,
-                       idx=(240, 360, 7200, 9597), responses=[resp[0].astype(np.float64)],
-                       freq_lows=[0.1])
-        xs = np.random.default_rng(7).standard_normal((4, 1, seg))
+                       idx=(240, 360, 7200, 9597), responses=[resp[0]], freq_lows=[0.1])
+        nc = 2 * th
+        xs = np.random.default_rng(7).standard_normal((nc, seg)).astype(np.float32)
         res["cpu_baseline"] = _stage_cpu(
-            lambda: o.process_batch(xs, oplan, whiten=2, normalize=3), 4, "station-windows/s",
-            "4 station-windows (oracle/sac2spec.py process_batch, FP64, same plan)")
+            lambda: cbind.cpu_s2s_process(xs, [0], oplan, 2, 3, 1, nthreads=th), nc,
+            "station-windows/s",
+            f"{nc} station-windows (oracle/cpu_stages.c cpu_s2s_process, FP32 with FP64 "
+            "sums, same plan)", threads=th)
     return res
 
 
@@ -437,13 +445,15 @@ def _stack_inputs(dev, npairs, T, B, n, kind, seed):
 
 def _stack_inputs_host(npairs, T, B, n, seed):
     """Host copy of the stack-stage input shapes for the CPU leg: sub-stack
-    means [npairs, G, n], weights [G], linear stacks [npairs, n] (FP64)."""
+    rows [npairs*G, n], weights [npairs*G], group offsets [npairs+1], linear
+    stacks [npairs, n] (float32)."""
     import numpy as np
 
     G = -(-T // B)
-    pre = np.random.default_rng(seed).standard_normal((npairs, G, n))
-    w = np.array([min(B, T - i * B) / T for i in range(G)])
-    return pre, w, pre.mean(axis=1)
+    pre = np.random.default_rng(seed).standard_normal((npairs * G, n)).astype(np.float32)
+    w = np.array([min(B, T - i * B) / T for i in range(G)] * npairs, dtype=np.float32)
+    off = np.arange(0, npairs * G + 1, G, dtype=np.int64)
+    return pre, w, off, pre.reshape(npairs, G, n).mean(axis=1)
 
 
 def stage_pws(dev, reps, cpu=True):
@@ -461,12 +471,15 @@ def stage_pws(dev, reps, cpu=True):
     del pre, w, lin, out
     torch.cuda.empty_cache()
     if cpu:
-        from oracle import stack as o
+        from oracle import cbind
 
-        pre_c, w_c, lin_c = _stack_inputs_host(16, T, B, n, 11)
+        th = STAGE_CPU_THREADS
+        m = 8 * th
+        pre_c, w_c, off_c, lin_c = _stack_inputs_host(m, T, B, n, 11)
         res["cpu_baseline"] = _stage_cpu(
-            lambda: [o.pws(pre_c[i], w_c, lin_c[i]) for i in range(16)], 16, "pairs/s",
-            f"16 pairs of {G} sub-stacks x {n} lags (oracle/stack.py pws, FP64)")
+            lambda: cbind.cpu_pws(pre_c, w_c, off_c, lin_c, nthreads=th), m, "pairs/s",
+            f"{m} pairs of {G} sub-stacks x {n} lags (oracle/cpu_stages.c cpu_pws, FP32)",
+            threads=th)
     return res
 
 
@@ -488,13 +501,14 @@ def stage_tfpws(dev, reps, cpu=True):
     del pre, w, lin, out
     torch.cuda.empty_cache()
     if cpu:
-        from oracle import stack as o
+        from oracle import cbind
 
-        pre_c, w_c, lin_c = _stack_inputs_host(1, T, B, n, 13)
+        th = STAGE_CPU_THREADS
+        pre_c, w_c, off_c, lin_c = _stack_inputs_host(th, T, B, n, 13)
         res["cpu_baseline"] = _stage_cpu(
-            lambda: o.tfpws(pre_c[0], w_c, lin_c[0]), 1, "pairs/s",
-            f"1 pair of {G} sub-stacks x {n} lags, {n // 2 + 1} voices "
-            "(oracle/stack.py tfpws, FP64)")
+            lambda: cbind.cpu_tfpws(pre_c, w_c, off_c, lin_c, nthreads=th), th, "pairs/s",
+            f"{th} pairs of {G} sub-stacks x {n} lags, {n // 2 + 1} voices "
+            "(oracle/cpu_stages.c cpu_tfpws, FP32)", threads=th)
     return res
 
 
@@ -517,12 +531,16 @@ def stage_linear(dev, reps, cpu=True):
     del x, out
     torch.cuda.empty_cache()
     if cpu:
-        from oracle import stack as o
+        from oracle import cbind
 
-        recs = np.random.default_rng(17).standard_normal((64, T, n)).astype(np.float32)
+        th = STAGE_CPU_THREADS
+        m = 64 * th
+        recs = np.random.default_rng(17).standard_normal((m * T, n)).astype(np.float32)
+        off_c = np.arange(0, m * T + 1, T, dtype=np.int64)
         res["cpu_baseline"] = _stage_cpu(
-            lambda: [o.linear_stack(recs[i]) for i in range(64)], 64, "pairs/s",
-            f"64 pairs of {T} records x {n} lags (oracle/stack.py linear_stack, FP64 sum)")
+            lambda: cbind.cpu_linear_stack(recs, off_c, nthreads=th), m, "pairs/s",
+            f"{m} pairs of {T} records x {n} lags (oracle/cpu_stages.c cpu_linear_stack, "
+            "FP64 sum)", threads=th)
     return res
 
 
@@ -560,8 +578,11 @@ _STAGES = {"sac2spec": "stage_sac2spec", "pws": "stage_pws", "tfpws": "stage_tfp
            "linear_stack": "stage_linear", "rotate": "stage_rotate"}
 
 
-def stages(reps=3, only="", cpu=True):
+def stages(reps=3, only="", cpu=True, threads=1):
     import torch
+
+    global STAGE_CPU_THREADS
+    STAGE_CPU_THREADS = threads
 
     dev = torch.device("cuda:0")
     out = []
@@ -892,7 +913,7 @@ def main():
         calibrate_copy(args.calibrate_copy_gib)
     rates = [float(x) for x in args.stream_rates.split(",") if x]
     if args.stages_only:
-        print(json.dumps({"stages": stages(only=args.stage, cpu=not args.no_cpu)}), flush=True)
+        print(json.dumps({"stages": stages(only=args.stage, cpu=not args.no_cpu, threads=cpu_threads(args))}), flush=True)
         return
     if args.batch_only:
         print(json.dumps({"batch": batch_measure()}), flush=True)
@@ -956,7 +977,7 @@ def main():
     del spec, out
     torch.cuda.empty_cache()
     if not args.no_stages:
-        res["stages"] = stages(only=args.stage, cpu=not args.no_cpu)
+        res["stages"] = stages(only=args.stage, cpu=not args.no_cpu, threads=cpu_threads(args))
     if not args.no_stream:
         res["stream"] = stream_all(rates, args.stream_seconds)
     if not args.no_sourcepack:

@@ -1,6 +1,8 @@
 """ctypes bindings of the oracle's C pieces -- test and baseline infrastructure only.
 
   libcpu_xc.so         oracle/cpu_xc.c (multi-threaded FP32 xc restatement)
+  libcpu_stages.so     oracle/cpu_stages.c (multi-threaded FP32 sac2spec, PWS, TF-PWS,
+                       linear stack)
   _ref/libref_s2s.so   the reference's fft_length.c and filter_response.c
   _ref/libref_xc.so    the reference's io/path_table.c
 The _ref libraries are compiled from /root/reference by `make -C oracle ref`
@@ -15,12 +17,17 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CPU_XC = os.path.join(HERE, "libcpu_xc.so")
+CPU_STAGES = os.path.join(HERE, "libcpu_stages.so")
 REF_S2S = os.path.join(HERE, "_ref", "libref_s2s.so")
 REF_XC = os.path.join(HERE, "_ref", "libref_xc.so")
 
 
 def have_cpu_xc() -> bool:
     return os.path.exists(CPU_XC)
+
+
+def have_cpu_stages() -> bool:
+    return os.path.exists(CPU_STAGES)
 
 
 def have_ref() -> bool:
@@ -44,6 +51,76 @@ def cpu_xc_correlate(spec: np.ndarray, src, rec, half_cc: int, dt: float,
             half_cc, float(dt), out.ctypes.data, int(nthreads))
     if rc != 0:
         raise RuntimeError("cpu_xc_correlate failed")
+    return out
+
+
+def _vp(a):
+    return a.ctypes.data
+
+
+def cpu_s2s_process(traces: np.ndarray, steps, plan, whiten: int = 2, normalize: int = 3,
+                    nch: int = 1, nthreads: int = 1) -> np.ndarray:
+    """traces [files, npts] (a group's channels adjacent), steps: window indices,
+    plan: oracle.sac2spec.Plan -> spectra [nsteps, files, nspec] complex64."""
+    lib = ctypes.CDLL(CPU_STAGES)
+    tr = np.ascontiguousarray(traces, dtype=np.float32)
+    st = np.ascontiguousarray(steps, dtype=np.int32)
+    idx = np.ascontiguousarray(plan.idx, dtype=np.int32)
+    resp = np.ascontiguousarray(plan.responses[0][: plan.nfft_filter // 2 + 1], dtype=np.float32)
+    out = np.zeros((len(st), tr.shape[0], plan.nspec), dtype=np.complex64)
+    fn = lib.cpu_s2s_process
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                   ctypes.c_float, ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    rc = fn(_vp(tr), tr.shape[0], tr.shape[1], _vp(st), len(st), plan.seg, plan.shift,
+            plan.nfft_filter, plan.nfft_out, int(nch), float(plan.delta), float(plan.freq_low),
+            _vp(idx), int(whiten), int(normalize), _vp(resp), _vp(out), int(nthreads))
+    if rc != 0:
+        raise RuntimeError("cpu_s2s_process failed (unsupported mode or length)")
+    return out
+
+
+def _stack_call(name, pre, w, off, lin, nthreads):
+    lib = ctypes.CDLL(CPU_STAGES)
+    pre = np.ascontiguousarray(pre, dtype=np.float32)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    lin = np.ascontiguousarray(lin, dtype=np.float32)
+    npairs, n = lin.shape
+    out = np.zeros((npairs, n), dtype=np.float32)
+    fn = getattr(lib, name)
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+    if fn(_vp(pre), _vp(w), _vp(off), _vp(lin), npairs, n, _vp(out), int(nthreads)) != 0:
+        raise RuntimeError(f"{name} failed (unsupported length {n})")
+    return out
+
+
+def cpu_pws(pre, w, off, lin, nthreads: int = 1) -> np.ndarray:
+    """pre [G, n] group means, w [G], off [npairs+1] group offsets, lin [npairs, n]."""
+    return _stack_call("cpu_pws", pre, w, off, lin, nthreads)
+
+
+def cpu_tfpws(pre, w, off, lin, nthreads: int = 1) -> np.ndarray:
+    """pre [G, n] group sums, w [G], off [npairs+1] group offsets, lin [npairs, n]."""
+    return _stack_call("cpu_tfpws", pre, w, off, lin, nthreads)
+
+
+def cpu_linear_stack(rec, off, nthreads: int = 1) -> np.ndarray:
+    """rec [R, n] rows of every pair, off [npairs+1] row offsets -> means [npairs, n]."""
+    lib = ctypes.CDLL(CPU_STAGES)
+    rec = np.ascontiguousarray(rec, dtype=np.float32)
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    npairs, n = len(off) - 1, rec.shape[1]
+    out = np.zeros((npairs, n), dtype=np.float32)
+    fn = lib.cpu_linear_stack
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                   ctypes.c_void_p, ctypes.c_int]
+    fn(_vp(rec), _vp(off), npairs, n, _vp(out), int(nthreads))
     return out
 
 
