@@ -132,14 +132,90 @@ __device__ __forceinline__ void fixed_dif_pass(float2 *buf, const float2 *__rest
   __syncthreads();
 }
 
-// Passes p..P-2 with LDS twiddles (the last pass is left to
-// fixed_last_pass_bin).
-template <class FP, int p, int kThreads>
+// Passes p..pEnd-1 with LDS twiddles (by default up to P-2; the last pass
+// is left to fixed_last_pass_bin).
+template <class FP, int p, int kThreads, int pEnd = FP::kPasses - 1>
 __device__ __forceinline__ void fixed_dif_passes_lds(float2 *buf, const float2 *twl, float sgn) {
-  if constexpr (p < FP::kPasses - 1) {
+  if constexpr (p < pEnd) {
     fixed_dif_pass<FP, p, kThreads, true>(buf, nullptr, twl, sgn);
-    fixed_dif_passes_lds<FP, p + 1, kThreads>(buf, twl, sgn);
+    fixed_dif_passes_lds<FP, p + 1, kThreads, pEnd>(buf, twl, sgn);
   }
+}
+
+// Pass P-2 when only the outputs near both ends of the transform are read
+// afterwards.  With G = M / (R_{P-2} R_{P-1}) and L = M / R_{P-1}, final bin
+// k = m + L q_last and m = lower + G q (lower < G, q the output of this
+// pass's butterfly).  Bins k <= n_hi with n_hi < 2G and k >= M - n_neg with
+// n_neg <= 2G have m < 2G or m >= L - 2G, so q is 0, 1, R-2 or R-1: those
+// four outputs of each odd-R butterfly are formed, twiddled and written, the
+// others are neither computed nor stored.  Twiddles: w = w_S^j from the LDS
+// base table, w^((R-1)/2) by chaining, w^(R-1) its square, w^(R-2) =
+// w^(R-1) conj(w) (|w| = 1).
+template <class FP>
+constexpr int fixed_edge_group() {
+  return FP::kM / (FP::kRadix[FP::kPasses - 2] * FP::kRadix[FP::kPasses - 1]);
+}
+template <class FP, int kThreads>
+__device__ __forceinline__ void fixed_dif_pass_edge(float2 *buf, const float2 *twl, float sgn) {
+  constexpr int p = FP::kPasses - 2;
+  constexpr int R = FP::kRadix[p];
+  static_assert((R & 1) == 1 && R >= 5, "edge pass: odd radix of at least 5");
+  constexpr int S = FP::span(p);
+  constexpr int s = FP::stride(p);
+  constexpr int nbt = FP::kM / R;
+  constexpr int H = (R - 1) / 2;
+  constexpr RootTable<R> tb{};
+  const float2 *tl = twl + fixed_lds_tw_offset<FP>(p);
+  constexpr bool kBlockMajor = s < 32;
+  constexpr int nblk = FP::kM / S;
+  for (int b = threadIdx.x; b < nbt; b += kThreads) {
+    const int blk = kBlockMajor ? b % nblk : b / s;
+    const int j = kBlockMajor ? b / nblk : b - blk * s;
+    float2 *base = buf + blk * S + j;
+    float2 v[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) v[m] = base[m * s];
+    float2 t[H + 1], u[H + 1];
+    float2 b0 = v[0];
+#pragma unroll
+    for (int m = 1; m <= H; ++m) {
+      t[m] = cadd(v[m], v[R - m]);
+      u[m] = csub(v[m], v[R - m]);
+      b0 = cadd(b0, t[m]);
+    }
+    float2 out[3];  // b_1, b_{R-2}, b_{R-1}
+#pragma unroll
+    for (int q = 1; q <= 2; ++q) {
+      float2 re = v[0];
+      float2 im = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int m = 1; m <= H; ++m) {
+        const int idx = (m * q) % R;
+        re.x += t[m].x * tb.c[idx];
+        re.y += t[m].y * tb.c[idx];
+        im.x += u[m].x * tb.s[idx];
+        im.y += u[m].y * tb.s[idx];
+      }
+      const float2 iim = cmul_isgn(im, sgn);
+      if (q == 1) {
+        out[0] = cadd(re, iim);
+        out[2] = csub(re, iim);
+      } else {
+        out[1] = csub(re, iim);
+      }
+    }
+    const float2 w = tl[j];
+    float2 wh = w;
+#pragma unroll
+    for (int q = 1; q < H; ++q) wh = cmul(wh, w);
+    const float2 wr1 = cmul(wh, wh);
+    const float2 wr2 = cmul(wr1, cconj(w));
+    base[0] = b0;
+    base[s] = cmul(out[0], w);
+    base[(R - 2) * s] = cmul(out[1], wr2);
+    base[(R - 1) * s] = cmul(out[2], wr1);
+  }
+  __syncthreads();
 }
 
 // Bin k after fixed_fft_head: the direct R-point DFT (R = last radix) of the
@@ -148,21 +224,33 @@ __device__ __forceinline__ void fixed_dif_passes_lds(float2 *buf, const float2 *
 // roots[u] = exp(sgn*2*pi*i*u/R).  kDirect (roots in LDS): every power is
 // read as roots[(m t) mod R]; otherwise the powers are chained and re-read
 // from roots every 8 terms.
-template <class FP, bool kDirect = false>
-__device__ __forceinline__ float2 fixed_last_pass_bin(const float2 *buf,
-                                                      const float2 *__restrict__ roots, int k) {
+// The block position (base) and the output index t of bin k in the last pass.
+template <class FP>
+__device__ __forceinline__ void fixed_last_pass_index(int k, int &t, int &base) {
   constexpr int P = FP::kPasses;
   constexpr int R = FP::kRadix[P - 1];
   constexpr int g = FP::kM / R;
-  const int t = k / g;
+  t = k / g;
   int lower = k - t * g;
-  int base = 0;
+  base = 0;
 #pragma unroll
   for (int p = 0; p < P - 1; ++p) {
     const int q = lower / FP::kRadix[p];
     base += (lower - q * FP::kRadix[p]) * FP::stride(p);
     lower = q;
   }
+}
+
+template <class FP, bool kDirect = false>
+__device__ __forceinline__ float2 fixed_last_pass_bin(const float2 *buf,
+                                                      const float2 *__restrict__ roots, int k);
+
+// fixed_last_pass_bin with the block position and output index given.
+template <class FP, bool kDirect = false>
+__device__ __forceinline__ float2 fixed_last_pass_at(const float2 *buf,
+                                                     const float2 *__restrict__ roots, int t,
+                                                     int base) {
+  constexpr int R = FP::kRadix[FP::kPasses - 1];
   float2 z = buf[base];
   if constexpr (kDirect) {
     int e = 0;
@@ -190,6 +278,14 @@ __device__ __forceinline__ float2 fixed_last_pass_bin(const float2 *buf,
     }
   }
   return z;
+}
+
+template <class FP, bool kDirect>
+__device__ __forceinline__ float2 fixed_last_pass_bin(const float2 *buf,
+                                                      const float2 *__restrict__ roots, int k) {
+  int t, base;
+  fixed_last_pass_index<FP>(k, t, base);
+  return fixed_last_pass_at<FP, kDirect>(buf, roots, t, base);
 }
 
 }  // namespace fxc

@@ -105,6 +105,8 @@ __global__ void station_scale_kernel(const unsigned int *__restrict__ amax_bits,
 // bins and the writes along the stations.
 constexpr int kSplitSta = 64;
 constexpr int kSplitBins = 16;
+constexpr int kSplitThreads = 256;
+constexpr int kSplitLoads = 8 * kSplitSta * kSplitBins / kSplitThreads;  // per thread and block
 
 // One workgroup's part: source stations [s0, s0 + 64) of spectra
 // [nstep][nsta][nspec], bins [k0, k0 + 16).  Source station s lands at operand
@@ -121,22 +123,31 @@ __device__ __forceinline__ void split_part(const float2 *__restrict__ spec, int 
   const int tid = threadIdx.x;
   const int ngrp = kh / 4;  // K groups of 8 values: kh/8 real + kh/8 imaginary
   for (int wb = 0; wb < kh; wb += 8) {
-    __syncthreads();
-    // load windows wb..wb+7: 16 lanes read one station's 16 bins (128 bytes)
-    for (int i = tid; i < 8 * kSplitSta * kSplitBins; i += blockDim.x) {
+    // load windows wb..wb+7: 16 lanes read one station's 16 bins (128 bytes);
+    // every load of the block is issued before the first LDS store, so their
+    // latencies overlap
+    float2 z[kSplitLoads];
+#pragma unroll
+    for (int j = 0; j < kSplitLoads; ++j) {
+      const int i = tid + j * kSplitThreads;
       const int kk = i % kSplitBins;
       const int ss = (i / kSplitBins) % kSplitSta;
       const int ww = i / (kSplitBins * kSplitSta);
       const int w = wb + ww, sidx = s0 + ss, k = k0 + kk;
-      float2 z = make_float2(0.f, 0.f);
+      z[j] = make_float2(0.f, 0.f);
       if (w < nstep && sidx < nsta && k < nspec && (!valid || valid[(size_t)w * nsta + sidx]))
-        z = spec[((size_t)w * nsta + sidx) * nspec + k];
-      tile[ww][ss][kk] = z;
+        z[j] = spec[((size_t)w * nsta + sidx) * nspec + k];
+    }
+    __syncthreads();  // the previous block's conversion is done with the tile
+#pragma unroll
+    for (int j = 0; j < kSplitLoads; ++j) {
+      const int i = tid + j * kSplitThreads;
+      tile[i / (kSplitBins * kSplitSta)][(i / kSplitBins) % kSplitSta][i % kSplitBins] = z[j];
     }
     __syncthreads();
     // convert: thread = (station, bin); 64 consecutive threads write 64
     // consecutive stations' 16-byte groups
-    for (int i = tid; i < kSplitSta * kSplitBins; i += blockDim.x) {
+    for (int i = tid; i < kSplitSta * kSplitBins; i += kSplitThreads) {
       const int ss = i % kSplitSta;
       const int kk = i / kSplitSta;
       const int sidx = s0 + ss, k = k0 + kk;
@@ -182,7 +193,7 @@ __device__ __forceinline__ void split_part(const float2 *__restrict__ spec, int 
 }
 
 template <int kMode>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSplitThreads)
 split_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec, int kh,
              const float *__restrict__ inv_scale, unsigned short *__restrict__ hi,
              unsigned short *__restrict__ lo) {
@@ -203,7 +214,7 @@ struct SplitSeg {
 };
 
 template <int kMode>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSplitThreads)
 split_segments_kernel(const float2 *__restrict__ spec, const unsigned char *__restrict__ valid,
                       const SplitSeg *__restrict__ segs, int nsta_dst, int nspec, int kh,
                       unsigned short *__restrict__ hi, unsigned short *__restrict__ lo) {
@@ -230,6 +241,14 @@ struct TileDesc {
 // spectra that the contraction re-reads from L2.
 __device__ __forceinline__ void store_stream(float2 *dst, float2 v) {
   __builtin_nontemporal_store(__builtin_bit_cast(unsigned long long, v), (unsigned long long *)dst);
+}
+// Two adjacent values (16 bytes; dst 16-byte aligned).
+__device__ __forceinline__ void store_stream2(float2 *dst, float2 a, float2 b) {
+  typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+  u64x2 v;
+  v.x = __builtin_bit_cast(unsigned long long, a);
+  v.y = __builtin_bit_cast(unsigned long long, b);
+  __builtin_nontemporal_store(v, (u64x2 *)dst);
 }
 
 // Work item -> (tile, bin chunk).  Workgroups are dispatched round-robin over
@@ -639,7 +658,7 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
                      const TileDesc *__restrict__ tiles, const int *__restrict__ tile_rows,
                      float2 *__restrict__ xspec, long long xpitch, int ntiles, float scale,
                      int kchunks) {
-  extern __shared__ float2 stage[];  // [kTilePairs][kBinChunk], swizzled
+  extern __shared__ __align__(16) float2 stage[];  // [kTilePairs][kBinChunk], swizzled
   __shared__ int rows[kTilePairs];
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
@@ -713,13 +732,24 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
       }
     }
     __syncthreads();
-    // 16 consecutive threads write one pair's 16 bins (128 bytes).
-    const int b = tid & 15;
+    // 8 consecutive threads write one pair's 16 bins (128 bytes), two
+    // adjacent bins (16 bytes) each.  The swizzle keeps bins 2c and 2c+1 in
+    // one 16-byte slot, (2c) ^ (p & 14), in swapped order when p is odd.
+    const int b = (tid & 7) * 2;
     const int k = ia.k0 + b;
     if (k < nspec) {
-      for (int p = tid >> 4; p < kTilePairs; p += blockDim.x >> 4) {
+      for (int p = tid >> 3; p < kTilePairs; p += blockDim.x >> 3) {
         const int row = rows[p];
-        if (row >= 0) store_stream(xspec + (size_t)row * xpitch + k, stage[stage_index(p, b)]);
+        if (row < 0) continue;
+        const float4 v = *(const float4 *)(stage + p * kBinChunk + (b ^ (p & 14)));
+        const bool sw = (p & 1) != 0;
+        const float2 v0 = sw ? make_float2(v.z, v.w) : make_float2(v.x, v.y);
+        const float2 v1 = sw ? make_float2(v.x, v.y) : make_float2(v.z, v.w);
+        float2 *dst = xspec + (size_t)row * xpitch + k;
+        if (k + 1 < nspec)
+          store_stream2(dst, v0, v1);
+        else
+          store_stream(dst, v0);
       }
     }
     item = nitem;
@@ -973,12 +1003,17 @@ __device__ __forceinline__ void lag_r0_butterfly(float2 *lds, float2 (&u)[FP::kR
   for (int q = 0; q < L::R0; ++q) lds[j + L::S0 * q] = u[q];
 }
 
+// [Z: M][base twiddles of passes 1..P-2][roots of the last radix, padded to
+// an even count][first lag bin of each thread: kLagThreads ints]
 template <class FP>
 constexpr int lag_r0_lds_count() {
-  return FP::kM + fixed_lds_tw_count<FP>() + FP::kRadix[FP::kPasses - 1];
+  return FP::kM + fixed_lds_tw_count<FP>() + (FP::kRadix[FP::kPasses - 1] + 1) / 2 * 2 +
+         kLagThreads / 2;
 }
 
-template <class FP>
+// kEdge: pass P-2 forms only the outputs the kept lags read
+// (fixed_dif_pass_edge; the host checks that half_cc allows it).
+template <class FP, bool kEdge>
 __global__ void __launch_bounds__(kLagThreads)
 lag_kernel_fixed_r0(const float2 *__restrict__ xspec, long long xpitch, long long nrows,
                     const int *__restrict__ row_out, float *__restrict__ cc, int half_cc,
@@ -988,15 +1023,27 @@ lag_kernel_fixed_r0(const float2 *__restrict__ xspec, long long xpitch, long lon
   using L = LagPass0<FP>;
   constexpr int R0 = L::R0, S0 = L::S0;
   constexpr int RL = FP::kRadix[FP::kPasses - 1];
-  extern __shared__ float2 lds[];  // [Z: M][base twiddles of passes 1..P-2][roots of RL]
+  extern __shared__ float2 lds[];  // layout: lag_r0_lds_count
   float2 *twl = lds + L::M;
   float2 *rts = twl + fixed_lds_tw_count<FP>();
+  int *firsts = (int *)(rts + (RL + 1) / 2 * 2);
   for (int i = threadIdx.x; i < fixed_lds_tw_count<FP>(); i += kLagThreads) twl[i] = twl_g[i];
   if (threadIdx.x < RL) rts[threadIdx.x] = roots[threadIdx.x];
   constexpr RootTable<2 * R0> cr{};  // c_m = exp(2 pi i m / (2 R0))
   constexpr RootTable<R0> er{};
   const float2 e1 = make_float2(er.c[1], er.s[1]);  // exp(2 pi i / R0)
   float2 a[L::kPer][R0], b[L::kPer][R0];
+  // lag bins: n in [n_lo, n_hi]; a thread's first bin has the same
+  // last-pass block and output index in every row, kept in LDS as
+  // base * 16 + t (a register would spill at this kernel's 128)
+  static_assert(RL <= 16, "output index fits 4 bits");
+  const int n_lo = -((half_cc + 1) / 2), n_hi = half_cc / 2;
+  if (threadIdx.x <= n_hi - n_lo) {
+    const int n = n_lo + (int)threadIdx.x;
+    int t, base;
+    fixed_last_pass_index<FP>(n < 0 ? n + L::M : n, t, base);
+    firsts[threadIdx.x] = base * 16 + t;
+  }
   long long row = blockIdx.x;
   if (row < nrows) lag_r0_load<FP>(a, b, xspec + row * xpitch);
   for (; row < nrows; row += gridDim.x) {
@@ -1035,14 +1082,24 @@ lag_kernel_fixed_r0(const float2 *__restrict__ xspec, long long xpitch, long lon
     __syncthreads();
     const long long nxt = row + gridDim.x;
     if (nxt < nrows) lag_r0_load<FP>(a, b, xspec + nxt * xpitch);
-    fixed_dif_passes_lds<FP, 1, kLagThreads>(lds, twl, 1.f);
+    if constexpr (kEdge) {
+      fixed_dif_passes_lds<FP, 1, kLagThreads, FP::kPasses - 2>(lds, twl, 1.f);
+      fixed_dif_pass_edge<FP, kLagThreads>(lds, twl, 1.f);
+    } else {
+      fixed_dif_passes_lds<FP, 1, kLagThreads>(lds, twl, 1.f);
+    }
     const int out_row = row_out ? row_out[row] : (int)row;
     float *y = cc + (size_t)out_row * (2 * half_cc + 1);
     const double wr = task_scale ? weight * (double)task_scale[out_row] : weight;
-    const int n_lo = -((half_cc + 1) / 2), n_hi = half_cc / 2;
     for (int i = threadIdx.x; i <= n_hi - n_lo; i += kLagThreads) {
       const int n = n_lo + i;
-      const float2 z = fixed_last_pass_bin<FP, true>(lds, rts, n < 0 ? n + L::M : n);
+      float2 z;
+      if (i == (int)threadIdx.x) {
+        const int first = firsts[threadIdx.x];
+        z = fixed_last_pass_at<FP, true>(lds, rts, first & 15, first >> 4);
+      } else {
+        z = fixed_last_pass_bin<FP, true>(lds, rts, n < 0 ? n + L::M : n);
+      }
       const int m0 = 2 * n + half_cc;
       if (m0 >= 0) y[m0] = (float)((double)z.x * wr);
       if (m0 + 1 <= 2 * half_cc) y[m0 + 1] = (float)((double)z.y * wr);
@@ -1519,13 +1576,13 @@ static int xc_prepare_spectra(fxc_xc_plan *p, const float2 *d_spec, hipStream_t 
     hipLaunchKernelGGL(station_scale_kernel, dim3((unsigned)ceil_div(p->nsta, 256)), dim3(256), 0,
                        st, p->d_amax, p->nsta, p->d_inv_scale, p->d_scale);
     if (p->precision == FXC_PREC_FP16)
-      hipLaunchKernelGGL(split_kernel<FXC_PREC_FP16>, grid, dim3(256), 0, st, d_spec, p->nstep,
+      hipLaunchKernelGGL(split_kernel<FXC_PREC_FP16>, grid, dim3(kSplitThreads), 0, st, d_spec, p->nstep,
                          p->nsta, p->nspec, p->kh, p->d_inv_scale, p->d_hi, p->d_lo);
     else
-      hipLaunchKernelGGL(split_kernel<FXC_PREC_FP16X3>, grid, dim3(256), 0, st, d_spec, p->nstep,
+      hipLaunchKernelGGL(split_kernel<FXC_PREC_FP16X3>, grid, dim3(kSplitThreads), 0, st, d_spec, p->nstep,
                          p->nsta, p->nspec, p->kh, p->d_inv_scale, p->d_hi, p->d_lo);
   } else {
-    hipLaunchKernelGGL(split_kernel<FXC_PREC_BF16X3>, grid, dim3(256), 0, st, d_spec, p->nstep,
+    hipLaunchKernelGGL(split_kernel<FXC_PREC_BF16X3>, grid, dim3(kSplitThreads), 0, st, d_spec, p->nstep,
                        p->nsta, p->nspec, p->kh, (const float *)nullptr, p->d_hi, p->d_lo);
   }
   FXC_HIP(hipGetLastError());
@@ -1624,6 +1681,16 @@ static int xc_contract_chunk(fxc_xc_plan *p, const float2 *d_spec, long long c, 
   return FXC_OK;
 }
 
+// FXC_XC_LAG_R0=0 selects the production-length lag kernel without the
+// fold in pass 0 (lag_kernel_fixed).
+static bool xc_lag_r0_on() {
+  static const bool off = [] {
+    const char *e = getenv("FXC_XC_LAG_R0");
+    return e && atoi(e) == 0;
+  }();
+  return !off;
+}
+
 // task_scale (per task, may be null) multiplies the output scale of each row.
 static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long long nrows,
                         const int *row_out, float *d_cc, hipStream_t st,
@@ -1667,15 +1734,15 @@ static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long
   // reference weight: 1.0 / (width * dt) with width converted to float
   const double weight = 1.0 / (double)((float)p->nfft * p->dt);
   FXC_TRY(xc_mark(p, kTimeLag, true, st));
-  static const bool r0_off = [] {
-    const char *e = getenv("FXC_XC_LAG_R0");
-    return e && atoi(e) == 0;
-  }();
-  if (p->d_fix_tw && !r0_off) {
+  if (p->d_fix_tw && xc_lag_r0_on()) {
     using FP = LagPlan18522;
     static_assert(lag_r0_lds_count<FP>() <= kLdsComplexCapacity, "lag LDS fits one CU");
     const size_t flds = (size_t)lag_r0_lds_count<FP>() * sizeof(float2);
-    auto fk = lag_kernel_fixed_r0<FP>;
+    // the edge form of pass P-2 holds when both lag ends stay within 2G bins
+    constexpr int G2 = 2 * fixed_edge_group<FP>();
+    const int n_neg = (p->half_cc + 1) / 2, n_pos = p->half_cc / 2;
+    const bool edge = n_pos < G2 && n_neg <= G2;
+    auto fk = edge ? lag_kernel_fixed_r0<FP, true> : lag_kernel_fixed_r0<FP, false>;
     FXC_HIP(hipFuncSetAttribute((const void *)fk, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)flds));
     hipLaunchKernelGGL(fk, dim3((unsigned)grid), dim3(kLagThreads), flds, st, xs, xpitch, nrows,
@@ -2047,7 +2114,7 @@ static int batch_run(fxc_xc_batch *b, const float2 *spec, const unsigned char *v
     FXC_TRY(xc_ensure_workspace(p, (size_t)maxrows));
     const dim3 grid((unsigned)ceil_div(p->nspec, kSplitBins),
                     (unsigned)ceil_div(g.max_nsta, kSplitSta), (unsigned)g.segs.size());
-    hipLaunchKernelGGL(split_segments_kernel<FXC_PREC_BF16X3>, grid, dim3(256), 0, st, spec,
+    hipLaunchKernelGGL(split_segments_kernel<FXC_PREC_BF16X3>, grid, dim3(kSplitThreads), 0, st, spec,
                        valid, g.d_segs, g.nsta_dst, p->nspec, p->kh, p->d_hi, p->d_lo);
     FXC_HIP(hipGetLastError());
     const long long nc = (long long)p->chunk_tile_begin.size() - 1;
