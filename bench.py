@@ -80,6 +80,8 @@ def parse():
     ap.add_argument("--stream-only", action="store_true", help="measure the streaming loop only")
     ap.add_argument("--no-sourcepack", action="store_true",
                     help="skip the SourcePack indexer measurement")
+    ap.add_argument("--no-band", action="store_true",
+                    help="skip the band-limited spec-mode lines (fxc_xc_set_band)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched uneven-segment measurement")
     ap.add_argument("--batch-only", action="store_true",
@@ -134,12 +136,14 @@ def fp64_rows(spec, pairs, half_cc, dt=0.1):
     return out
 
 
-def run_mode(precision, spec, src, rec, args, out, check_rows):
+def run_mode(precision, spec, src, rec, args, out, check_rows, band=None):
     import torch
     from fastxc_amd.ops import XcPlan
 
     plan = XcPlan(args.nspec, args.nstep, 0.1, args.half_cc, precision, chunk_rows=args.chunk_rows)
     plan.set_pairs(args.nsta, src, rec)
+    if band is not None:
+        plan.set_band(*band)
     for _ in range(args.warmup):
         plan.correlate(spec, out)
     torch.cuda.synchronize()
@@ -157,7 +161,7 @@ def run_mode(precision, spec, src, rec, args, out, check_rows):
     rows = out[torch.tensor(check_rows, device=out.device)].double().cpu().numpy()
     ms = wall * 1e3 / args.steps
     return {
-        "precision": precision,
+        "precision": precision if band is None else f"{precision} band {band[0]}-{band[1]}",
         "ms_per_step": ms,
         "pair_windows_per_s": npairs * args.nstep / (ms / 1e3),
         "contract_ms": tm["contract_ms"] / args.steps,
@@ -168,6 +172,55 @@ def run_mode(precision, spec, src, rec, args, out, check_rows):
         "chunks": info["nchunks"],
         "_rows": rows,
     }
+
+
+def band_measure(args, src, rec, out, check_pairs, check_rows):
+    """Spec mode fxc_xc_set_band beside parity, on the spectra sac2spec makes
+    of seeded noise (1000 stations x 1 day at 0.1 s, a common component plus
+    independent noise, RUN-ABS, whitening after on 0.1-2 Hz: the whitening
+    taper spans bins [240, 9597) of the 36000-sample window, 0.067-2.67 Hz).
+    Bands: the taper's support on the output grid (nfft 37044) with 16 bins of
+    margin, and the flat 0.1-2 Hz band (bins 370-7409, ~7040 retained).  Each
+    line carries its FP64 error over the sampled pairs, the FP64 reference
+    taking every bin."""
+    import numpy as np
+    import torch
+    from fastxc_amd.ops import S2sPlan
+
+    nsta, npts, seg, nfft_f, nfft_o, nstep = args.nsta, 864000, 36000, 36288, 37044, args.nstep
+    if (args.nspec, nstep) != (nfft_o // 2 + 1, 24):
+        return {"skipped": "band spec mode is measured at nspec 18523 x 24 windows"}
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(99)
+    x = torch.randn((nsta, npts), dtype=torch.float32, device=dev, generator=g)
+    x += torch.randn((1, npts), dtype=torch.float32, device=dev, generator=g)
+    resp = _butter_power(nfft_f, 0.1, 0.1, 2.0)[None]
+    idx = [240, 360, 7200, 9597]
+    plan = S2sPlan(npts, seg, seg, nfft_f, nfft_o, 1, 0.1, 0.1, idx, resp, [0.1], whiten=2,
+                   normalize=3)
+    spec = torch.empty((nstep, nsta, nfft_o // 2 + 1), dtype=torch.complex64, device=dev)
+    plan.process(x, np.arange(nstep, dtype=np.int32), spec)
+    plan.close()
+    del x
+    torch.cuda.empty_cache()
+    scale = nfft_o / seg
+    taper = (int(idx[0] * scale) - 16, min(int(np.ceil(idx[3] * scale)) + 16, args.nspec - 1))
+    flat = (int(round(0.1 * nfft_o * 0.1)), int(round(2.0 * nfft_o * 0.1)))
+    lines = [run_mode("bf16x3", spec, src, rec, args, out, check_rows),
+             run_mode("bf16x3", spec, src, rec, args, out, check_rows, band=taper),
+             run_mode("bf16x3", spec, src, rec, args, out, check_rows, band=flat)]
+    want = fp64_rows(spec, check_pairs, args.half_cc)
+    for m, b in zip(lines, (None, taper, flat)):
+        rows = m.pop("_rows")
+        d = rows - want
+        m["rel_l2_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(want))
+        m["max_abs_vs_fp64"] = float(np.abs(d).max())
+        m["retained_bins"] = args.nspec if b is None else b[1] - b[0] + 1
+    del spec
+    torch.cuda.empty_cache()
+    return {"spectra": "sac2spec of seeded noise (RUN-ABS, whitening after, 0.1-2 Hz)",
+            "lines": lines}
 
 
 def algorithmic_bytes(args, npairs):
@@ -974,7 +1027,12 @@ def main():
     for m in modes[1:]:
         res["ratio_vs_" + m["precision"]] = round(head["pair_windows_per_s"] /
                                                    m["pair_windows_per_s"], 3)
-    del spec, out
+    del spec
+    torch.cuda.empty_cache()
+    if not args.no_band:
+        print("# band spec mode", file=sys.stderr, flush=True)
+        res["band_spec_mode"] = band_measure(args, src, rec, out, check_pairs, check_rows)
+    del out
     torch.cuda.empty_cache()
     if not args.no_stages:
         res["stages"] = stages(only=args.stage, cpu=not args.no_cpu, threads=cpu_threads(args))

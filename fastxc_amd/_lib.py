@@ -75,6 +75,7 @@ _SIGNATURES = {
     "fxc_xc_batch_capture": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fxc_xc_batch_replay": (c_int, [c_void_p, c_void_p]),
     "fxc_xc_set_timing": (c_int, [c_void_p, c_int]),
+    "fxc_xc_set_band": (c_int, [c_void_p, c_int, c_int]),
     "fxc_pws": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "fxc_tfpws": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_float,
                           c_float, c_float, c_float, c_void_p, c_void_p]),

@@ -196,9 +196,9 @@ template <int kMode>
 __global__ void __launch_bounds__(kSplitThreads)
 split_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec, int kh,
              const float *__restrict__ inv_scale, unsigned short *__restrict__ hi,
-             unsigned short *__restrict__ lo) {
+             unsigned short *__restrict__ lo, int bin0) {
   split_part<kMode>(spec, nstep, nsta, nspec, kh, inv_scale, nullptr, 0, nsta,
-                    blockIdx.y * kSplitSta, blockIdx.x * kSplitBins, hi, lo);
+                    blockIdx.y * kSplitSta, bin0 + blockIdx.x * kSplitBins, hi, lo);
 }
 
 // Uneven segments in one launch: segment z = blockIdx.z holds spectra
@@ -275,11 +275,13 @@ __device__ __forceinline__ bool tile_chunk(int ntiles, int nchunks, int &tile, i
 __global__ void __launch_bounds__(256)
 contract_fp32_kernel(const float2 *__restrict__ spec, int nstep, int nsta, int nspec,
                      const TileDesc *__restrict__ tiles, const int *__restrict__ tile_rows,
-                     float2 *__restrict__ xspec, long long xpitch, int ntiles, float scale) {
+                     float2 *__restrict__ xspec, long long xpitch, int ntiles, float scale,
+                     int chunk0, int nchunks) {
   __shared__ int rows[kTilePairs];
   __shared__ int active[16];
   int tile, chunk;
-  if (!tile_chunk(ntiles, (nspec + kBinChunk - 1) / kBinChunk, tile, chunk)) return;
+  if (!tile_chunk(ntiles, nchunks, tile, chunk)) return;
+  chunk += chunk0;
   const int tid = threadIdx.x;
   const int *trow = tile_rows + (size_t)tile * kTilePairs;
   for (int i = tid; i < kTilePairs; i += blockDim.x) rows[i] = trow[i];
@@ -657,7 +659,7 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
                      int nsta, int nspec, const float *__restrict__ sta_scale,
                      const TileDesc *__restrict__ tiles, const int *__restrict__ tile_rows,
                      float2 *__restrict__ xspec, long long xpitch, int ntiles, float scale,
-                     int kchunks) {
+                     int kchunks, int chunk0, int nchunks) {
   extern __shared__ __align__(16) float2 stage[];  // [kTilePairs][kBinChunk], swizzled
   __shared__ int rows[kTilePairs];
   const int tid = threadIdx.x;
@@ -667,7 +669,6 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
   constexpr bool kSplit = kMode == FXC_PREC_BF16X3 || kMode == FXC_PREC_FP16X3;
   constexpr bool kPipe = NK16 * (kSplit ? 2 : 1) <= 8;
   constexpr int kBins = kBinChunk / 8;  // bins per wave and item
-  const int nchunks = (nspec + kBinChunk - 1) / kBinChunk;
   const long long nitems = (long long)ntiles * ((nchunks + kChunkGroup - 1) / kChunkGroup * kChunkGroup);
   // next valid item at or after `it` (items whose chunk is past the end are skipped)
   auto next_item = [&](long long it, int &tile, int &chunk) -> long long {
@@ -679,7 +680,7 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
   long long item = next_item(blockIdx.x, tile, chunk);
   if (item >= nitems) return;
   ItemAddr ia;
-  item_addr<kMode>(tiles[tile], chunk, nsta, lane_r, lane_g, scale, sta_scale, ia);
+  item_addr<kMode>(tiles[tile], chunk0 + chunk, nsta, lane_r, lane_g, scale, sta_scale, ia);
   TileOps<NK16, kSplit> cur;
   // (the chunked-depth form takes the unpipelined path)
   constexpr bool pipe = kPipe && !kChunked;
@@ -697,7 +698,7 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
     const long long nitem = next_item(item + gridDim.x, ntile, nchunk);
     ItemAddr na = ia;
     if (nitem < nitems) {
-      item_addr<kMode>(tiles[ntile], nchunk, nsta, lane_r, lane_g, scale, sta_scale, na);
+      item_addr<kMode>(tiles[ntile], chunk0 + nchunk, nsta, lane_r, lane_g, scale, sta_scale, na);
       rreg0 = tile_rows[(size_t)ntile * kTilePairs + tid];
       rreg1 = tile_rows[(size_t)ntile * kTilePairs + tid + 512];
     }
@@ -1198,6 +1199,11 @@ struct fxc_xc_plan {
   // tables of the compile-time lag plan (nfft/2 == LagPlan18522::kM only)
   float2 *d_fix_tw = nullptr, *d_fix_twl = nullptr, *d_fix_fold = nullptr, *d_fix_roots = nullptr;
   long long chunk_rows_max = 0;  // cross-spectrum rows per workspace fill
+  // spec mode (fxc_xc_set_band): the stack is formed for the 16-bin chunks
+  // [band_c0, band_c1) only; the workspace columns outside them hold zeros
+  // for the first band_zero_rows rows
+  int band_c0 = 0, band_c1 = -1;
+  size_t band_zero_rows = 0;
   // nfft/2 beyond one workgroup's LDS: lag synthesis by the four-step
   // inverse transform (work buffer for kLagLargeChunk rows)
   bool large_lag = false;
@@ -1530,6 +1536,16 @@ extern "C" int fxc_xc_pairs_info(const fxc_xc_plan *p, long long *ntiles, long l
   return FXC_OK;
 }
 
+// Spec mode: bins [0, k0) and [k1, nspec) of every workspace row set to 0
+// (the band's contraction never writes them).
+__global__ void band_zero_kernel(float2 *__restrict__ x, long long pitch, long long rows, int k0,
+                                 int k1, int nspec) {
+  const int nout = k0 + (nspec - k1);
+  for (long long r = blockIdx.y; r < rows; r += gridDim.y)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nout; i += gridDim.x * blockDim.x)
+      x[r * pitch + (i < k0 ? i : k1 + (i - k0))] = make_float2(0.f, 0.f);
+}
+
 static int xc_ensure_workspace(fxc_xc_plan *p, size_t rows) {
   if (rows > p->xspec_rows) {
     dfree(p->d_xspec);
@@ -1537,6 +1553,7 @@ static int xc_ensure_workspace(fxc_xc_plan *p, size_t rows) {
     p->xspec_rows = 0;
     FXC_TRY(dmalloc((void **)&p->d_xspec, rows * (size_t)p->xpitch * sizeof(float2)));
     p->xspec_rows = rows;
+    p->band_zero_rows = 0;
   }
   if (p->precision == FXC_PREC_BF16X3 || p->precision == FXC_PREC_FP16 ||
       p->precision == FXC_PREC_FP16X3) {
@@ -1564,11 +1581,30 @@ static int xc_ensure_workspace(fxc_xc_plan *p, size_t rows) {
   return FXC_OK;
 }
 
+// Spec mode: zero the out-of-band columns of the workspace rows not yet cleared.
+static int xc_band_prepare(fxc_xc_plan *p, hipStream_t st) {
+  if (p->band_c1 < 0 || p->band_zero_rows >= p->xspec_rows) return FXC_OK;
+  const int k0 = p->band_c0 * kBinChunk, k1 = std::min(p->nspec, p->band_c1 * kBinChunk);
+  if (k0 + (p->nspec - k1) > 0) {
+    const long long rows = (long long)p->xspec_rows;
+    hipLaunchKernelGGL(band_zero_kernel, dim3(16, (unsigned)std::min<long long>(rows, 65535)),
+                       dim3(256), 0, st, p->d_xspec, p->xpitch, rows, k0, k1, p->nspec);
+    FXC_HIP(hipGetLastError());
+  }
+  p->band_zero_rows = p->xspec_rows;
+  return FXC_OK;
+}
+
 // Split pass for the MFMA forms (no-op for FP32).
 static int xc_prepare_spectra(fxc_xc_plan *p, const float2 *d_spec, hipStream_t st) {
   if (p->precision == FXC_PREC_FP32 || p->precision == FXC_PREC_PLAIN) return FXC_OK;
   FXC_TRY(xc_mark(p, kTimeSplit, true, st));
-  const dim3 grid((unsigned)ceil_div(p->nspec, kSplitBins), (unsigned)ceil_div(p->nsta, kSplitSta));
+  // spec mode: only the band's chunks (kSplitBins == kBinChunk)
+  static_assert(kSplitBins == kBinChunk, "split blocks are contraction chunks");
+  const bool band = p->band_c1 >= 0;
+  const int bin0 = band ? p->band_c0 * kSplitBins : 0;
+  const dim3 grid((unsigned)(band ? p->band_c1 - p->band_c0 : ceil_div(p->nspec, kSplitBins)),
+                  (unsigned)ceil_div(p->nsta, kSplitSta));
   if (p->precision == FXC_PREC_FP16 || p->precision == FXC_PREC_FP16X3) {
     FXC_HIP(hipMemsetAsync(p->d_amax, 0, p->nsta * sizeof(unsigned int), st));
     hipLaunchKernelGGL(station_absmax_kernel, dim3(8, p->nsta), dim3(256), 0, st, d_spec,
@@ -1577,13 +1613,13 @@ static int xc_prepare_spectra(fxc_xc_plan *p, const float2 *d_spec, hipStream_t 
                        st, p->d_amax, p->nsta, p->d_inv_scale, p->d_scale);
     if (p->precision == FXC_PREC_FP16)
       hipLaunchKernelGGL(split_kernel<FXC_PREC_FP16>, grid, dim3(kSplitThreads), 0, st, d_spec, p->nstep,
-                         p->nsta, p->nspec, p->kh, p->d_inv_scale, p->d_hi, p->d_lo);
+                         p->nsta, p->nspec, p->kh, p->d_inv_scale, p->d_hi, p->d_lo, bin0);
     else
       hipLaunchKernelGGL(split_kernel<FXC_PREC_FP16X3>, grid, dim3(kSplitThreads), 0, st, d_spec, p->nstep,
-                         p->nsta, p->nspec, p->kh, p->d_inv_scale, p->d_hi, p->d_lo);
+                         p->nsta, p->nspec, p->kh, p->d_inv_scale, p->d_hi, p->d_lo, bin0);
   } else {
     hipLaunchKernelGGL(split_kernel<FXC_PREC_BF16X3>, grid, dim3(kSplitThreads), 0, st, d_spec, p->nstep,
-                       p->nsta, p->nspec, p->kh, (const float *)nullptr, p->d_hi, p->d_lo);
+                       p->nsta, p->nspec, p->kh, (const float *)nullptr, p->d_hi, p->d_lo, bin0);
   }
   FXC_HIP(hipGetLastError());
   FXC_TRY(xc_mark(p, kTimeSplit, false, st));
@@ -1594,7 +1630,7 @@ template <int kMode, int KH, bool kChunked = false>
 static int launch_mfma_kh(dim3 grid, size_t lds, hipStream_t st, const unsigned short *hi,
                           const unsigned short *lo, int nsta, int nspec, const float *sc,
                           const TileDesc *tiles, const int *trows, float2 *dst, long long xpitch,
-                          float scale, int kchunks) {
+                          float scale, int kchunks, int chunk0, int nchunks) {
   auto kern = contract_mfma_kernel<kMode, KH / 8, kChunked>;
   // persistent grid: a multiple of 8 workgroups (one per CU, LDS-bound),
   // capped by the number of work items
@@ -1607,7 +1643,7 @@ static int launch_mfma_kh(dim3 grid, size_t lds, hipStream_t st, const unsigned 
   FXC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
   hipLaunchKernelGGL(kern, grid1, dim3(512), lds, st, hi, lo, nsta, nspec, sc, tiles, trows, dst,
-                     xpitch, (int)(grid.y), scale, kchunks);
+                     xpitch, (int)(grid.y), scale, kchunks, chunk0, nchunks);
   return FXC_OK;
 }
 
@@ -1615,12 +1651,13 @@ template <int kMode>
 static int launch_mfma_mode(int kh, dim3 grid, size_t lds, hipStream_t st,
                             const unsigned short *hi, const unsigned short *lo, int nsta, int nspec,
                             const float *sc, const TileDesc *tiles, const int *trows, float2 *dst,
-                            long long xpitch, float scale) {
+                            long long xpitch, float scale, int chunk0, int nchunks) {
   if (xc_depth_chunks(kh) > 1)
     return launch_mfma_kh<kMode, 32, true>(grid, lds, st, hi, lo, nsta, nspec, sc, tiles, trows,
-                                           dst, xpitch, scale, xc_depth_chunks(kh));
+                                           dst, xpitch, scale, xc_depth_chunks(kh), chunk0,
+                                           nchunks);
 #define FXC_KH(V) \
-  case V: return launch_mfma_kh<kMode, V>(grid, lds, st, hi, lo, nsta, nspec, sc, tiles, trows, dst, xpitch, scale, 1);
+  case V: return launch_mfma_kh<kMode, V>(grid, lds, st, hi, lo, nsta, nspec, sc, tiles, trows, dst, xpitch, scale, 1, chunk0, nchunks);
   switch (kh) {
     FXC_KH(8) FXC_KH(16) FXC_KH(24) FXC_KH(32) FXC_KH(40) FXC_KH(48) FXC_KH(64) FXC_KH(96)
     FXC_KH(128)
@@ -1634,22 +1671,27 @@ static int launch_mfma_mode(int kh, dim3 grid, size_t lds, hipStream_t st,
 static int launch_mfma_contract(int precision, int kh, dim3 grid, size_t lds, hipStream_t st,
                                 const unsigned short *hi, const unsigned short *lo, int nsta,
                                 int nspec, const float *sc, const TileDesc *tiles,
-                                const int *trows, float2 *dst, long long xpitch, float scale) {
+                                const int *trows, float2 *dst, long long xpitch, float scale,
+                                int chunk0, int nchunks) {
   if (precision == FXC_PREC_FP16X3)
     return launch_mfma_mode<FXC_PREC_FP16X3>(kh, grid, lds, st, hi, lo, nsta, nspec, sc, tiles,
-                                             trows, dst, xpitch, scale);
+                                             trows, dst, xpitch, scale, chunk0, nchunks);
   if (precision == FXC_PREC_BF16X3)
     return launch_mfma_mode<FXC_PREC_BF16X3>(kh, grid, lds, st, hi, lo, nsta, nspec, sc, tiles,
-                                             trows, dst, xpitch, scale);
+                                             trows, dst, xpitch, scale, chunk0, nchunks);
   return launch_mfma_mode<FXC_PREC_FP16>(kh, grid, lds, st, hi, lo, nsta, nspec, sc, tiles, trows,
-                                         dst, xpitch, scale);
+                                         dst, xpitch, scale, chunk0, nchunks);
 }
 
 static int xc_contract_chunk(fxc_xc_plan *p, const float2 *d_spec, long long c, float2 *dst,
                              hipStream_t st) {
   const long long t0 = p->chunk_tile_begin[c], t1 = p->chunk_tile_begin[c + 1];
   const unsigned ntl = (unsigned)(t1 - t0);
-  const unsigned nbc = (unsigned)ceil_div(p->nspec, kBinChunk);
+  int c0 = 0, nbc = ceil_div(p->nspec, kBinChunk);
+  if (p->band_c1 >= 0 && p->precision != FXC_PREC_PLAIN) {
+    c0 = p->band_c0;
+    nbc = p->band_c1 - p->band_c0;
+  }
   // 1-D grid covering ntl tiles x nbc chunks in groups of kChunkGroup chunks
   const unsigned nwg = ntl * (unsigned)(ceil_div(nbc, kChunkGroup) * kChunkGroup);
   const float scale = 1.0f / (float)p->nstep;
@@ -1668,13 +1710,13 @@ static int xc_contract_chunk(fxc_xc_plan *p, const float2 *d_spec, long long c, 
                          dst, p->xpitch, nbk, scale);
   } else if (p->precision == FXC_PREC_FP32) {
     hipLaunchKernelGGL(contract_fp32_kernel, dim3(nwg), dim3(256), 0, st, d_spec, p->nstep,
-                       p->nsta, p->nspec, tiles, trows, dst, p->xpitch, (int)ntl, scale);
+                       p->nsta, p->nspec, tiles, trows, dst, p->xpitch, (int)ntl, scale, c0, nbc);
   } else {
     const size_t lds = (size_t)kTilePairs * kBinChunk * sizeof(float2);
     // grid.y carries the tile count to the launcher (the launch itself is 1-D)
     FXC_TRY(launch_mfma_contract(p->precision, p->kh, dim3(nwg, ntl), lds, st, p->d_hi, p->d_lo,
                                  p->nsta, p->nspec, p->d_scale, tiles, trows, dst, p->xpitch,
-                                 scale));
+                                 scale, c0, nbc));
   }
   FXC_HIP(hipGetLastError());
   FXC_TRY(xc_mark(p, kTimeContract, false, st));
@@ -1778,6 +1820,7 @@ extern "C" int fxc_xc_cross_spectra(fxc_xc_plan *p, const float *d_spec, float *
   long long maxrows = 0;
   FXC_TRY(fxc_xc_pairs_info(p, nullptr, nullptr, &maxrows));
   FXC_TRY(xc_ensure_workspace(p, (size_t)maxrows));
+  FXC_TRY(xc_band_prepare(p, st));
   FXC_TRY(xc_prepare_spectra(p, (const float2 *)d_spec, st));
   const long long nc = (long long)p->chunk_tile_begin.size() - 1;
   for (long long c = 0; c < nc; ++c) {
@@ -1809,6 +1852,7 @@ extern "C" int fxc_xc_correlate(fxc_xc_plan *p, const float *d_spec, float *d_cc
   long long maxrows = 0;
   FXC_TRY(fxc_xc_pairs_info(p, nullptr, nullptr, &maxrows));
   FXC_TRY(xc_ensure_workspace(p, (size_t)maxrows));
+  FXC_TRY(xc_band_prepare(p, st));
   FXC_TRY(xc_prepare_spectra(p, (const float2 *)d_spec, st));
   const long long nc = (long long)p->chunk_tile_begin.size() - 1;
   for (long long c = 0; c < nc; ++c) {
@@ -1816,6 +1860,25 @@ extern "C" int fxc_xc_correlate(fxc_xc_plan *p, const float *d_spec, float *d_cc
     const long long r0 = p->chunk_row_begin[c], r1 = p->chunk_row_begin[c + 1];
     FXC_TRY(xc_lags_rows(p, p->d_xspec, p->xpitch, r1 - r0, p->d_row_task + r0, d_cc, st));
   }
+  return FXC_OK;
+}
+
+extern "C" int fxc_xc_set_band(fxc_xc_plan *p, int k_lo, int k_hi) {
+  FXC_CHECK_ARG(p, "fxc_xc_set_band: null plan");
+  FXC_CHECK_ARG(k_lo >= 0 && k_lo <= k_hi && k_hi < p->nspec,
+                "fxc_xc_set_band: band [%d, %d] outside bins [0, %d)", k_lo, k_hi, p->nspec);
+  if (k_lo == 0 && k_hi == p->nspec - 1) {
+    p->band_c0 = 0;
+    p->band_c1 = -1;
+    return FXC_OK;
+  }
+  if (p->precision == FXC_PREC_PLAIN) {
+    set_error("fxc_xc_set_band: the PLAIN form contracts every bin (parity only)");
+    return FXC_ERR_UNSUPPORTED;
+  }
+  p->band_c0 = k_lo / kBinChunk;
+  p->band_c1 = ceil_div(k_hi + 1, kBinChunk);
+  p->band_zero_rows = 0;
   return FXC_OK;
 }
 

@@ -137,6 +137,14 @@ class XcPlan:
                                 _stream_ptr(xspec.device)), "fxc_xc_lags")
         return out
 
+    def set_band(self, k_lo: int | None = None, k_hi: int | None = None) -> None:
+        """Spec mode: form the windowed stack only for bins k_lo..k_hi (rounded
+        out to 16-bin chunks), every other cross-spectrum bin 0.  No
+        arguments: the parity mode (every bin)."""
+        lo = 0 if k_lo is None else int(k_lo)
+        hi = self.nspec - 1 if k_hi is None else int(k_hi)
+        check(lib().fxc_xc_set_band(self._h, lo, hi), "fxc_xc_set_band")
+
     def set_timing(self, enable: bool) -> None:
         check(lib().fxc_xc_set_timing(self._h, int(enable)), "fxc_xc_set_timing")
 

@@ -91,6 +91,14 @@ int fxc_xc_lags(fxc_xc_plan *plan, const float *d_xspec, long long nrows, float 
  * contraction and lag-synthesis kernels since the last set_timing. */
 int fxc_xc_set_timing(fxc_xc_plan *plan, int enable);
 int fxc_xc_get_timing(fxc_xc_plan *plan, double *ms3, long long *launches3);
+/* Spec mode (no reference counterpart: the reference contracts all nspec
+ * bins, native/xc/worker/async_workers.cu:66-83): the windowed stack is
+ * formed only for the 16-bin chunks that hold bins k_lo..k_hi, and every
+ * other bin of the cross-spectrum is 0 before the lag synthesis.  For spectra
+ * whose whitening band lies inside [k_lo, k_hi] the lags differ from the
+ * parity result by the energy the spectra hold outside the band.  k_lo = 0,
+ * k_hi = nspec-1 restores the parity mode (the default).  Not for PLAIN. */
+int fxc_xc_set_band(fxc_xc_plan *plan, int k_lo, int k_hi);
 /* Stack + lag synthesis for every task (execute_row_batch_job,
  * native/xc/worker/async_workers.cu:18-86, minus file I/O). */
 int fxc_xc_correlate(fxc_xc_plan *plan, const float *d_spec, float *d_cc, void *stream);
