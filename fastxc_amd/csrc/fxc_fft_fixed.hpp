@@ -288,4 +288,107 @@ __device__ __forceinline__ float2 fixed_last_pass_bin(const float2 *buf,
   return fixed_last_pass_at<FP, kDirect>(buf, roots, t, base);
 }
 
+// ---------------------------------------------------------------------------
+// Whole transforms on a compile-time plan: forward DIF (natural in,
+// digit-reversed out) and inverse DIT (digit-reversed in, natural out), for
+// kernels that chain several transforms of one row in LDS.  Base twiddles
+// w = exp(-2 pi i j / S) (sign -1; the inverse conjugates them): pass 0 reads
+// its stride(0) values from a global table (fixed_pass0_twiddles, L2
+// resident), passes 1..P-2 from the LDS table of fixed_lds_twiddles(-1); the
+// last pass has stride 1 and no twiddles.  The R-1 powers are chained.
+// ---------------------------------------------------------------------------
+template <class FP>
+std::vector<float2> fixed_pass0_twiddles() {
+  std::vector<float2> t((size_t)FP::stride(0));
+  for (int j = 0; j < FP::stride(0); ++j) {
+    const double a = -2.0 * 3.14159265358979323846 * (double)j / (double)FP::kM;
+    t[(size_t)j] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  return t;
+}
+
+// One pass p: DIF (butterfly, then twiddle the outputs) or DIT (twiddle the
+// inputs, then butterfly); kInv selects the +1 sign.
+template <class FP, int p, int kThreads, bool kDit, bool kInv>
+__device__ __forceinline__ void fixed_pass(float2 *buf, const float2 *__restrict__ tw0,
+                                           const float2 *twl) {
+  constexpr int R = FP::kRadix[p];
+  constexpr int S = FP::span(p);
+  constexpr int s = FP::stride(p);
+  constexpr int nbt = FP::kM / R;
+  constexpr bool kBlockMajor = s < 32;  // see fixed_dif_pass
+  constexpr int nblk = FP::kM / S;
+  constexpr float sgn = kInv ? 1.f : -1.f;
+  for (int b = threadIdx.x; b < nbt; b += kThreads) {
+    const int blk = kBlockMajor ? b % nblk : b / s;
+    const int j = kBlockMajor ? b / nblk : b - blk * s;
+    float2 *base = buf + blk * S + j;
+    float2 w1 = make_float2(1.f, 0.f);
+    if constexpr (s > 1) {
+      w1 = p == 0 ? tw0[j] : twl[fixed_lds_tw_offset<FP>(p) + j];
+      if (kInv) w1.y = -w1.y;
+    }
+    float2 v[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) v[m] = base[m * s];
+    // v[q] *= w1^q, the powers chained as they are used (two live values)
+    auto twiddle = [&]() {
+      float2 wq = w1;
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        v[q] = cmul(v[q], wq);
+        if (q + 1 < R) wq = cmul(wq, w1);
+      }
+    };
+    if constexpr (s > 1 && kDit) twiddle();
+    dft<R>(v, sgn);
+    if constexpr (s > 1 && !kDit) twiddle();
+#pragma unroll
+    for (int q = 0; q < R; ++q) base[q * s] = v[q];
+  }
+  __syncthreads();
+}
+
+template <class FP, int p, int kThreads>
+__device__ __forceinline__ void fixed_fft_fwd_from(float2 *buf, const float2 *__restrict__ tw0,
+                                                   const float2 *twl) {
+  if constexpr (p < FP::kPasses) {
+    fixed_pass<FP, p, kThreads, false, false>(buf, tw0, twl);
+    fixed_fft_fwd_from<FP, p + 1, kThreads>(buf, tw0, twl);
+  }
+}
+template <class FP, int p, int kThreads>
+__device__ __forceinline__ void fixed_fft_inv_from(float2 *buf, const float2 *__restrict__ tw0,
+                                                   const float2 *twl) {
+  if constexpr (p >= 0) {
+    fixed_pass<FP, p, kThreads, true, true>(buf, tw0, twl);
+    fixed_fft_inv_from<FP, p - 1, kThreads>(buf, tw0, twl);
+  }
+}
+// Forward (sign -1) DIF: natural order in, digit-reversed (fixed_pos) out.
+template <class FP, int kThreads>
+__device__ __forceinline__ void fixed_fft_fwd(float2 *buf, const float2 *__restrict__ tw0,
+                                              const float2 *twl) {
+  fixed_fft_fwd_from<FP, 0, kThreads>(buf, tw0, twl);
+}
+// Inverse (sign +1, unnormalised) DIT: digit-reversed in, natural out.
+template <class FP, int kThreads>
+__device__ __forceinline__ void fixed_fft_inv(float2 *buf, const float2 *__restrict__ tw0,
+                                              const float2 *twl) {
+  fixed_fft_inv_from<FP, FP::kPasses - 1, kThreads>(buf, tw0, twl);
+}
+
+// Position of bin k in the digit-reversed result of fixed_fft_fwd.
+template <class FP>
+__device__ __forceinline__ int fixed_pos(int k) {
+  int pos = 0;
+#pragma unroll
+  for (int p = 0; p < FP::kPasses; ++p) {
+    const int q = k / FP::kRadix[p];
+    pos += (k - q * FP::kRadix[p]) * FP::stride(p);
+    k = q;
+  }
+  return pos;
+}
+
 }  // namespace fxc

@@ -17,6 +17,7 @@
 // the normalisation divide and clip, whitening divide and taper, and the
 // phase-only option are all folded into those passes.  Sums the reference
 // takes in FP64 (mean, trend, running means) stay FP64.
+#include "fxc_fft_fixed.hpp"
 #include "fxc_fft_large.hpp"
 #include "fxc_plan_cache.hpp"
 #include "fastxc_amd.h"
@@ -710,6 +711,390 @@ __global__ void s2sL_out_kernel(float2 *__restrict__ out, int nspec, float dt, i
   out[row * nspec + k] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Fused one-channel path at the production lengths (3600 s windows at 0.1 s:
+// seg 36000, prefilter FFT 36288, output FFT 37044).  The whole chain of one
+// station-window runs in one workgroup's LDS on compile-time plans, in at
+// most three kernels instead of five, and only the trace window, the
+// whitening spectrum X (when whitening) and the output spectrum touch HBM:
+//   front: window cut, preprocess, prefilter, [normalise], R2C(seg) -> X  or
+//          R2C(nfft_out) -> output
+//   back:  X -> whitening weights (running mean of |X| over the bins), X/w,
+//          taper, C2R(seg), [normalise], R2C(nfft_out) -> output or
+//          R2C(seg) -> X (whitening before and after)
+// A frame of one channel owns its weights, so the running means over time
+// (RUN-ABS) and over bins (whitening) are taken in LDS: each thread slides
+// an FP64 window over a contiguous block of outputs, as the reference's
+// smoothRowsRollingKernel (kernels/real_matrix.cu:155-191) with its clamp.
+// ---------------------------------------------------------------------------
+// Radix orders keep the passes whose legs sit within one block (stride < 32,
+// taken block-major) on odd spans (81, 9; 75, 5; 63, 9): neighbouring lanes
+// then address blocks an odd number of words apart, which spreads them over
+// the LDS banks, where even spans such as 108 put 8 to 16 lanes on one bank.
+using S2sFixF = FixedPlan<18144, 16, 14, 9, 9>;   // prefilter: nfft 36288
+using S2sFixS = FixedPlan<18000, 16, 15, 15, 5>;  // window: seg 36000
+using S2sFixO = FixedPlan<18522, 6, 7, 7, 7, 9>;  // output: nfft 37044
+static_assert(S2sFixF::valid() && S2sFixS::valid() && S2sFixO::valid(), "plans");
+constexpr int kFusedPer = 37;   // RUN-ABS outputs per thread (odd: conflict-free LDS reads)
+constexpr int kFusedPerB = 19;  // whitening bins per thread
+static_assert(kFusedPer * kS2sThreads >= 2 * S2sFixS::kM, "RUN-ABS blocks cover seg");
+static_assert(kFusedPerB * kS2sThreads >= S2sFixS::kM + 1, "whitening blocks cover seg/2+1");
+
+struct FixedTables {
+  const float2 *tw0;   // pass-0 base twiddles (global)
+  const float2 *twl;   // passes 1..P-2 base twiddles (copied to LDS)
+  const float2 *fold;  // exp(-2 pi i k / 2M), k <= M, two-level (copied to LDS)
+};
+
+template <class FP>
+__device__ inline void stage_fixed(float2 *twl_lds, float2 *fold_lds, const FixedTables &t) {
+  for (int i = threadIdx.x; i < fixed_lds_tw_count<FP>(); i += blockDim.x) twl_lds[i] = t.twl[i];
+  for (int i = threadIdx.x; i < fold_lds_count(FP::kM); i += blockDim.x) fold_lds[i] = t.fold[i];
+}
+
+// fold_lds_count as a constant expression
+constexpr int fused_fold_count(int M) { return kTwFine + (M + 1 + kTwFine - 1) / kTwFine; }
+
+template <class FPA, class FPB>
+constexpr int fused_lds_count() {
+  return (FPA::kM > FPB::kM ? FPA::kM : FPB::kM) + 1 + fixed_lds_tw_count<FPA>() +
+         fixed_lds_tw_count<FPB>() + fused_fold_count(FPA::kM) + fused_fold_count(FPB::kM);
+}
+
+// After fixed_fft_fwd of the packed real row: X[k], k = 0..M, to st(k, X[k])
+// (the unfold of r2c_unfold at the plan's digit-reversed positions).
+template <class FP, class St>
+__device__ inline void fixed_r2c_emit(const float2 *lds, const float2 *fold, const St &st) {
+  constexpr int M = FP::kM;
+  for (int k = threadIdx.x; k <= M; k += blockDim.x) {
+    const float2 a = lds[fixed_pos<FP>(k == M ? 0 : k)];
+    const float2 b = cconj(lds[fixed_pos<FP>(k == 0 ? 0 : M - k)]);
+    const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+    const float2 d = csub(a, b);
+    const float2 o = make_float2(0.5f * d.y, -0.5f * d.x);
+    float2 x = cadd(e, cmul(fold_lookup(fold, k), o));
+    if (k == 0 || k == M) x.y = 0.f;
+    st(k, x);
+  }
+}
+
+// spectral_pairs at the plan's digit-reversed positions: f(k, X[k], X[M-k])
+// -> (Y[k], Y[M-k]), refolded in place for fixed_fft_inv.
+template <class FP, class F>
+__device__ inline void fixed_spectral_pairs(float2 *lds, const float2 *fold, const F &f) {
+  constexpr int M = FP::kM;
+  for (int k = threadIdx.x; 2 * k <= M; k += blockDim.x) {
+    const int pk = fixed_pos<FP>(k);
+    const int pm = fixed_pos<FP>(k == 0 ? 0 : M - k);
+    const float2 a = lds[pk];
+    const float2 b = cconj(lds[pm]);
+    float2 xk, xm;
+    if (k == 0) {
+      xk = make_float2(a.x + a.y, 0.f);
+      xm = make_float2(a.x - a.y, 0.f);
+    } else {
+      const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+      const float2 d = csub(a, b);
+      const float2 o = make_float2(0.5f * d.y, -0.5f * d.x);
+      xk = cadd(e, cmul(fold_lookup(fold, k), o));
+      const float2 a2 = lds[pm];
+      const float2 b2 = cconj(lds[pk]);
+      const float2 e2 = make_float2(0.5f * (a2.x + b2.x), 0.5f * (a2.y + b2.y));
+      const float2 d2 = csub(a2, b2);
+      const float2 o2 = make_float2(0.5f * d2.y, -0.5f * d2.x);
+      xm = cadd(e2, cmul(fold_lookup(fold, M - k), o2));
+    }
+    float2 yk, ym;
+    f(k, xk, xm, yk, ym);
+    if (k == 0) {
+      lds[pk] = make_float2(yk.x + ym.x, yk.x - ym.x);
+    } else {
+      const float2 s1 = cadd(yk, cconj(ym));
+      const float2 d1 = cmul(cconj(fold_lookup(fold, k)), csub(yk, cconj(ym)));
+      const float2 zk = make_float2(s1.x - d1.y, s1.y + d1.x);
+      if (2 * k != M) {
+        const float2 s2 = cadd(ym, cconj(yk));
+        const float2 d2 = cmul(cconj(fold_lookup(fold, M - k)), csub(ym, cconj(yk)));
+        lds[pm] = make_float2(s2.x - d2.y, s2.y + d2.x);
+      }
+      lds[pk] = zk;
+    }
+  }
+  __syncthreads();
+}
+
+// Running mean w_t = (1/win) sum_{i = t - win/2}^{t - win/2 + win - 1} a(i)
+// over i in [0, n) (FP64 window sum, float result, clamped to >= MINVAL),
+// for the kPer contiguous outputs t0 = threadIdx.x * kPer + m of this
+// thread, each handed to use(m, t, w_t).
+template <int kPer, class A, class U>
+__device__ inline void sliding_mean(int n, int win, const A &a, const U &use) {
+  const int t0 = (int)threadIdx.x * kPer;
+  if (t0 >= n) return;
+  int start = t0 - win / 2, end = start + win - 1;
+  const double inv = 1.0 / (double)win;
+  double sum = 0.0;
+  for (int i = max(start, 0); i <= min(end, n - 1); ++i) sum += (double)a(i);
+#pragma unroll
+  for (int m = 0; m < kPer; ++m) {
+    const int t = t0 + m;
+    if (t < n) {
+      float w = (float)(inv * sum);
+      if (!isfinite(w) || w < kMinVal) w = kMinVal;
+      use(m, t, w);
+      if (start >= 0 && start < n) sum -= (double)a(start);
+      ++start;
+      ++end;
+      if (end >= 0 && end < n) sum += (double)a(end);
+    }
+  }
+}
+
+// Time-domain normalisation of the row x[0..n) in LDS (one channel):
+// RUN-ABS x / w with w the running mean of |x| (0 for an invalid weight,
+// non-finite -> 0, clip to +-MAXVAL), or one-bit sign(x)
+// (normalization.cu:91-213, compute.cu:144-176).
+__device__ inline void normalise_row(float *x, int n, int mode, int win) {
+  if (mode == kLoadRunabs) {
+    float y[kFusedPer];
+    sliding_mean<kFusedPer>(n, win, [&](int i) { return fabsf(x[i]); },
+                            [&](int m, int t, float d) {
+                              float v = x[t];
+                              v = (!isfinite(d) || d < kMinValD) ? 0.f : v / d;
+                              if (!isfinite(v)) v = 0.f;
+                              y[m] = v > kMaxVal ? kMaxVal : (v < -kMaxVal ? -kMaxVal : v);
+                            });
+    __syncthreads();
+    const int t0 = (int)threadIdx.x * kFusedPer;
+#pragma unroll
+    for (int m = 0; m < kFusedPer; ++m)
+      if (t0 + m < n) x[t0 + m] = y[m];
+    __syncthreads();
+  } else if (mode == kLoadOnebit) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const float v = x[i];
+      x[i] = v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f);
+    }
+    __syncthreads();
+  }
+}
+
+// Window cut + non-finite -> 0 + demean/detrend (FP64 sums) + cosine taper
+// into the float view of LDS, zero padded to kTwoM samples
+// (compute.cu:34-58, core/preprocess.cu:23-105).  Every load of the window
+// is issued before the first LDS store, so their latencies overlap.
+template <int kTwoM>
+__device__ inline void preprocess_row(float *ldsf, const float *__restrict__ src, long long off,
+                                      int npts, int seg, int taper_size) {
+  constexpr int kPer = (kTwoM + kS2sThreads - 1) / kS2sThreads;
+  double s = 0.0, unused1 = 0.0, unused2 = 0.0;
+  float v[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int i = threadIdx.x + j * kS2sThreads;
+    v[j] = (i < seg && off + i < npts) ? src[off + i] : 0.f;
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int i = threadIdx.x + j * kS2sThreads;
+    if (i < kTwoM) {
+      const float x = isfinite(v[j]) ? v[j] : 0.f;
+      ldsf[i] = x;
+      s += x;  // zero past seg
+    }
+  }
+  block_sum3(s, unused1, unused2);
+  const double mean = s / seg;
+  double s1 = 0.0, si = 0.0;
+  for (int i = threadIdx.x; i < seg; i += blockDim.x) {
+    const float v = (float)((double)ldsf[i] - mean);
+    ldsf[i] = v;
+    s1 += v;
+    si += (double)v * i;
+  }
+  unused1 = 0.0;
+  block_sum3(s1, si, unused1);
+  const double W = seg;
+  const double a12 = 0.5 * W * (W - 1);
+  const double a11 = a12 * (2 * W - 1) / 3.;
+  const double a22 = W;
+  const double det = a11 * a22 - a12 * a12;
+  const double ca = (a22 * si - a12 * s1) / det;
+  const double cb = (a11 * s1 - a12 * si) / det;
+  for (int i = threadIdx.x; i < seg; i += blockDim.x) {
+    float v = (float)((double)ldsf[i] - (ca * i + cb));
+    float factor = 1.f;
+    if (i < taper_size)
+      factor = (float)(0.5 * (1.0 - cos(M_PI * i / taper_size)));
+    else if (i >= seg - taper_size)
+      factor = (float)(0.5 * (1.0 - cos(M_PI * (seg - i) / taper_size)));
+    ldsf[i] = v * factor;
+  }
+  __syncthreads();
+}
+
+// The last transform of a fused kernel: the real row x[0..seg) in LDS ->
+// R2C(2 MN) x dt, sanitised, to X (whitening spectrum, k <= MN) or to the
+// output spectrum (k < nspec, optional phase-only).
+template <class FPN, bool kOut>
+__device__ inline void fused_emit(float2 *lds, int seg, const float2 *__restrict__ tw0n,
+                                  const float2 *twln, const float2 *foldn, float dt,
+                                  int phase_only, float2 *__restrict__ dst, int nspec) {
+  float *ldsf = (float *)lds;
+  if constexpr (kOut) {
+    for (int i = seg + (int)threadIdx.x; i < 2 * FPN::kM; i += blockDim.x) ldsf[i] = 0.f;
+    __syncthreads();
+  }
+  fixed_fft_fwd<FPN, kS2sThreads>(lds, tw0n, twln);
+  fixed_r2c_emit<FPN>(lds, foldn, [&](int k, float2 v) {
+    if (kOut && k >= nspec) return;
+    v = cscale(v, dt);
+    if (!cfinite(v)) v = make_float2(0.f, 0.f);
+    if (kOut && phase_only) {
+      const float a = hypotf(v.x, v.y);
+      v = (!isfinite(a) || a <= kMinVal) ? make_float2(0.f, 0.f) : cscale(v, 1.f / a);
+      if (!cfinite(v)) v = make_float2(0.f, 0.f);
+    }
+    dst[k] = v;
+  });
+}
+
+template <class FPF, class FPN, bool kOut>
+__global__ void __launch_bounds__(kS2sThreads)
+s2s_front_kernel(const float *__restrict__ traces, int npts, int nfiles,
+                 const int *__restrict__ steps, int shift, int seg, int taper_size,
+                 FixedTables tf, FixedTables tn, const float *__restrict__ resp0, float dt,
+                 double inv_weight, int norm_mode, int runabs_win, int phase_only,
+                 float2 *__restrict__ dst, long long dpitch, int nspec) {
+  extern __shared__ float2 lds[];
+  constexpr int kData = (FPF::kM > FPN::kM ? FPF::kM : FPN::kM) + 1;
+  float2 *twlf = lds + kData;
+  float2 *twln = twlf + fixed_lds_tw_count<FPF>();
+  float2 *foldf = twln + fixed_lds_tw_count<FPN>();
+  float2 *foldn = foldf + fold_lds_count(FPF::kM);
+  stage_fixed<FPF>(twlf, foldf, tf);
+  stage_fixed<FPN>(twln, foldn, tn);
+  float *ldsf = (float *)lds;
+  const int row = blockIdx.x;
+  const int step = row / nfiles, file = row - step * nfiles;
+  preprocess_row<2 * FPF::kM>(ldsf, traces + (size_t)file * npts,
+                              (long long)steps[step] * shift, npts, seg, taper_size);
+  constexpr int Mf = FPF::kM;
+  fixed_fft_fwd<FPF, kS2sThreads>(lds, tf.tw0, twlf);
+  fixed_spectral_pairs<FPF>(lds, foldf, [&](int k, float2 xk, float2 xm, float2 &yk, float2 &ym) {
+    xk = cscale(xk, dt);
+    xm = cscale(xm, dt);
+    if (!cfinite(xk)) xk = make_float2(0.f, 0.f);
+    if (!cfinite(xm)) xm = make_float2(0.f, 0.f);
+    yk = cscale(xk, resp0[k]);
+    ym = cscale(xm, resp0[k == 0 ? Mf : Mf - k]);
+  });
+  fixed_fft_inv<FPF, kS2sThreads>(lds, tf.tw0, twlf);
+  for (int i = threadIdx.x; i < seg; i += blockDim.x) ldsf[i] = (float)((double)ldsf[i] * inv_weight);
+  __syncthreads();
+  normalise_row(ldsf, seg, norm_mode, runabs_win);
+  fused_emit<FPN, kOut>(lds, seg, tn.tw0, twln, foldn, dt, phase_only,
+                        dst + (size_t)row * dpitch, nspec);
+}
+
+template <class FPS, class FPN, bool kOut>
+__global__ void __launch_bounds__(kS2sThreads)
+s2s_back_kernel(float2 *X, long long fpitch, int seg, int i1, int i2, int i3, int i4,
+                int whiten_win, FixedTables ts, FixedTables tn, double inv_weight_seg,
+                int norm_mode, int runabs_win, float dt, int phase_only, float2 *dst,
+                long long dpitch, int nspec) {
+  extern __shared__ float2 lds[];
+  constexpr int kData = (FPS::kM > FPN::kM ? FPS::kM : FPN::kM) + 1;
+  constexpr int Ms = FPS::kM;
+  float2 *twls = lds + kData;
+  float2 *twln = twls + fixed_lds_tw_count<FPS>();
+  float2 *folds = twln + fixed_lds_tw_count<FPN>();
+  float2 *foldn = folds + fold_lds_count(FPS::kM);
+  stage_fixed<FPS>(twls, folds, ts);
+  stage_fixed<FPN>(twln, foldn, tn);
+  const int row = blockIdx.x;
+  const float2 *Xr = X + (size_t)row * fpitch;
+  {
+    // every load of the row before the first LDS store
+    constexpr int kPer = (Ms + 1 + kS2sThreads - 1) / kS2sThreads;
+    float2 v[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int k = threadIdx.x + j * kS2sThreads;
+      if (k <= Ms) v[j] = Xr[k];
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int k = threadIdx.x + j * kS2sThreads;
+      if (k <= Ms) lds[k] = v[j];
+    }
+  }
+  __syncthreads();
+  // whitening weights (running mean of |X| over the bins, one channel) and
+  // X / w x taper, sanitised (spectrum.cu:205-300)
+  const int nbin = Ms + 1;
+  const int t4 = i4 < nbin ? i4 : nbin;
+  float wk[kFusedPerB];
+  sliding_mean<kFusedPerB>(
+      nbin, whiten_win,
+      [&](int k) {
+        const float2 z = lds[k];
+        return sqrtf(z.x * z.x + z.y * z.y);
+      },
+      [&](int m, int, float d) { wk[m] = d; });
+  __syncthreads();  // every |X| read is done: each thread rewrites its own bins
+  {
+    const int k0 = (int)threadIdx.x * kFusedPerB;
+#pragma unroll
+    for (int m = 0; m < kFusedPerB; ++m) {
+      const int k = k0 + m;
+      if (k < nbin) {
+        const float d = wk[m];
+        float2 v = lds[k];
+        v = (!isfinite(d) || d < kMinValD) ? make_float2(0.f, 0.f) : make_float2(v.x / d, v.y / d);
+        v = cscale(v, spec_taper(k, i1, i2, i3, t4));
+        if (!cfinite(v)) v = make_float2(0.f, 0.f);
+        lds[k] = v;
+      }
+    }
+  }
+  __syncthreads();
+  // fold into the digit-reversed positions of the inverse (every value read
+  // before any is written)
+  constexpr int kFoldPer = (Ms + kS2sThreads - 1) / kS2sThreads;
+  float2 z[kFoldPer];
+#pragma unroll
+  for (int i = 0; i < kFoldPer; ++i) {
+    const int k = threadIdx.x + i * kS2sThreads;
+    if (k < Ms) {
+      float2 a = lds[k];
+      float2 b = cconj(lds[Ms - k]);
+      if (k == 0) {
+        a.y = 0.f;
+        b.y = 0.f;
+      }
+      const float2 sv = cadd(a, b);
+      const float2 dv = cmul(cconj(fold_lookup(folds, k)), csub(a, b));
+      z[i] = make_float2(sv.x - dv.y, sv.y + dv.x);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kFoldPer; ++i) {
+    const int k = threadIdx.x + i * kS2sThreads;
+    if (k < Ms) lds[fixed_pos<FPS>(k)] = z[i];
+  }
+  __syncthreads();
+  fixed_fft_inv<FPS, kS2sThreads>(lds, ts.tw0, twls);
+  float *ldsf = (float *)lds;
+  for (int i = threadIdx.x; i < seg; i += blockDim.x)
+    ldsf[i] = (float)((double)ldsf[i] * inv_weight_seg);
+  __syncthreads();
+  normalise_row(ldsf, seg, norm_mode, runabs_win);
+  fused_emit<FPN, kOut>(lds, seg, tn.tw0, twln, foldn, dt, phase_only, dst + (size_t)row * dpitch,
+                        nspec);
+}
+
 }  // namespace fxc
 
 using namespace fxc;
@@ -742,6 +1127,10 @@ struct fxc_s2s_plan {
   float2 *d_L = nullptr;             // [rows][lpitch]
   float2 *d_Lw = nullptr;            // four-step work, kLargeChunk rows
   long long cap_L = 0;
+  // fused one-channel path at the production lengths (s2s_front/back_kernel)
+  bool fused = false;
+  float2 *d_fx = nullptr;  // pass-0 and LDS base twiddles of the three plans
+  FixedTables fxf{}, fxs{}, fxo{};
 };
 
 // Rows per four-step call of the long-window path (bounds its work buffer).
@@ -766,6 +1155,36 @@ template <class K>
 static int set_lds(K kern, size_t bytes) {
   FXC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)bytes));
+  return FXC_OK;
+}
+
+// The fused path applies to one channel, every normalisation but RUN-ABS-MF,
+// and the production lengths; FXC_S2S_FUSED=0 keeps the five-kernel path.
+static int s2s_fused_setup(fxc_s2s_plan *p) {
+  const fxc_s2s_config &c = p->cfg;
+  const char *env = getenv("FXC_S2S_FUSED");
+  if (env && atoi(env) == 0) return FXC_OK;
+  if (c.num_ch != 1 || c.normalize == 1 || c.nfft_filter / 2 != S2sFixF::kM ||
+      c.seg / 2 != S2sFixS::kM || c.nfft_out / 2 != S2sFixO::kM)
+    return FXC_OK;
+  std::vector<float2> all;
+  size_t off[6];
+  auto add = [&](const std::vector<float2> &v, size_t *o) {
+    *o = all.size();
+    all.insert(all.end(), v.begin(), v.end());
+  };
+  add(fixed_pass0_twiddles<S2sFixF>(), &off[0]);
+  add(fixed_lds_twiddles<S2sFixF>(-1), &off[1]);
+  add(fixed_pass0_twiddles<S2sFixS>(), &off[2]);
+  add(fixed_lds_twiddles<S2sFixS>(-1), &off[3]);
+  add(fixed_pass0_twiddles<S2sFixO>(), &off[4]);
+  add(fixed_lds_twiddles<S2sFixO>(-1), &off[5]);
+  FXC_TRY(dmalloc((void **)&p->d_fx, all.size() * sizeof(float2)));
+  FXC_HIP(hipMemcpy(p->d_fx, all.data(), all.size() * sizeof(float2), hipMemcpyHostToDevice));
+  p->fxf = {p->d_fx + off[0], p->d_fx + off[1], p->tf.fold};
+  p->fxs = {p->d_fx + off[2], p->d_fx + off[3], p->ts.fold};
+  p->fxo = {p->d_fx + off[4], p->d_fx + off[5], p->to.fold};
+  p->fused = true;
   return FXC_OK;
 }
 
@@ -846,6 +1265,10 @@ extern "C" int fxc_s2s_plan_create(fxc_s2s_plan **out, const fxc_s2s_config *cfg
     delete p;
     return rc;
   }
+  if (!large && (rc = s2s_fused_setup(p)) != FXC_OK) {
+    fxc_s2s_plan_destroy(p);
+    return rc;
+  }
   *out = p;
   return FXC_OK;
 }
@@ -854,7 +1277,8 @@ extern "C" int fxc_s2s_plan_destroy(fxc_s2s_plan *p) {
   if (!p) return FXC_OK;
   for (void *q : {(void *)p->d_resp, (void *)p->d_x, (void *)p->d_w, (void *)p->d_xb,
                   (void *)p->d_total, (void *)p->d_X, (void *)p->d_base, (void *)p->d_amp,
-                  (void *)p->d_ww, (void *)p->d_steps, (void *)p->d_L, (void *)p->d_Lw})
+                  (void *)p->d_ww, (void *)p->d_steps, (void *)p->d_L, (void *)p->d_Lw,
+                  (void *)p->d_fx})
     dfree(q);
   delete p;
   return FXC_OK;
@@ -872,6 +1296,18 @@ extern "C" int fxc_s2s_plan_info(const fxc_s2s_plan *p, int *nspec, int *taper_s
 
 static int s2s_ensure(fxc_s2s_plan *p, long long rows, long long frames, int nsteps) {
   const fxc_s2s_config &c = p->cfg;
+  if (p->fused) {
+    // only the whitening spectra X travel between the fused kernels
+    if (c.whiten && rows > p->cap_rows) {
+      dfree(p->d_X);
+      p->d_X = nullptr;
+      p->cap_rows = 0;
+      FXC_TRY(dmalloc((void **)&p->d_X, (size_t)rows * p->fpitch * sizeof(float2)));
+      p->cap_rows = rows;
+    }
+    frames = 0;
+    rows = 0;
+  }
   if (rows > p->cap_rows) {
     for (void *q : {(void *)p->d_x, (void *)p->d_xb, (void *)p->d_total, (void *)p->d_X,
                     (void *)p->d_base, (void *)p->d_amp})
@@ -1089,6 +1525,67 @@ static int s2s_process_large(fxc_s2s_plan *p, const float *d_traces, int nfiles,
   return FXC_OK;
 }
 
+template <class FPN, bool kOut>
+static int launch_front(fxc_s2s_plan *p, const float *d_traces, int nfiles, long long rows,
+                        const FixedTables &tn, int norm, float2 *dst, long long dpitch,
+                        hipStream_t st) {
+  const fxc_s2s_config &c = p->cfg;
+  constexpr size_t lds = (size_t)fused_lds_count<S2sFixF, FPN>() * sizeof(float2);
+  static_assert(lds <= (size_t)kLdsComplexCapacity * sizeof(float2), "fused front fits LDS");
+  auto kern = s2s_front_kernel<S2sFixF, FPN, kOut>;
+  FXC_TRY(set_lds(kern, lds));
+  const double inv_f = 1.0 / (double)((float)c.nfft_filter * c.delta);
+  hipLaunchKernelGGL(kern, dim3((unsigned)rows), dim3(kS2sThreads), lds, st, d_traces, c.npts,
+                     nfiles, p->d_steps, c.shift, c.seg, p->taper_size, p->fxf, tn, p->d_resp,
+                     c.delta, inv_f, norm, p->runabs_win, c.phase_only, dst, dpitch, p->nspec);
+  FXC_HIP(hipGetLastError());
+  return FXC_OK;
+}
+
+template <class FPN, bool kOut>
+static int launch_back(fxc_s2s_plan *p, long long rows, const FixedTables &tn, int norm,
+                       float2 *dst, long long dpitch, hipStream_t st) {
+  const fxc_s2s_config &c = p->cfg;
+  constexpr size_t lds = (size_t)fused_lds_count<S2sFixS, FPN>() * sizeof(float2);
+  static_assert(lds <= (size_t)kLdsComplexCapacity * sizeof(float2), "fused back fits LDS");
+  auto kern = s2s_back_kernel<S2sFixS, FPN, kOut>;
+  FXC_TRY(set_lds(kern, lds));
+  const double inv_s = 1.0 / (double)((float)c.seg * c.delta);
+  hipLaunchKernelGGL(kern, dim3((unsigned)rows), dim3(kS2sThreads), lds, st, p->d_X, p->fpitch,
+                     c.seg, c.idx[0], c.idx[1], c.idx[2], c.idx[3], p->whiten_win, p->fxs, tn,
+                     inv_s, norm, p->runabs_win, c.delta, c.phase_only, dst, dpitch, p->nspec);
+  FXC_HIP(hipGetLastError());
+  return FXC_OK;
+}
+
+// Fused one-channel path: whiten 0 -> front(norm) to the output; 2 ->
+// front(norm) to X, back to the output; 1 -> front to X, back(norm) to the
+// output; 3 -> front to X, back(norm) to X, back to the output.
+static int s2s_process_fused(fxc_s2s_plan *p, const float *d_traces, int nfiles, long long rows,
+                             float2 *d_out, hipStream_t st) {
+  const fxc_s2s_config &c = p->cfg;
+  FXC_CHECK_ARG(rows <= 0x7fffffffLL, "sac2spec: %lld rows exceed one launch", rows);
+  const int norm = c.normalize == 3 ? kLoadRunabs : (c.normalize == 2 ? kLoadOnebit : kLoadPlain);
+  const long long np = p->nspec;
+  switch (c.whiten) {
+    case 0:
+      return launch_front<S2sFixO, true>(p, d_traces, nfiles, rows, p->fxo, norm, d_out, np, st);
+    case 2:
+      FXC_TRY((launch_front<S2sFixS, false>(p, d_traces, nfiles, rows, p->fxs, norm, p->d_X,
+                                           p->fpitch, st)));
+      return launch_back<S2sFixO, true>(p, rows, p->fxo, kLoadPlain, d_out, np, st);
+    case 1:
+      FXC_TRY((launch_front<S2sFixS, false>(p, d_traces, nfiles, rows, p->fxs, kLoadPlain, p->d_X,
+                                           p->fpitch, st)));
+      return launch_back<S2sFixO, true>(p, rows, p->fxo, norm, d_out, np, st);
+    default:
+      FXC_TRY((launch_front<S2sFixS, false>(p, d_traces, nfiles, rows, p->fxs, kLoadPlain, p->d_X,
+                                           p->fpitch, st)));
+      FXC_TRY((launch_back<S2sFixS, false>(p, rows, p->fxs, norm, p->d_X, p->fpitch, st)));
+      return launch_back<S2sFixO, true>(p, rows, p->fxo, kLoadPlain, d_out, np, st);
+  }
+}
+
 // Processes nfiles traces [nfiles][npts] (nfiles = groups * num_ch) for the
 // windows listed in h_steps; writes [nsteps][nfiles][nspec] complex64.
 extern "C" int fxc_s2s_process(fxc_s2s_plan *p, const float *d_traces, int nfiles,
@@ -1105,6 +1602,7 @@ extern "C" int fxc_s2s_process(fxc_s2s_plan *p, const float *d_traces, int nfile
   FXC_TRY(s2s_ensure(p, rows, frames, nsteps));
   FXC_HIP(hipMemcpyAsync(p->d_steps, h_steps, nsteps * sizeof(int), hipMemcpyHostToDevice, st));
   if (p->large) return s2s_process_large(p, d_traces, nfiles, rows, frames, (float2 *)d_out, st);
+  if (p->fused) return s2s_process_fused(p, d_traces, nfiles, rows, (float2 *)d_out, st);
   // K1 prefilter
   const int Mf = c.nfft_filter / 2;
   size_t lds = lds_bytes(Mf + 1, Mf);

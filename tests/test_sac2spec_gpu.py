@@ -124,3 +124,46 @@ def test_long_windows_dt005(gpu, tmp_path, whiten, normalize, bands, nch):
     err = rel_l2(out, want)
     print(f"dt 0.05 whiten={whiten} normalize={normalize} rel_l2={err:.3e}")
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("whiten,normalize,phase_only", [
+    (2, 3, 0), (0, 3, 0), (0, 0, 1), (1, 2, 0), (3, 3, 0), (2, 2, 1), (1, 3, 0), (3, 0, 0)])
+def test_fused_production_path(gpu, tmp_path, monkeypatch, whiten, normalize, phase_only):
+    """The fused one-channel path (front/back kernels on compile-time plans,
+    taken at seg 36000 / 36288 / 37044 with one channel) for every whitening
+    mode and the RUN-ABS, one-bit and no normalisation: against the FP64
+    oracle and against the five-kernel path of the same plan (FXC_S2S_FUSED=0).
+    Three full windows of two stations (a window that runs past the record
+    end is zero padded, and RUN-ABS or one-bit then scale the prefilter's
+    leakage in the padding to O(1) values that FP32 and FP64 round apart)."""
+    import torch
+
+    path = str(tmp_path / "filter.txt")
+    s2s.write_filter_file(path, 0.1, "0.1/2")
+    plan = s2s.plan_lengths(3 * 3600, 0.1, 3600, 3600, 100, s2s.read_filters(path))
+    assert (plan.seg, plan.nfft_filter, plan.nfft_out) == (36000, 36288, 37044)
+    rng = np.random.default_rng(40 + 4 * whiten + normalize)
+    tr = _traces(rng, 2, plan.npts)
+    steps = [0, 1, 2]
+    gp = _gpu_plan(plan, 1, whiten, normalize, phase_only)
+    out = gp.process(torch.from_numpy(tr).to(gpu), steps).cpu().numpy()
+    monkeypatch.setenv("FXC_S2S_FUSED", "0")
+    ref = _gpu_plan(plan, 1, whiten, normalize, phase_only)
+    out5 = ref.process(torch.from_numpy(tr).to(gpu), steps).cpu().numpy()
+    win = s2s.cut_windows(tr.astype(np.float64), plan, steps)
+    want = s2s.process_batch(win.reshape(-1, 1, plan.seg), plan, whiten, normalize,
+                             bool(phase_only)).reshape(len(steps), 2, -1)
+    if phase_only:
+        amp = np.abs(s2s.process_batch(win.reshape(-1, 1, plan.seg), plan, whiten, normalize,
+                                       False)).reshape(want.shape)
+        keep = amp > 1e-3 * amp.max(axis=-1, keepdims=True)
+        err = rel_l2(out[keep], want[keep])
+        print(f"fused phase-only whiten={whiten} normalize={normalize} rel_l2={err:.3e}")
+        assert err < 1e-4
+        assert rel_l2(out[keep], out5[keep]) < 1e-4
+        return
+    err = rel_l2(out, want)
+    print(f"fused whiten={whiten} normalize={normalize} rel_l2={err:.3e} "
+          f"vs five-kernel {rel_l2(out, out5):.3e}")
+    assert err < 1e-5
+    assert rel_l2(out5, want) < 1e-5
