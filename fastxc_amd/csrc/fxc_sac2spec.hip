@@ -731,9 +731,11 @@ __global__ void s2sL_out_kernel(float2 *__restrict__ out, int nspec, float dt, i
 // taken block-major) on odd spans (81, 9; 75, 5; 63, 9): neighbouring lanes
 // then address blocks an odd number of words apart, which spreads them over
 // the LDS banks, where even spans such as 108 put 8 to 16 lanes on one bank.
+// Four passes measured faster than five better-balanced ones (6, 6, 8, 9, 7
+// and 18, 8, 5, 5, 5: 15.2 against 14.8 ms for 1000 x 24 station-windows).
 using S2sFixF = FixedPlan<18144, 16, 14, 9, 9>;   // prefilter: nfft 36288
 using S2sFixS = FixedPlan<18000, 16, 15, 15, 5>;  // window: seg 36000
-using S2sFixO = FixedPlan<18522, 6, 7, 7, 7, 9>;  // output: nfft 37044
+using S2sFixO = FixedPlan<18522, 6, 7, 7, 7, 9>;   // output: nfft 37044
 static_assert(S2sFixF::valid() && S2sFixS::valid() && S2sFixO::valid(), "plans");
 constexpr int kFusedPer = 37;   // RUN-ABS outputs per thread (odd: conflict-free LDS reads)
 constexpr int kFusedPerB = 19;  // whitening bins per thread
@@ -747,7 +749,7 @@ struct FixedTables {
 };
 
 template <class FP>
-__device__ inline void stage_fixed(float2 *twl_lds, float2 *fold_lds, const FixedTables &t) {
+__device__ __forceinline__ void stage_fixed(float2 *twl_lds, float2 *fold_lds, const FixedTables &t) {
   for (int i = threadIdx.x; i < fixed_lds_tw_count<FP>(); i += blockDim.x) twl_lds[i] = t.twl[i];
   for (int i = threadIdx.x; i < fold_lds_count(FP::kM); i += blockDim.x) fold_lds[i] = t.fold[i];
 }
@@ -755,16 +757,18 @@ __device__ inline void stage_fixed(float2 *twl_lds, float2 *fold_lds, const Fixe
 // fold_lds_count as a constant expression
 constexpr int fused_fold_count(int M) { return kTwFine + (M + 1 + kTwFine - 1) / kTwFine; }
 
+// [data][LDS twiddles A][LDS twiddles B][fold A][fold B][block sums: kS2sThreads floats]
 template <class FPA, class FPB>
 constexpr int fused_lds_count() {
   return (FPA::kM > FPB::kM ? FPA::kM : FPB::kM) + 1 + fixed_lds_tw_count<FPA>() +
-         fixed_lds_tw_count<FPB>() + fused_fold_count(FPA::kM) + fused_fold_count(FPB::kM);
+         fixed_lds_tw_count<FPB>() + fused_fold_count(FPA::kM) + fused_fold_count(FPB::kM) +
+         kS2sThreads / 2;
 }
 
 // After fixed_fft_fwd of the packed real row: X[k], k = 0..M, to st(k, X[k])
 // (the unfold of r2c_unfold at the plan's digit-reversed positions).
 template <class FP, class St>
-__device__ inline void fixed_r2c_emit(const float2 *lds, const float2 *fold, const St &st) {
+__device__ __forceinline__ void fixed_r2c_emit(const float2 *lds, const float2 *fold, const St &st) {
   constexpr int M = FP::kM;
   for (int k = threadIdx.x; k <= M; k += blockDim.x) {
     const float2 a = lds[fixed_pos<FP>(k == M ? 0 : k)];
@@ -781,7 +785,7 @@ __device__ inline void fixed_r2c_emit(const float2 *lds, const float2 *fold, con
 // spectral_pairs at the plan's digit-reversed positions: f(k, X[k], X[M-k])
 // -> (Y[k], Y[M-k]), refolded in place for fixed_fft_inv.
 template <class FP, class F>
-__device__ inline void fixed_spectral_pairs(float2 *lds, const float2 *fold, const F &f) {
+__device__ __forceinline__ void fixed_spectral_pairs(float2 *lds, const float2 *fold, const F &f) {
   constexpr int M = FP::kM;
   for (int k = threadIdx.x; 2 * k <= M; k += blockDim.x) {
     const int pk = fixed_pos<FP>(k);
@@ -828,7 +832,7 @@ __device__ inline void fixed_spectral_pairs(float2 *lds, const float2 *fold, con
 // for the kPer contiguous outputs t0 = threadIdx.x * kPer + m of this
 // thread, each handed to use(m, t, w_t).
 template <int kPer, class A, class U>
-__device__ inline void sliding_mean(int n, int win, const A &a, const U &use) {
+__device__ __forceinline__ void sliding_mean(int n, int win, const A &a, const U &use) {
   const int t0 = (int)threadIdx.x * kPer;
   if (t0 >= n) return;
   int start = t0 - win / 2, end = start + win - 1;
@@ -850,20 +854,68 @@ __device__ inline void sliding_mean(int n, int win, const A &a, const U &use) {
   }
 }
 
+// sliding_mean whose first window is assembled from block sums: each thread
+// sums its own kPer values (FP64, stored to bs[] in LDS as float), and the
+// first window of a thread adds the blocks it covers whole and reads only
+// its two partial ends (about win/kPer + 2 kPer values instead of win).  The
+// window sum then carries the float rounding of its whole blocks (relative
+// 6e-8 each); the slide itself stays FP64.  Every thread must call it.
+template <int kPer, class A, class U>
+__device__ __forceinline__ void sliding_mean_blocked(int n, int win, const A &a, const U &use,
+                                                     float *bs) {
+  const int t0 = (int)threadIdx.x * kPer;
+  {
+    double b = 0.0;
+#pragma unroll
+    for (int m = 0; m < kPer; ++m)
+      if (t0 + m < n) b += (double)a(t0 + m);
+    bs[threadIdx.x] = (float)b;
+  }
+  __syncthreads();
+  if (t0 >= n) return;
+  int start = t0 - win / 2, end = start + win - 1;
+  const int lo = max(start, 0), hi = min(end, n - 1);
+  const int ub = (lo + kPer - 1) / kPer;  // first whole block
+  const int ue = (hi + 1) / kPer;         // blocks [ub, ue) lie inside [lo, hi]
+  double sum = 0.0;
+  if (ub < ue) {
+    for (int i = lo; i < ub * kPer; ++i) sum += (double)a(i);
+    for (int u = ub; u < ue; ++u) sum += (double)bs[u];
+    for (int i = ue * kPer; i <= hi; ++i) sum += (double)a(i);
+  } else {
+    for (int i = lo; i <= hi; ++i) sum += (double)a(i);
+  }
+  const double inv = 1.0 / (double)win;
+#pragma unroll
+  for (int m = 0; m < kPer; ++m) {
+    const int t = t0 + m;
+    if (t < n) {
+      float w = (float)(inv * sum);
+      if (!isfinite(w) || w < kMinVal) w = kMinVal;
+      use(m, t, w);
+      if (start >= 0 && start < n) sum -= (double)a(start);
+      ++start;
+      ++end;
+      if (end >= 0 && end < n) sum += (double)a(end);
+    }
+  }
+}
+
 // Time-domain normalisation of the row x[0..n) in LDS (one channel):
 // RUN-ABS x / w with w the running mean of |x| (0 for an invalid weight,
 // non-finite -> 0, clip to +-MAXVAL), or one-bit sign(x)
-// (normalization.cu:91-213, compute.cu:144-176).
-__device__ inline void normalise_row(float *x, int n, int mode, int win) {
+// (normalization.cu:91-213, compute.cu:144-176).  bs: kS2sThreads floats.
+__device__ __forceinline__ void normalise_row(float *x, int n, int mode, int win, float *bs) {
   if (mode == kLoadRunabs) {
     float y[kFusedPer];
-    sliding_mean<kFusedPer>(n, win, [&](int i) { return fabsf(x[i]); },
+    sliding_mean_blocked<kFusedPer>(n, win, [&](int i) { return fabsf(x[i]); },
                             [&](int m, int t, float d) {
                               float v = x[t];
                               v = (!isfinite(d) || d < kMinValD) ? 0.f : v / d;
                               if (!isfinite(v)) v = 0.f;
                               y[m] = v > kMaxVal ? kMaxVal : (v < -kMaxVal ? -kMaxVal : v);
-                            });
+                            },
+                            bs);
     __syncthreads();
     const int t0 = (int)threadIdx.x * kFusedPer;
 #pragma unroll
@@ -884,7 +936,7 @@ __device__ inline void normalise_row(float *x, int n, int mode, int win) {
 // (compute.cu:34-58, core/preprocess.cu:23-105).  Every load of the window
 // is issued before the first LDS store, so their latencies overlap.
 template <int kTwoM>
-__device__ inline void preprocess_row(float *ldsf, const float *__restrict__ src, long long off,
+__device__ __forceinline__ void preprocess_row(float *ldsf, const float *__restrict__ src, long long off,
                                       int npts, int seg, int taper_size) {
   constexpr int kPer = (kTwoM + kS2sThreads - 1) / kS2sThreads;
   double s = 0.0, unused1 = 0.0, unused2 = 0.0;
@@ -937,7 +989,7 @@ __device__ inline void preprocess_row(float *ldsf, const float *__restrict__ src
 // R2C(2 MN) x dt, sanitised, to X (whitening spectrum, k <= MN) or to the
 // output spectrum (k < nspec, optional phase-only).
 template <class FPN, bool kOut>
-__device__ inline void fused_emit(float2 *lds, int seg, const float2 *__restrict__ tw0n,
+__device__ __forceinline__ void fused_emit(float2 *lds, int seg, const float2 *__restrict__ tw0n,
                                   const float2 *twln, const float2 *foldn, float dt,
                                   int phase_only, float2 *__restrict__ dst, int nspec) {
   float *ldsf = (float *)lds;
@@ -972,6 +1024,7 @@ s2s_front_kernel(const float *__restrict__ traces, int npts, int nfiles,
   float2 *twln = twlf + fixed_lds_tw_count<FPF>();
   float2 *foldf = twln + fixed_lds_tw_count<FPN>();
   float2 *foldn = foldf + fold_lds_count(FPF::kM);
+  float *bs = (float *)(foldn + fold_lds_count(FPN::kM));
   stage_fixed<FPF>(twlf, foldf, tf);
   stage_fixed<FPN>(twln, foldn, tn);
   float *ldsf = (float *)lds;
@@ -992,7 +1045,7 @@ s2s_front_kernel(const float *__restrict__ traces, int npts, int nfiles,
   fixed_fft_inv<FPF, kS2sThreads>(lds, tf.tw0, twlf);
   for (int i = threadIdx.x; i < seg; i += blockDim.x) ldsf[i] = (float)((double)ldsf[i] * inv_weight);
   __syncthreads();
-  normalise_row(ldsf, seg, norm_mode, runabs_win);
+  normalise_row(ldsf, seg, norm_mode, runabs_win, bs);
   fused_emit<FPN, kOut>(lds, seg, tn.tw0, twln, foldn, dt, phase_only,
                         dst + (size_t)row * dpitch, nspec);
 }
@@ -1010,6 +1063,7 @@ s2s_back_kernel(float2 *X, long long fpitch, int seg, int i1, int i2, int i3, in
   float2 *twln = twls + fixed_lds_tw_count<FPS>();
   float2 *folds = twln + fixed_lds_tw_count<FPN>();
   float2 *foldn = folds + fold_lds_count(FPS::kM);
+  float *bs = (float *)(foldn + fold_lds_count(FPN::kM));
   stage_fixed<FPS>(twls, folds, ts);
   stage_fixed<FPN>(twln, foldn, tn);
   const int row = blockIdx.x;
@@ -1090,7 +1144,7 @@ s2s_back_kernel(float2 *X, long long fpitch, int seg, int i1, int i2, int i3, in
   for (int i = threadIdx.x; i < seg; i += blockDim.x)
     ldsf[i] = (float)((double)ldsf[i] * inv_weight_seg);
   __syncthreads();
-  normalise_row(ldsf, seg, norm_mode, runabs_win);
+  normalise_row(ldsf, seg, norm_mode, runabs_win, bs);
   fused_emit<FPN, kOut>(lds, seg, tn.tw0, twln, foldn, dt, phase_only, dst + (size_t)row * dpitch,
                         nspec);
 }
