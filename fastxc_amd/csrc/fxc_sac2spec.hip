@@ -1053,7 +1053,8 @@ s2s_front_kernel(const float *__restrict__ traces, int npts, int nfiles,
 template <class FPS, class FPN, bool kOut>
 __global__ void __launch_bounds__(kS2sThreads)
 s2s_back_kernel(float2 *X, long long fpitch, int seg, int i1, int i2, int i3, int i4,
-                int whiten_win, FixedTables ts, FixedTables tn, double inv_weight_seg,
+                int whiten_win, const float *__restrict__ staper, FixedTables ts, FixedTables tn,
+                double inv_weight_seg,
                 int norm_mode, int runabs_win, float dt, int phase_only, float2 *dst,
                 long long dpitch, int nspec) {
   extern __shared__ float2 lds[];
@@ -1085,9 +1086,9 @@ s2s_back_kernel(float2 *X, long long fpitch, int seg, int i1, int i2, int i3, in
   }
   __syncthreads();
   // whitening weights (running mean of |X| over the bins, one channel) and
-  // X / w x taper, sanitised (spectrum.cu:205-300)
+  // X / w x taper (staper[k] = spec_taper(k, i1, i2, i3, min(i4, seg/2+1)),
+  // a table the plan fills), sanitised (spectrum.cu:205-300)
   const int nbin = Ms + 1;
-  const int t4 = i4 < nbin ? i4 : nbin;
   float wk[kFusedPerB];
   sliding_mean<kFusedPerB>(
       nbin, whiten_win,
@@ -1106,7 +1107,7 @@ s2s_back_kernel(float2 *X, long long fpitch, int seg, int i1, int i2, int i3, in
         const float d = wk[m];
         float2 v = lds[k];
         v = (!isfinite(d) || d < kMinValD) ? make_float2(0.f, 0.f) : make_float2(v.x / d, v.y / d);
-        v = cscale(v, spec_taper(k, i1, i2, i3, t4));
+        v = cscale(v, staper[k]);
         if (!cfinite(v)) v = make_float2(0.f, 0.f);
         lds[k] = v;
       }
@@ -1184,6 +1185,7 @@ struct fxc_s2s_plan {
   // fused one-channel path at the production lengths (s2s_front/back_kernel)
   bool fused = false;
   float2 *d_fx = nullptr;  // pass-0 and LDS base twiddles of the three plans
+  float *d_staper = nullptr;  // spectral taper factor of each whitening bin
   FixedTables fxf{}, fxs{}, fxo{};
 };
 
@@ -1238,6 +1240,23 @@ static int s2s_fused_setup(fxc_s2s_plan *p) {
   p->fxf = {p->d_fx + off[0], p->d_fx + off[1], p->tf.fold};
   p->fxs = {p->d_fx + off[2], p->d_fx + off[3], p->ts.fold};
   p->fxo = {p->d_fx + off[4], p->d_fx + off[5], p->to.fold};
+  // the spectral taper of the whitening bins, as spec_taper forms it
+  // (double cosine, rounded to float) -- a table instead of a cosine per bin
+  {
+    const int nbin = c.seg / 2 + 1;
+    const int i1 = c.idx[0], i2 = c.idx[1], i3 = c.idx[2], i4 = std::min(c.idx[3], nbin);
+    std::vector<float> tp((size_t)nbin);
+    for (int k = 0; k < nbin; ++k) {
+      float f = 1.f;
+      if (k < i1) f = 0.f;
+      else if (k < i2) f = (float)((1.0 - std::cos(M_PI / (i2 - i1) * (k - i1))) / 2.0);
+      else if (k >= i3 && k < i4) f = (float)((1.0 + std::cos(M_PI / (i4 - i3) * (k - i3))) / 2.0);
+      else if (k >= i4) f = 0.f;
+      tp[(size_t)k] = f;
+    }
+    FXC_TRY(dmalloc((void **)&p->d_staper, tp.size() * sizeof(float)));
+    FXC_HIP(hipMemcpy(p->d_staper, tp.data(), tp.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
   p->fused = true;
   return FXC_OK;
 }
@@ -1332,7 +1351,7 @@ extern "C" int fxc_s2s_plan_destroy(fxc_s2s_plan *p) {
   for (void *q : {(void *)p->d_resp, (void *)p->d_x, (void *)p->d_w, (void *)p->d_xb,
                   (void *)p->d_total, (void *)p->d_X, (void *)p->d_base, (void *)p->d_amp,
                   (void *)p->d_ww, (void *)p->d_steps, (void *)p->d_L, (void *)p->d_Lw,
-                  (void *)p->d_fx})
+                  (void *)p->d_fx, (void *)p->d_staper})
     dfree(q);
   delete p;
   return FXC_OK;
@@ -1606,8 +1625,9 @@ static int launch_back(fxc_s2s_plan *p, long long rows, const FixedTables &tn, i
   FXC_TRY(set_lds(kern, lds));
   const double inv_s = 1.0 / (double)((float)c.seg * c.delta);
   hipLaunchKernelGGL(kern, dim3((unsigned)rows), dim3(kS2sThreads), lds, st, p->d_X, p->fpitch,
-                     c.seg, c.idx[0], c.idx[1], c.idx[2], c.idx[3], p->whiten_win, p->fxs, tn,
-                     inv_s, norm, p->runabs_win, c.delta, c.phase_only, dst, dpitch, p->nspec);
+                     c.seg, c.idx[0], c.idx[1], c.idx[2], c.idx[3], p->whiten_win, p->d_staper,
+                     p->fxs, tn, inv_s, norm, p->runabs_win, c.delta, c.phase_only, dst, dpitch,
+                     p->nspec);
   FXC_HIP(hipGetLastError());
   return FXC_OK;
 }
