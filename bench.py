@@ -80,6 +80,8 @@ def parse():
     ap.add_argument("--stream-only", action="store_true", help="measure the streaming loop only")
     ap.add_argument("--no-sourcepack", action="store_true",
                     help="skip the SourcePack indexer measurement")
+    ap.add_argument("--xc-file-to-file", action="store_true",
+                    help="time bin/xc_fast file to file at 1000 stations x 24 windows only")
     ap.add_argument("--no-band", action="store_true",
                     help="skip the band-limited spec-mode lines (fxc_xc_set_band)")
     ap.add_argument("--no-batch", action="store_true",
@@ -221,6 +223,66 @@ def band_measure(args, src, rec, out, check_pairs, check_rows):
     torch.cuda.empty_cache()
     return {"spectra": "sac2spec of seeded noise (RUN-ABS, whitening after, 0.1-2 Hz)",
             "lines": lines}
+
+
+def xc_file_to_file(nsta=1000, nstep=24, nspec=18523, half_cc=1000, precision="bf16x3"):
+    """bin/xc_fast file to file on one timestamp: a StepPack v3 batch of nsta
+    stations x nstep windows x nspec bins (seeded complex Gaussian, 3.56 GB at
+    the default size) and an allowed-path table of every pair i < j in, the
+    xcpack parts (SAC records + TSV sidecars) out, on the box's scratch disk.
+    The wall time of the executable covers the StepPack read, the upload, the
+    correlation, the result download and the xcpack writes."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    import numpy as np
+    from tests import fxformats as ff
+
+    exe = os.path.join(ROOT, "bin", "xc_fast")
+    if not os.path.exists(exe):
+        return {"skipped": "bin/xc_fast not built"}
+    tmp = tempfile.mkdtemp(prefix="fxc_f2f_")
+    try:
+        rng = np.random.default_rng(21)
+        spec = np.empty((nstep, nsta, nspec), dtype=np.complex64)
+        for w in range(nstep):
+            spec[w].real = rng.standard_normal((nsta, nspec), dtype=np.float32)
+            spec[w].imag = rng.standard_normal((nsta, nspec), dtype=np.float32)
+        nodes = [dict(timestamp="2020.123.0000", nsl_id="XX.S%04d." % i, gnsl_id=i + 1,
+                      stla=30.0 + i * 0.01, stlo=100.0 - i * 0.01, network="XX",
+                      station="S%04d" % i, location="", component="BHZ") for i in range(nsta)]
+        t0 = time.perf_counter()
+        sp_dir = os.path.join(tmp, "stepack")
+        ff.write_stepack(sp_dir, 0, 0, nodes, spec, 0.1, 1.0 / (2 * (nspec - 1) * 0.1),
+                         start_group=0, group_count=nsta)
+        del spec
+        recs = [(a, b, 0.01, 1.0, 10.0, 20.0) for a in range(1, nsta + 1)
+                for b in range(a + 1, nsta + 1)]
+        paths = os.path.join(tmp, "allowed_paths.tsv")
+        ff.write_allowed_paths(paths, recs)
+        prep = time.perf_counter() - t0
+        out = os.path.join(tmp, "ncf")
+        t0 = time.perf_counter()
+        r = subprocess.run([exe, "-I", sp_dir, "-P", paths, "-O", out, "-C",
+                            str(half_cc * 0.1), "--precision", precision],
+                           capture_output=True, text=True, timeout=900)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            return {"error": r.stderr[-500:]}
+        xdir = os.path.join(out, "xcpack")
+        nbytes = sum(os.path.getsize(os.path.join(xdir, f)) for f in os.listdir(xdir))
+        npairs = len(recs)
+        return {"stations": nsta, "windows": nstep, "nspec": nspec, "pairs": npairs,
+                "precision": precision, "wall_s": round(wall, 3),
+                "pair_windows_per_s": round(npairs * nstep / wall, 1),
+                "records_per_s": round(npairs / wall, 1),
+                "stepack_bytes": os.path.getsize(os.path.join(sp_dir, "w000.b000000.stepack")),
+                "xcpack_dir_bytes": nbytes, "input_write_s": round(prep, 1),
+                "detail": "bin/xc_fast file to file on the box's scratch disk: StepPack read, "
+                          "upload, correlation, download, xcpack SAC records and TSVs written"}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 def algorithmic_bytes(args, npairs):
@@ -970,6 +1032,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps({"batch": batch_measure()}), flush=True)
+        return
+    if args.xc_file_to_file:
+        print(json.dumps({"xc_file_to_file": xc_file_to_file()}), flush=True)
         return
     if args.stream_only:
         print(json.dumps({"stream": stream_all(rates, args.stream_seconds)}), flush=True)

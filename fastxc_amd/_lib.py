@@ -97,6 +97,7 @@ _SIGNATURES = {
     "fxc_plan_paths_free": (c_int, [c_void_p]),
     "fxc_s2s_plan_create": (c_int, [POINTER(c_void_p), c_void_p]),
     "fxc_s2s_plan_destroy": (c_int, [c_void_p]),
+    "fxc_s2s_set_sums": (c_int, [c_void_p, c_int]),
     "fxc_s2s_plan_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int),
                                   POINTER(c_int)]),
     "fxc_s2s_process": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),

@@ -827,49 +827,71 @@ __device__ __forceinline__ void fixed_spectral_pairs(float2 *lds, const float2 *
   __syncthreads();
 }
 
+// Window-sum accumulators of the running means.  Acc64: an FP64 sum, the
+// reference's arithmetic (kernels/real_matrix.cu:155-191).  AccComp (spec
+// mode, fxc_s2s_set_sums): a float sum with Neumaier compensation, so the
+// running means use no FP64; the window mean is (s + c) * (1/win) in float.
+struct Acc64 {
+  double s = 0.0;
+  __device__ __forceinline__ void add(float x) { s += (double)x; }
+  __device__ __forceinline__ void sub(float x) { s -= (double)x; }
+  __device__ __forceinline__ float mean(int win) const { return (float)((1.0 / (double)win) * s); }
+  __device__ __forceinline__ float total() const { return (float)s; }
+};
+struct AccComp {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float x) {
+    const float t = s + x;
+    c += fabsf(s) >= fabsf(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ void sub(float x) { add(-x); }
+  __device__ __forceinline__ float mean(int win) const { return (s + c) * (1.f / (float)win); }
+  __device__ __forceinline__ float total() const { return s + c; }
+};
+
 // Running mean w_t = (1/win) sum_{i = t - win/2}^{t - win/2 + win - 1} a(i)
-// over i in [0, n) (FP64 window sum, float result, clamped to >= MINVAL),
+// over i in [0, n) (window sum in Acc, float result, clamped to >= MINVAL),
 // for the kPer contiguous outputs t0 = threadIdx.x * kPer + m of this
 // thread, each handed to use(m, t, w_t).
-template <int kPer, class A, class U>
+template <int kPer, class Acc, class A, class U>
 __device__ __forceinline__ void sliding_mean(int n, int win, const A &a, const U &use) {
   const int t0 = (int)threadIdx.x * kPer;
   if (t0 >= n) return;
   int start = t0 - win / 2, end = start + win - 1;
-  const double inv = 1.0 / (double)win;
-  double sum = 0.0;
-  for (int i = max(start, 0); i <= min(end, n - 1); ++i) sum += (double)a(i);
+  Acc sum;
+  for (int i = max(start, 0); i <= min(end, n - 1); ++i) sum.add(a(i));
 #pragma unroll
   for (int m = 0; m < kPer; ++m) {
     const int t = t0 + m;
     if (t < n) {
-      float w = (float)(inv * sum);
+      float w = sum.mean(win);
       if (!isfinite(w) || w < kMinVal) w = kMinVal;
       use(m, t, w);
-      if (start >= 0 && start < n) sum -= (double)a(start);
+      if (start >= 0 && start < n) sum.sub(a(start));
       ++start;
       ++end;
-      if (end >= 0 && end < n) sum += (double)a(end);
+      if (end >= 0 && end < n) sum.add(a(end));
     }
   }
 }
 
 // sliding_mean whose first window is assembled from block sums: each thread
-// sums its own kPer values (FP64, stored to bs[] in LDS as float), and the
-// first window of a thread adds the blocks it covers whole and reads only
-// its two partial ends (about win/kPer + 2 kPer values instead of win).  The
-// window sum then carries the float rounding of its whole blocks (relative
-// 6e-8 each); the slide itself stays FP64.  Every thread must call it.
-template <int kPer, class A, class U>
+// sums its own kPer values (stored to bs[] in LDS as float), and the first
+// window of a thread adds the blocks it covers whole and reads only its two
+// partial ends (about win/kPer + 2 kPer values instead of win).  The window
+// sum then carries the float rounding of its whole blocks (relative 6e-8
+// each); the slide itself runs in Acc.  Every thread must call it.
+template <int kPer, class Acc, class A, class U>
 __device__ __forceinline__ void sliding_mean_blocked(int n, int win, const A &a, const U &use,
                                                      float *bs) {
   const int t0 = (int)threadIdx.x * kPer;
   {
-    double b = 0.0;
+    Acc b;
 #pragma unroll
     for (int m = 0; m < kPer; ++m)
-      if (t0 + m < n) b += (double)a(t0 + m);
-    bs[threadIdx.x] = (float)b;
+      if (t0 + m < n) b.add(a(t0 + m));
+    bs[threadIdx.x] = b.total();
   }
   __syncthreads();
   if (t0 >= n) return;
@@ -877,26 +899,25 @@ __device__ __forceinline__ void sliding_mean_blocked(int n, int win, const A &a,
   const int lo = max(start, 0), hi = min(end, n - 1);
   const int ub = (lo + kPer - 1) / kPer;  // first whole block
   const int ue = (hi + 1) / kPer;         // blocks [ub, ue) lie inside [lo, hi]
-  double sum = 0.0;
+  Acc sum;
   if (ub < ue) {
-    for (int i = lo; i < ub * kPer; ++i) sum += (double)a(i);
-    for (int u = ub; u < ue; ++u) sum += (double)bs[u];
-    for (int i = ue * kPer; i <= hi; ++i) sum += (double)a(i);
+    for (int i = lo; i < ub * kPer; ++i) sum.add(a(i));
+    for (int u = ub; u < ue; ++u) sum.add(bs[u]);
+    for (int i = ue * kPer; i <= hi; ++i) sum.add(a(i));
   } else {
-    for (int i = lo; i <= hi; ++i) sum += (double)a(i);
+    for (int i = lo; i <= hi; ++i) sum.add(a(i));
   }
-  const double inv = 1.0 / (double)win;
 #pragma unroll
   for (int m = 0; m < kPer; ++m) {
     const int t = t0 + m;
     if (t < n) {
-      float w = (float)(inv * sum);
+      float w = sum.mean(win);
       if (!isfinite(w) || w < kMinVal) w = kMinVal;
       use(m, t, w);
-      if (start >= 0 && start < n) sum -= (double)a(start);
+      if (start >= 0 && start < n) sum.sub(a(start));
       ++start;
       ++end;
-      if (end >= 0 && end < n) sum += (double)a(end);
+      if (end >= 0 && end < n) sum.add(a(end));
     }
   }
 }
@@ -905,10 +926,11 @@ __device__ __forceinline__ void sliding_mean_blocked(int n, int win, const A &a,
 // RUN-ABS x / w with w the running mean of |x| (0 for an invalid weight,
 // non-finite -> 0, clip to +-MAXVAL), or one-bit sign(x)
 // (normalization.cu:91-213, compute.cu:144-176).  bs: kS2sThreads floats.
+template <class Acc>
 __device__ __forceinline__ void normalise_row(float *x, int n, int mode, int win, float *bs) {
   if (mode == kLoadRunabs) {
     float y[kFusedPer];
-    sliding_mean_blocked<kFusedPer>(n, win, [&](int i) { return fabsf(x[i]); },
+    sliding_mean_blocked<kFusedPer, Acc>(n, win, [&](int i) { return fabsf(x[i]); },
                             [&](int m, int t, float d) {
                               float v = x[t];
                               v = (!isfinite(d) || d < kMinValD) ? 0.f : v / d;
@@ -1016,7 +1038,7 @@ __global__ void __launch_bounds__(kS2sThreads)
 s2s_front_kernel(const float *__restrict__ traces, int npts, int nfiles,
                  const int *__restrict__ steps, int shift, int seg, int taper_size,
                  FixedTables tf, FixedTables tn, const float *__restrict__ resp0, float dt,
-                 double inv_weight, int norm_mode, int runabs_win, int phase_only,
+                 double inv_weight, int norm_mode, int runabs_win, int phase_only, int comp_sums,
                  float2 *__restrict__ dst, long long dpitch, int nspec) {
   extern __shared__ float2 lds[];
   constexpr int kData = (FPF::kM > FPN::kM ? FPF::kM : FPN::kM) + 1;
@@ -1045,7 +1067,10 @@ s2s_front_kernel(const float *__restrict__ traces, int npts, int nfiles,
   fixed_fft_inv<FPF, kS2sThreads>(lds, tf.tw0, twlf);
   for (int i = threadIdx.x; i < seg; i += blockDim.x) ldsf[i] = (float)((double)ldsf[i] * inv_weight);
   __syncthreads();
-  normalise_row(ldsf, seg, norm_mode, runabs_win, bs);
+  if (comp_sums)
+    normalise_row<AccComp>(ldsf, seg, norm_mode, runabs_win, bs);
+  else
+    normalise_row<Acc64>(ldsf, seg, norm_mode, runabs_win, bs);
   fused_emit<FPN, kOut>(lds, seg, tn.tw0, twln, foldn, dt, phase_only,
                         dst + (size_t)row * dpitch, nspec);
 }
@@ -1055,8 +1080,8 @@ __global__ void __launch_bounds__(kS2sThreads)
 s2s_back_kernel(float2 *X, long long fpitch, int seg, int i1, int i2, int i3, int i4,
                 int whiten_win, const float *__restrict__ staper, FixedTables ts, FixedTables tn,
                 double inv_weight_seg,
-                int norm_mode, int runabs_win, float dt, int phase_only, float2 *dst,
-                long long dpitch, int nspec) {
+                int norm_mode, int runabs_win, float dt, int phase_only, int comp_sums,
+                float2 *dst, long long dpitch, int nspec) {
   extern __shared__ float2 lds[];
   constexpr int kData = (FPS::kM > FPN::kM ? FPS::kM : FPN::kM) + 1;
   constexpr int Ms = FPS::kM;
@@ -1090,13 +1115,15 @@ s2s_back_kernel(float2 *X, long long fpitch, int seg, int i1, int i2, int i3, in
   // a table the plan fills), sanitised (spectrum.cu:205-300)
   const int nbin = Ms + 1;
   float wk[kFusedPerB];
-  sliding_mean<kFusedPerB>(
-      nbin, whiten_win,
-      [&](int k) {
-        const float2 z = lds[k];
-        return sqrtf(z.x * z.x + z.y * z.y);
-      },
-      [&](int m, int, float d) { wk[m] = d; });
+  auto absx = [&](int k) {
+    const float2 z = lds[k];
+    return sqrtf(z.x * z.x + z.y * z.y);
+  };
+  auto keep = [&](int m, int, float d) { wk[m] = d; };
+  if (comp_sums)
+    sliding_mean<kFusedPerB, AccComp>(nbin, whiten_win, absx, keep);
+  else
+    sliding_mean<kFusedPerB, Acc64>(nbin, whiten_win, absx, keep);
   __syncthreads();  // every |X| read is done: each thread rewrites its own bins
   {
     const int k0 = (int)threadIdx.x * kFusedPerB;
@@ -1145,7 +1172,10 @@ s2s_back_kernel(float2 *X, long long fpitch, int seg, int i1, int i2, int i3, in
   for (int i = threadIdx.x; i < seg; i += blockDim.x)
     ldsf[i] = (float)((double)ldsf[i] * inv_weight_seg);
   __syncthreads();
-  normalise_row(ldsf, seg, norm_mode, runabs_win, bs);
+  if (comp_sums)
+    normalise_row<AccComp>(ldsf, seg, norm_mode, runabs_win, bs);
+  else
+    normalise_row<Acc64>(ldsf, seg, norm_mode, runabs_win, bs);
   fused_emit<FPN, kOut>(lds, seg, tn.tw0, twln, foldn, dt, phase_only, dst + (size_t)row * dpitch,
                         nspec);
 }
@@ -1186,6 +1216,7 @@ struct fxc_s2s_plan {
   bool fused = false;
   float2 *d_fx = nullptr;  // pass-0 and LDS base twiddles of the three plans
   float *d_staper = nullptr;  // spectral taper factor of each whitening bin
+  int comp_sums = 0;          // spec mode: compensated-FP32 running means (fxc_s2s_set_sums)
   FixedTables fxf{}, fxs{}, fxo{};
 };
 
@@ -1354,6 +1385,16 @@ extern "C" int fxc_s2s_plan_destroy(fxc_s2s_plan *p) {
                   (void *)p->d_fx, (void *)p->d_staper})
     dfree(q);
   delete p;
+  return FXC_OK;
+}
+
+extern "C" int fxc_s2s_set_sums(fxc_s2s_plan *p, int mode) {
+  FXC_CHECK_ARG(p && (mode == 0 || mode == 1), "fxc_s2s_set_sums: invalid arguments");
+  if (mode == 1 && !p->fused) {
+    set_error("fxc_s2s_set_sums: compensated-FP32 running means need the fused one-channel path");
+    return FXC_ERR_UNSUPPORTED;
+  }
+  p->comp_sums = mode;
   return FXC_OK;
 }
 
@@ -1610,7 +1651,8 @@ static int launch_front(fxc_s2s_plan *p, const float *d_traces, int nfiles, long
   const double inv_f = 1.0 / (double)((float)c.nfft_filter * c.delta);
   hipLaunchKernelGGL(kern, dim3((unsigned)rows), dim3(kS2sThreads), lds, st, d_traces, c.npts,
                      nfiles, p->d_steps, c.shift, c.seg, p->taper_size, p->fxf, tn, p->d_resp,
-                     c.delta, inv_f, norm, p->runabs_win, c.phase_only, dst, dpitch, p->nspec);
+                     c.delta, inv_f, norm, p->runabs_win, c.phase_only, p->comp_sums, dst, dpitch,
+                     p->nspec);
   FXC_HIP(hipGetLastError());
   return FXC_OK;
 }
@@ -1626,8 +1668,8 @@ static int launch_back(fxc_s2s_plan *p, long long rows, const FixedTables &tn, i
   const double inv_s = 1.0 / (double)((float)c.seg * c.delta);
   hipLaunchKernelGGL(kern, dim3((unsigned)rows), dim3(kS2sThreads), lds, st, p->d_X, p->fpitch,
                      c.seg, c.idx[0], c.idx[1], c.idx[2], c.idx[3], p->whiten_win, p->d_staper,
-                     p->fxs, tn, inv_s, norm, p->runabs_win, c.delta, c.phase_only, dst, dpitch,
-                     p->nspec);
+                     p->fxs, tn, inv_s, norm, p->runabs_win, c.delta, c.phase_only, p->comp_sums,
+                     dst, dpitch, p->nspec);
   FXC_HIP(hipGetLastError());
   return FXC_OK;
 }

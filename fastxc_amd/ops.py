@@ -450,6 +450,11 @@ class S2sPlan:
         except Exception:
             pass
 
+    def set_sums(self, compensated: bool) -> None:
+        """Spec mode: compensated-FP32 running means instead of FP64 window
+        sums (fused one-channel path only)."""
+        check(lib().fxc_s2s_set_sums(self._h, int(bool(compensated))), "fxc_s2s_set_sums")
+
     def process(self, traces: torch.Tensor, steps, out: torch.Tensor | None = None) -> torch.Tensor:
         """traces: float32 [nfiles, npts] (device); steps: window indices.
         Returns complex64 [len(steps), nfiles, nspec]."""

@@ -179,6 +179,13 @@ int fxc_s2s_plan_info(const fxc_s2s_plan *plan, int *nspec, int *taper_size, int
  * d_out: [nsteps][nfiles][nspec] complex64, the StepPack payload order. */
 int fxc_s2s_process(fxc_s2s_plan *plan, const float *d_traces, int nfiles, const int *h_steps,
                     int nsteps, float *d_out, void *stream);
+/* Running-mean arithmetic (RUN-ABS weights and whitening weights): 0 = FP64
+ * window sums as the reference (smoothRowsRollingKernel,
+ * native/sac2spec/kernels/real_matrix.cu:155-191; the default); 1 = spec
+ * mode, float window sums with Neumaier compensation (no FP64 in the running
+ * means).  Mode 1 needs the fused one-channel path (FXC_ERR_UNSUPPORTED
+ * otherwise). */
+int fxc_s2s_set_sums(fxc_s2s_plan *plan, int mode);
 
 /* --------------------------------------------------------------- stacks --
  * Phase-weighted stacks of cross-correlations of length n (2H+1).  Groups of

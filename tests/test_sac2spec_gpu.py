@@ -167,3 +167,35 @@ def test_fused_production_path(gpu, tmp_path, monkeypatch, whiten, normalize, ph
           f"vs five-kernel {rel_l2(out, out5):.3e}")
     assert err < 1e-5
     assert rel_l2(out5, want) < 1e-5
+
+
+@pytest.mark.parametrize("whiten,normalize", [(2, 3), (3, 3), (1, 3)])
+def test_compensated_fp32_running_means(gpu, tmp_path, whiten, normalize):
+    """Spec mode fxc_s2s_set_sums(1): the running means (RUN-ABS weights,
+    whitening weights) in float with Neumaier compensation instead of FP64
+    sums agree with the FP64-sum output to 1e-6 relative L2 and with the FP64
+    oracle to 1e-5; the five-kernel path rejects the mode."""
+    import torch
+
+    path = str(tmp_path / "filter.txt")
+    s2s.write_filter_file(path, 0.1, "0.1/2")
+    plan = s2s.plan_lengths(3 * 3600, 0.1, 3600, 3600, 100, s2s.read_filters(path))
+    rng = np.random.default_rng(77 + whiten)
+    tr = _traces(rng, 2, plan.npts)
+    steps = [0, 1, 2]
+    gp = _gpu_plan(plan, 1, whiten, normalize, 0)
+    ref = gp.process(torch.from_numpy(tr).to(gpu), steps).cpu().numpy()
+    gp.set_sums(True)
+    comp = gp.process(torch.from_numpy(tr).to(gpu), steps).cpu().numpy()
+    win = s2s.cut_windows(tr.astype(np.float64), plan, steps)
+    want = s2s.process_batch(win.reshape(-1, 1, plan.seg), plan, whiten, normalize,
+                             False).reshape(len(steps), 2, -1)
+    d = rel_l2(comp, ref)
+    print(f"compensated vs FP64 sums whiten={whiten}: {d:.3e}, vs oracle {rel_l2(comp, want):.3e}")
+    assert d < 1e-6
+    assert rel_l2(comp, want) < 1e-5
+    gp.set_sums(False)
+    assert np.array_equal(gp.process(torch.from_numpy(tr).to(gpu), steps).cpu().numpy(), ref)
+    multi = _gpu_plan(plan, 3, whiten, normalize, 0)  # three channels: five-kernel path
+    with pytest.raises(RuntimeError):
+        multi.set_sums(True)
