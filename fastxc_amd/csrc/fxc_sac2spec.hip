@@ -1142,13 +1142,20 @@ s2s_back_kernel(float2 *X, long long fpitch, int seg, int i1, int i2, int i3, in
   }
   __syncthreads();
   // fold into the digit-reversed positions of the inverse (every value read
-  // before any is written)
-  constexpr int kFoldPer = (Ms + kS2sThreads - 1) / kS2sThreads;
-  float2 z[kFoldPer];
+  // before any is written): Z[k] = S + i D with S = X[k] + conj X[Ms-k],
+  // D = conj(w_k) (X[k] - conj X[Ms-k]), w_k = exp(-i pi k / Ms).  Bin Ms-k
+  // folds from the same two values, Z[Ms-k] = conj S + i conj D (w_{Ms-k} =
+  // -conj w_k), so a thread takes the pair (k, Ms-k) for k = 0 .. Ms/2; k = 0
+  // (with X[Ms]) and k = Ms/2 fold alone.  Two loads per pair and 2 x
+  // kPairPer values held across the barrier.
+  static_assert(Ms % 2 == 0, "pairwise fold needs an even half length");
+  constexpr int kHalf = Ms / 2;
+  constexpr int kPairPer = (kHalf + 1 + kS2sThreads - 1) / kS2sThreads;
+  float2 zk[kPairPer], zm[kPairPer];
 #pragma unroll
-  for (int i = 0; i < kFoldPer; ++i) {
+  for (int i = 0; i < kPairPer; ++i) {
     const int k = threadIdx.x + i * kS2sThreads;
-    if (k < Ms) {
+    if (k <= kHalf) {
       float2 a = lds[k];
       float2 b = cconj(lds[Ms - k]);
       if (k == 0) {
@@ -1157,14 +1164,18 @@ s2s_back_kernel(float2 *X, long long fpitch, int seg, int i1, int i2, int i3, in
       }
       const float2 sv = cadd(a, b);
       const float2 dv = cmul(cconj(fold_lookup(folds, k)), csub(a, b));
-      z[i] = make_float2(sv.x - dv.y, sv.y + dv.x);
+      zk[i] = make_float2(sv.x - dv.y, sv.y + dv.x);
+      zm[i] = make_float2(sv.x + dv.y, dv.x - sv.y);
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < kFoldPer; ++i) {
+  for (int i = 0; i < kPairPer; ++i) {
     const int k = threadIdx.x + i * kS2sThreads;
-    if (k < Ms) lds[fixed_pos<FPS>(k)] = z[i];
+    if (k <= kHalf) {
+      lds[fixed_pos<FPS>(k)] = zk[i];
+      if (k != 0 && k != kHalf) lds[fixed_pos<FPS>(Ms - k)] = zm[i];
+    }
   }
   __syncthreads();
   fixed_fft_inv<FPS, kS2sThreads>(lds, ts.tw0, twls);

@@ -112,6 +112,22 @@ __device__ __forceinline__ void stack_fft(float2 *buf, int nt, const StackFftDev
   __syncthreads();
 }
 
+// stack_fft with the direction and the Bluestein choice fixed at compile
+// time.  In a kernel that inlines both directions of the runtime form the
+// compiler copies the plan structure into scratch memory (976 B per lane at
+// the plan sizes here); with both choices fixed, the plan is indexed in the
+// kernel arguments.
+template <int kSgn, bool kBlue>
+__device__ __forceinline__ void stack_fft_dir(float2 *buf, int nt, const StackFftDev &F,
+                                              const float2 *twl) {
+  if constexpr (kBlue)
+    stack_fft(buf, nt, F, twl, (float)kSgn);
+  else if constexpr (kSgn < 0)
+    lds_fft<kRadixAll>(buf, nt, F.fwd, twl, -1.f);
+  else
+    lds_fft<kRadixAll>(buf, nt, F.inv, twl + kTwFine + F.fwd.ncoarse, 1.f);
+}
+
 // Analytic spectra of real rows: A[k] = 2 X[k] for 1 <= k < (n+1)/2, else 0
 // (hilbertTransformKernel after R2C).  Output natural order, [rows][n].
 __global__ void __launch_bounds__(kStkThreads)
@@ -565,31 +581,78 @@ __device__ __forceinline__ void stream_phi(float2 *lds, const float *__restrict_
   __syncthreads();
 }
 
-// Closes one sub-stack per workgroup: S[slot] += cnt phi(open[slot] / cnt),
-// open[slot] = 0.
-template <int kPer>
+// Closes up to kCloseRows sub-stacks per workgroup (per_wg of them):
+// S[slot] += cnt phi(open[slot] / cnt), open[slot] = 0, with phi the steps
+// of stream_phi.  At n = 2001 one row keeps only 69-87 of the 256 threads
+// busy in the radix-23 and radix-29 passes, so the rows of a workgroup share
+// each transform call (as in analytic_kernel) and the twiddles are staged
+// once for all of them.
+constexpr int kCloseRows = 3;
+
+template <int kPer, bool kBlue>
 __global__ void __launch_bounds__(kStkThreads)
 stream_pws_close_kernel(float2 *__restrict__ S, float *__restrict__ open,
-                        const int *__restrict__ slot, const int *__restrict__ cnt, StackFftDev F,
+                        const int *__restrict__ slot, const int *__restrict__ cnt,
+                        long long nclose, int per_wg, StackFftDev F,
                         const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i) {
   extern __shared__ float2 lds[];
-  const int n = F.n;
-  float2 *twl = lds + F.ld;
+  const int n = F.n, ld = F.ld, tid = threadIdx.x;
+  float2 *twl = lds + (size_t)per_wg * ld;
   stage_stack_tw(twl, F, tw_f, tw_i);
-  const long long sl = slot[blockIdx.x];
-  const int c = cnt[blockIdx.x];
-  float *o = open + sl * n;
-  float2 phi[kPer];
-  stream_phi<kPer>(lds, o, 1.0f / (float)c, F, twl, phi);
-  float2 *Sr = S + sl * n;
-  const float fc = (float)c;
+  const long long r0 = (long long)blockIdx.x * per_wg;
+  const int nt = (int)std::min<long long>(per_wg, nclose - r0);
+  for (int i = tid; i < nt * ld; i += kStkThreads) {
+    const int t = i / ld, j = i - t * ld;
+    lds[i] = j < n ? make_float2(open[(long long)slot[r0 + t] * n + j] * (1.0f / (float)cnt[r0 + t]),
+                                 0.f)
+                   : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  stack_fft_dir<-1, kBlue>(lds, nt, F, twl);
+  // analytic mask into natural order (through registers)
+  const int half = (n + 1) / 2;
+  float2 a[kCloseRows][kPer];
 #pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int t = threadIdx.x + k * kStkThreads;
-    if (t < n) {
-      const float2 v = Sr[t];
-      Sr[t] = make_float2(v.x + fc * phi[k].x, v.y + fc * phi[k].y);
-      o[t] = 0.f;
+  for (int t = 0; t < kCloseRows; ++t)
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int f = tid + k * kStkThreads;
+      a[t][k] = make_float2(0.f, 0.f);
+      if (t < nt && f >= 1 && f < half) {
+        const float2 z = lds[t * ld + stack_pos(F, f)];
+        a[t][k] = make_float2(2.f * z.x, 2.f * z.y);
+      }
+    }
+  __syncthreads();
+  for (int i = tid; i < nt * ld; i += kStkThreads) lds[i] = make_float2(0.f, 0.f);
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < kCloseRows; ++t)
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int f = tid + k * kStkThreads;
+      if (t < nt && f < n) lds[t * ld + f] = a[t][k];
+    }
+  __syncthreads();
+  stack_fft_dir<1, kBlue>(lds, nt, F, twl);
+  const float invn = 1.f / (float)n;
+  for (int t = 0; t < nt; ++t) {
+    const long long sl = slot[r0 + t];
+    const float fc = (float)cnt[r0 + t];
+    float2 *Sr = S + sl * n;
+    float *o = open + sl * n;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int j = tid + k * kStkThreads;
+      if (j < n) {
+        const float2 z = lds[t * ld + stack_pos(F, j)];
+        const float re = z.x * invn, im = z.y * invn;
+        float mod = sqrtf(re * re + im * im);
+        mod = mod > 1e-7f ? mod : 1e-7f;
+        const float2 v = Sr[j];
+        Sr[j] = make_float2(v.x + fc * (re / mod), v.y + fc * (im / mod));
+        o[j] = 0.f;
+      }
     }
   }
 }
@@ -845,13 +908,20 @@ int stream_pws_close(float2 *S, float *open, const int *d_slot, const int *d_cnt
   size_t lds = 0;
   FXC_TRY(stream_pws_launch_check(n, &F, &lds));
   const int per = (n + kStkThreads - 1) / kStkThreads;
+  // rows per workgroup: as many of kCloseRows as the LDS holds beside the
+  // twiddles (3 at n = 2001, where three workgroups share a CU)
+  const int rows = (int)std::min<long long>(
+      kCloseRows, ((long long)kLdsComplexCapacity - kLdsReserve - F.twc) / F.dev.ld);
+  FXC_CHECK_ARG(rows >= 1, "stream PWS: length %d does not fit the LDS", n);
+  lds = ((size_t)rows * F.dev.ld + F.twc) * sizeof(float2);
+  const unsigned grid = (unsigned)ceil_div((size_t)nclose, (size_t)rows);
 #define FXC_SPC(P)                                                                            \
   if (per <= P) {                                                                            \
-    auto *k = stream_pws_close_kernel<P>;                                                    \
+    auto *k = F.dev.blue ? stream_pws_close_kernel<P, true> : stream_pws_close_kernel<P, false>; \
     FXC_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,   \
                                 (int)lds));                                                  \
-    hipLaunchKernelGGL(k, dim3((unsigned)nclose), dim3(kStkThreads), lds, st, S, open, d_slot, \
-                       d_cnt, F.dev, F.tw_f, F.tw_i);                                        \
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kStkThreads), lds, st, S, open, d_slot, d_cnt,     \
+                       nclose, rows, F.dev, F.tw_f, F.tw_i);                                 \
     FXC_HIP(hipGetLastError());                                                              \
     return FXC_OK;                                                                           \
   }
