@@ -130,31 +130,120 @@ __device__ __forceinline__ void stack_fft_dir(float2 *buf, int nt, const StackFf
 
 // Analytic spectra of real rows: A[k] = 2 X[k] for 1 <= k < (n+1)/2, else 0
 // (hilbertTransformKernel after R2C).  Output natural order, [rows][n].
+// Two real rows share one complex transform, z = s0 x0 + i s1 x1, whose
+// spectrum separates as s0 A0[k] = Z[k] + conj Z[n-k] and
+// s1 A1[k] = -i (Z[k] - conj Z[n-k]).  s0, s1 are powers of two that bring
+// each row's largest |x| into [1, 2), so both rows carry the transform's
+// rounding relative to their own size and the unscaling is exact.  per_wg
+// transforms (2 per_wg rows) per workgroup.  A workgroup whose rows hold a
+// non-finite value transforms its rows one by one instead, so such a value
+// stays in its own row, as in a transform per row.
+constexpr int kAnalyticMaxPer = 3;
+
+template <bool kBlue>
 __global__ void __launch_bounds__(kStkThreads)
 analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, StackFftDev F,
                 const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i,
                 float2 *__restrict__ A) {
   extern __shared__ float2 lds[];
-  const int n = F.n, ld = F.ld;
+  __shared__ unsigned rmax[2 * kAnalyticMaxPer];  // float bits of max |x| per row
+  __shared__ int bad;                             // a non-finite value in the rows
+  const int n = F.n, ld = F.ld, tid = threadIdx.x;
   float2 *twl = lds + (size_t)per_wg * ld;
   stage_stack_tw(twl, F, tw_f, tw_i);
-  const long long r0 = (long long)blockIdx.x * per_wg;
-  const int nt = (int)std::min<long long>(per_wg, rows - r0);
-  for (int i = threadIdx.x; i < nt * n; i += blockDim.x) {
+  const long long r0 = (long long)blockIdx.x * 2 * per_wg;
+  const int nr = (int)std::min<long long>(2 * per_wg, rows - r0);
+  const int nt = (nr + 1) / 2;
+  if (tid < 2 * kAnalyticMaxPer) rmax[tid] = 0u;
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  bool nonfinite = false;
+  float m[2 * kAnalyticMaxPer];
+#pragma unroll
+  for (int r = 0; r < 2 * kAnalyticMaxPer; ++r) m[r] = 0.f;
+  for (int i = tid; i < nt * n; i += blockDim.x) {
     const int t = i / n, j = i - t * n;
-    lds[t * ld + j] = make_float2(x[(r0 + t) * n + j], 0.f);
+    const float a = x[(r0 + 2 * t) * n + j];
+    const float b = 2 * t + 1 < nr ? x[(r0 + 2 * t + 1) * n + j] : 0.f;
+    lds[t * ld + j] = make_float2(a, b);
+    nonfinite |= !isfinite(a) || !isfinite(b);
+#pragma unroll
+    for (int r = 0; r < kAnalyticMaxPer; ++r)
+      if (r == t) {
+        m[2 * r] = fmaxf(m[2 * r], fabsf(a));
+        m[2 * r + 1] = fmaxf(m[2 * r + 1], fabsf(b));
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < 2 * kAnalyticMaxPer; ++r)
+    if (m[r] > 0.f) atomicMax(&rmax[r], __float_as_uint(m[r]));  // |x| >= 0: bits order as values
+  if (nonfinite) bad = 1;
+  __syncthreads();
+  const int half = (n + 1) / 2;
+  if (bad) {
+    for (int h = 0; h < 2; ++h) {
+      for (int i = tid; i < nt * n; i += blockDim.x) {
+        const int t = i / n, j = i - t * n, r = 2 * t + h;
+        lds[t * ld + j] = make_float2(r < nr ? x[(r0 + r) * n + j] : 0.f, 0.f);
+      }
+      __syncthreads();
+      stack_fft_dir<-1, kBlue>(lds, nt, F, twl);
+      for (int i = tid; i < nt * n; i += blockDim.x) {
+        const int t = i / n, k = i - t * n, r = 2 * t + h;
+        float2 v = make_float2(0.f, 0.f);
+        if (k >= 1 && k < half) {
+          const float2 z = lds[t * ld + stack_pos(F, k)];
+          v = make_float2(2.f * z.x, 2.f * z.y);
+        }
+        if (r < nr) A[(r0 + r) * n + k] = v;
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  // scales 2^-e with max |x| in [2^(e-1), 2^e); 1 for an all-zero row (or
+  // a non-finite maximum, which then propagates as in a single transform)
+  float sc[2 * kAnalyticMaxPer];
+#pragma unroll
+  for (int r = 0; r < 2 * kAnalyticMaxPer; ++r) {
+    const float mx = __uint_as_float(rmax[r]);
+    int e = 0;
+    if (mx > 0.f && isfinite(mx)) (void)frexpf(mx, &e);
+    sc[r] = ldexpf(1.f, -e);
+  }
+  for (int i = tid; i < nt * n; i += blockDim.x) {
+    const int t = i / n;
+    float s0 = 1.f, s1 = 1.f;
+#pragma unroll
+    for (int r = 0; r < kAnalyticMaxPer; ++r)
+      if (r == t) {
+        s0 = sc[2 * r];
+        s1 = sc[2 * r + 1];
+      }
+    const float2 v = lds[t * ld + (i - t * n)];
+    lds[t * ld + (i - t * n)] = make_float2(v.x * s0, v.y * s1);
   }
   __syncthreads();
-  stack_fft(lds, nt, F, twl, -1.f);
-  const int half = (n + 1) / 2;
-  for (int i = threadIdx.x; i < nt * n; i += blockDim.x) {
+  stack_fft_dir<-1, kBlue>(lds, nt, F, twl);
+  for (int i = tid; i < nt * n; i += blockDim.x) {
     const int t = i / n, k = i - t * n;
-    float2 v = make_float2(0.f, 0.f);
+    float2 a0 = make_float2(0.f, 0.f), a1 = make_float2(0.f, 0.f);
     if (k >= 1 && k < half) {
-      const float2 z = lds[t * ld + stack_pos(F, k)];
-      v = make_float2(2.f * z.x, 2.f * z.y);
+      const float2 zk = lds[t * ld + stack_pos(F, k)];
+      const float2 zm = lds[t * ld + stack_pos(F, n - k)];
+      a0 = make_float2(zk.x + zm.x, zk.y - zm.y);      // Z[k] + conj Z[n-k]
+      const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);  // Z[k] - conj Z[n-k]
+      a1 = make_float2(d.y, -d.x);                     // -i d
     }
-    A[(r0 + t) * n + k] = v;
+    float u0 = 1.f, u1 = 1.f;
+#pragma unroll
+    for (int r = 0; r < kAnalyticMaxPer; ++r)
+      if (r == t) {
+        u0 = 1.f / sc[2 * r];
+        u1 = 1.f / sc[2 * r + 1];
+      }
+    A[(r0 + 2 * t) * n + k] = make_float2(a0.x * u0, a0.y * u0);
+    if (2 * t + 1 < nr) A[(r0 + 2 * t + 1) * n + k] = make_float2(a1.x * u1, a1.y * u1);
   }
 }
 
@@ -815,25 +904,31 @@ static int stack_batch(const StackFft &S, size_t fixed, int extra_rows, const lo
   return (int)std::min<long long>(std::min<long long>(kStkBatch, gmax), room / S.dev.ld);
 }
 
-// Device offsets and the analytic spectra of every group and linear stack,
-// into A [groups + pairs][n] and offsets [npairs + 1] (caller's buffers).
+// Device offsets and the analytic spectra of every group and (with_lin,
+// TF-PWS) of every linear stack, into A [groups + pairs][n] and offsets
+// [npairs + 1] (caller's buffers).  PWS weights only the groups' phases, so
+// it leaves the linear stacks' rows out.
 static int stack_analytic(const float *d_pre, const float *d_lin, const long long *h_off,
                           int npairs, int n, float2 *A, long long *offsets, const StackFft &S,
-                          hipStream_t st) {
+                          hipStream_t st, bool with_lin) {
   const long long ng = h_off[npairs];
   FXC_HIP(hipMemcpyAsync(offsets, h_off, (size_t)(npairs + 1) * sizeof(long long),
                          hipMemcpyHostToDevice, st));
-  // at most 3 rows per workgroup: three workgroups share a CU at n = 2001
-  const int per = std::max(1, std::min(3, (kLdsComplexCapacity - S.twc) / S.dev.ld));
+  // at most 3 transforms (6 rows) per workgroup: three workgroups share a CU
+  // at n = 2001
+  const int per = std::max(1, std::min(kAnalyticMaxPer, (kLdsComplexCapacity - kLdsReserve - S.twc) /
+                                                            S.dev.ld));
   const size_t lds = ((size_t)per * S.dev.ld + S.twc) * sizeof(float2);
-  FXC_HIP(hipFuncSetAttribute((const void *)analytic_kernel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  auto *kern = S.dev.blue ? analytic_kernel<true> : analytic_kernel<false>;
+  FXC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds));
   if (ng > 0) {
-    hipLaunchKernelGGL(analytic_kernel, dim3((unsigned)ceil_div((size_t)ng, (size_t)per)),
+    hipLaunchKernelGGL(kern, dim3((unsigned)ceil_div((size_t)ng, (size_t)(2 * per))),
                        dim3(kStkThreads), lds, st, d_pre, ng, per, S.dev, S.tw_f, S.tw_i, A);
     FXC_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(analytic_kernel, dim3((unsigned)ceil_div((size_t)npairs, (size_t)per)),
+  if (!with_lin) return FXC_OK;
+  hipLaunchKernelGGL(kern, dim3((unsigned)ceil_div((size_t)npairs, (size_t)(2 * per))),
                      dim3(kStkThreads), lds, st, d_lin, (long long)npairs, per, S.dev, S.tw_f,
                      S.tw_i, A + (size_t)ng * n);
   FXC_HIP(hipGetLastError());
@@ -841,11 +936,11 @@ static int stack_analytic(const float *d_pre, const float *d_lin, const long lon
 }
 
 static int stack_common(const float *d_pre, const float *d_lin, const long long *h_off, int npairs,
-                        int n, StackWork &w, const StackFft &S, hipStream_t st) {
-  const long long rows = h_off[npairs] + npairs;
-  FXC_TRY(dmalloc((void **)&w.A, (size_t)rows * n * sizeof(float2)));
+                        int n, StackWork &w, const StackFft &S, hipStream_t st, bool with_lin) {
+  const long long rows = h_off[npairs] + (with_lin ? npairs : 0);
+  FXC_TRY(dmalloc((void **)&w.A, (size_t)std::max<long long>(rows, 1) * n * sizeof(float2)));
   FXC_TRY(dmalloc((void **)&w.offsets, (size_t)(npairs + 1) * sizeof(long long)));
-  return stack_analytic(d_pre, d_lin, h_off, npairs, n, w.A, w.offsets, S, st);
+  return stack_analytic(d_pre, d_lin, h_off, npairs, n, w.A, w.offsets, S, st, with_lin);
 }
 
 // The voice kernel's per-thread W registers come in steps of 4 positions.
@@ -1021,7 +1116,7 @@ extern "C" int fxc_pws_spec(const float *d_prestack, const float *d_weights,
   FXC_CHECK_ARG(per <= 16, "fxc_pws: length %d too long", n);
   hipStream_t st = (hipStream_t)stream;
   StackWork w;
-  FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, S, st));
+  FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, S, st, false));
   FXC_TRY(pws_launch(S, nbatch, npairs, w.A, d_weights, w.offsets, d_linear, power, d_out, st));
   FXC_HIP(hipStreamSynchronize(st));  // workspace is released on return
   return FXC_OK;
@@ -1048,7 +1143,7 @@ std::map<int, unsigned char *> g_scratch_ptr;  // device -> address of `scratch`
 // 256-byte boundary.
 static size_t noalloc_bytes(long long groups, int npairs, int n, size_t *spec_off) {
   *spec_off = ((size_t)(npairs + 1) * sizeof(long long) + 255) / 256 * 256;
-  return *spec_off + (size_t)(groups + npairs) * n * sizeof(float2);
+  return *spec_off + (size_t)std::max<long long>(groups, 1) * n * sizeof(float2);
 }
 }  // namespace fxc
 
@@ -1098,7 +1193,7 @@ extern "C" int fxc_pws_noalloc(const float *d_prestack, const float *d_weights,
   hipStream_t st = (hipStream_t)stream;
   long long *offsets = (long long *)base;
   float2 *A = (float2 *)(base + spec_off);
-  FXC_TRY(stack_analytic(d_prestack, d_linear, h_offsets, npairs, n, A, offsets, S, st));
+  FXC_TRY(stack_analytic(d_prestack, d_linear, h_offsets, npairs, n, A, offsets, S, st, false));
   FXC_TRY(pws_launch(S, nbatch, npairs, A, d_weights, offsets, d_linear, 1.f, d_out, st));
   // complete on return, as fxc_pws (the host offsets are read by the copy)
   FXC_HIP(hipStreamSynchronize(st));
@@ -1137,7 +1232,7 @@ extern "C" int fxc_tfpws_spec(const float *d_prestack, const float *d_weights,
                 "fxc_tfpws: length %d too long for the voice kernel", n);
   hipStream_t st = (hipStream_t)stream;
   StackWork w;
-  FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, S, st));
+  FXC_TRY(stack_common(d_prestack, d_linear, h_offsets, npairs, n, w, S, st, true));
   const int nfreq = n / 2 + 1;
   const int nchunks = (nfreq + kVoiceChunk - 1) / kVoiceChunk;
   const float df = band_limited ? (float)(1.0 / ((double)n * delta)) : 0.f;

@@ -228,8 +228,8 @@ int fxc_tfpws_spec(const float *d_prestack, const float *d_weights, const long l
 /* PWS without device allocation.  No counterpart in the reference: it is
  * for callers that cannot allocate (code under HIP graph capture, tight
  * serving loops).  Same arguments and result as fxc_pws (power 1); its
- * workspace (offsets and analytic spectra, 8 (npairs+1) + 8 n (groups +
- * npairs) bytes) lives in a fixed 64 MiB device array per GPU, held until
+ * workspace (offsets and the groups' analytic spectra, 8 (npairs+1) +
+ * 8 n groups bytes) lives in a fixed 64 MiB device array per GPU, held until
  * the process ends; larger calls are FXC_ERR_ARG.  fxc_stack_prepare(n)
  * must run first on the device (once per device and length): it builds
  * the length-n FFT plans and resolves the array's device address (it

@@ -77,6 +77,46 @@ def test_pws_matches_oracle(gpu, N):
         assert err < 1e-5
 
 
+def test_pws_rows_of_unlike_size_and_nonfinite_rows(gpu):
+    """Two group rows share one complex transform in the analytic step: rows
+    of very different size must keep their own accuracy (pairs scaled by
+    1e4 and 1e-4 next to each other), an odd row count leaves one transform
+    with a single row, and a non-finite value stays in its own pair."""
+    import torch
+    from fastxc_amd import ops
+
+    N = 2001
+    rng = np.random.default_rng(11)
+    recs = [_records(rng, 10 + p, N) * np.float32(s) for p, s in enumerate((1e4, 1e-4, 1.0))]
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(gpu)
+
+    def run(rs):
+        pre, w, lin, off = [], [], [], [0]
+        for r in rs:
+            a, b, c = _prep32(r, 4, "pws")
+            pre.append(a)
+            w.append(b)
+            lin.append(c)
+            off.append(off[-1] + len(b))
+        assert off[-1] % 2 == 1  # 3 + 3 + 3 groups: an odd row count
+        return ops.pws(d(np.concatenate(pre)), d(np.concatenate(w)), np.array(off, np.int64),
+                       d(np.stack(lin))).cpu().numpy()
+
+    def check(out, pairs):
+        for p in pairs:
+            g, wg, l = stack.pws_prepare(recs[p].astype(np.float64), 4)
+            err = rel_l2(out[p], stack.pws(g, wg, l))
+            print(f"pair {p} rel_l2={err:.3e}")
+            assert err < 1e-5
+
+    check(run(recs), range(3))
+    bad = [r.copy() for r in recs]
+    bad[0][2, 100] = np.nan
+    out = run(bad)
+    assert not np.isfinite(out[0]).all()
+    check(out, (1, 2))
+
+
 @pytest.mark.parametrize("N,band", [(87, None), (345, None), (2001, None), (345, (0.1, 0.3, 0.02)),
                                     (201, None), (1001, None), (401, (0.1, 0.3, 0.02)),
                                     (4999, None)])
