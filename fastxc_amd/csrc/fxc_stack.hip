@@ -336,7 +336,8 @@ tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Ali
                    const float *__restrict__ weights, const long long *__restrict__ offsets,
                    StackFftDev F, const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i,
                    int nrows, int nbatch, int nfreq, float power,
-                   const unsigned char *__restrict__ chunk_active, float2 *__restrict__ out_spec) {
+                   const unsigned char *__restrict__ chunk_active,
+                   const float *__restrict__ gtab, float2 *__restrict__ out_spec) {
   extern __shared__ float2 lds[];
   const int n = F.n, ld = F.ld;
   float *gauss = (float *)lds;                       // [n] floats (n/2 complex slots)
@@ -351,26 +352,14 @@ tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Ali
   const long long g0 = offsets[pair], g1 = offsets[pair + 1];
   const float2 *Al = Alin + (size_t)pair * n;
   const int half = (n + 1) / 2;
-  const float pi2 = -2.0f * (float)M_PI * (float)M_PI;
   const float nf = (float)n;
   const int tid = threadIdx.x;
   for (int f = f0; f < std::min(nfreq, f0 + kVoiceChunk); ++f) {
     __syncthreads();
-    // Gaussian window of voice f (gaussianModulateSub, scale 0.1)
-    for (int c = tid; c < n; c += kThreads) {
-      float g;
-      if (f == 0) {
-        g = 1.f;
-      } else if (c == 0) {
-        g = 0.f;
-      } else {
-        // even n: the shift c = n/2 meets neither branch of the reference
-        // (c < half, or a mirror n - c < half) and gets 0
-        const int cm = c < half ? c : n - c;
-        g = cm < half ? expf(pi2 * (float)cm * (float)cm * 0.1f / ((float)f * (float)f)) : 0.f;
-      }
-      gauss[c] = g;
-    }
+    // Gaussian window of voice f (gaussianModulateSub, scale 0.1), from the
+    // table of tfpws_gauss_kernel: g(c) = gtab[f][min(c, n - c)]
+    const float *gf = gtab + (size_t)f * (half + 1);
+    for (int c = tid; c < n; c += kThreads) gauss[c] = gf[c < half ? c : n - c];
     float2 W[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) W[k] = make_float2(0.f, 0.f);
@@ -406,9 +395,10 @@ tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Ali
             const float2 z = buf[b * ld + p];
             float fa = sqrtf(z.x * z.x + z.y * z.y);
             if (fa <= 1e-7f) fa = 1.f;
-            const float wg = weights[gb + b];
-            acc.x += wg * z.x / (fa * nf);
-            acc.y += wg * z.y / (fa * nf);
+            // w z / (|z| N) with one division per position
+            const float r = weights[gb + b] / (fa * nf);
+            acc.x += z.x * r;
+            acc.y += z.y * r;
           }
           W[k] = acc;
         }
@@ -851,11 +841,13 @@ struct StackWork {
   long long *offsets = nullptr;
   float2 *spec = nullptr;   // [pairs][n]
   unsigned char *active = nullptr;
+  float *gauss = nullptr;   // [nfreq][(n+1)/2 + 1] Gaussian windows of the voices
   ~StackWork() {
     dfree(A);
     dfree(offsets);
     dfree(spec);
     dfree(active);
+    dfree(gauss);
   }
 };
 
@@ -944,10 +936,33 @@ static int stack_common(const float *d_pre, const float *d_lin, const long long 
 }
 
 // The voice kernel's per-thread W registers come in steps of 4 positions.
+// Gaussian windows of every voice for tfpws_voice_kernel, by the shift's
+// distance cm = min(c, n - c) from bin 0: gtab[f][cm], cm in [0, (n+1)/2],
+// the values the voice kernel would form (gaussianModulateSub, scale 0.1):
+// 1 at voice 0, 0 at c = 0, and 0 at cm = n/2 for even n, where the shift
+// meets neither branch of the reference (c < half, or a mirror n - c < half).
+__global__ void __launch_bounds__(kStkThreads)
+tfpws_gauss_kernel(int n, int nfreq, float *__restrict__ gtab) {
+  const int half = (n + 1) / 2;
+  const float pi2 = -2.0f * (float)M_PI * (float)M_PI;
+  const int f = blockIdx.y;
+  for (int cm = blockIdx.x * kStkThreads + threadIdx.x; cm <= half; cm += gridDim.x * kStkThreads) {
+    float g;
+    if (f == 0)
+      g = 1.f;
+    else if (cm == 0)
+      g = 0.f;
+    else
+      g = cm < half ? expf(pi2 * (float)cm * (float)cm * 0.1f / ((float)f * (float)f)) : 0.f;
+    gtab[(size_t)f * (half + 1) + cm] = g;
+  }
+}
+
 static int launch_voice(int n, size_t lds, dim3 grid, hipStream_t st, const float2 *Ag,
                         const float2 *Alin, const float *wts, const long long *off,
                         const StackFft &S, int nrows, int nbatch, int nfreq, float power,
-                        int nvoices, const unsigned char *active, float2 *spec) {
+                        int nvoices, const unsigned char *active, const float *gtab,
+                        float2 *spec) {
   const int per = (n + kVoiceThreads - 1) / kVoiceThreads;
 #define FXC_VOICE(P)                                                                        \
   if (per <= P) {                                                                          \
@@ -962,7 +977,7 @@ static int launch_voice(int n, size_t lds, dim3 grid, hipStream_t st, const floa
       FXC_HIP(hipFuncSetAttribute((const void *)k,                                         \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));  \
       hipLaunchKernelGGL(k, grid, dim3(kVoiceThreads), lds, st, Ag, Alin, wts, off, S.dev, \
-                         S.tw_f, S.tw_i, nrows, nbatch, nfreq, power, active, spec);       \
+                         S.tw_f, S.tw_i, nrows, nbatch, nfreq, power, active, gtab, spec); \
     }                                                                                      \
     FXC_HIP(hipGetLastError());                                                            \
     return FXC_OK;                                                                         \
@@ -1250,10 +1265,18 @@ extern "C" int fxc_tfpws_spec(const float *d_prestack, const float *d_weights,
     FXC_HIP(hipMemcpyAsync(w.active, act.data(), nchunks, hipMemcpyHostToDevice, st));
   }
   const long long ng = h_offsets[npairs];
+  if (nvoices == 0) {
+    const int gh = (n + 1) / 2 + 1;
+    FXC_TRY(dmalloc((void **)&w.gauss, (size_t)nfreq * gh * sizeof(float)));
+    hipLaunchKernelGGL(tfpws_gauss_kernel,
+                       dim3((unsigned)ceil_div((size_t)gh, (size_t)kStkThreads), (unsigned)nfreq),
+                       dim3(kStkThreads), 0, st, n, nfreq, w.gauss);
+    FXC_HIP(hipGetLastError());
+  }
   size_t lds = ((size_t)(n + 1) / 2 + (size_t)nrows * S.dev.ld + S.twc) * sizeof(float2);
   FXC_TRY(launch_voice(n, lds, dim3((unsigned)nchunks, (unsigned)npairs), st, w.A,
                        w.A + (size_t)ng * n, d_weights, w.offsets, S, nrows, nbatch, nfreq,
-                       power, nvoices, (const unsigned char *)w.active, w.spec));
+                       power, nvoices, (const unsigned char *)w.active, w.gauss, w.spec));
   lds = ((size_t)S.dev.ld + S.twc) * sizeof(float2);
   FXC_HIP(hipFuncSetAttribute((const void *)tfpws_final_kernel,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
