@@ -871,19 +871,21 @@ def batch_measure(nseg=64, seed=13, reps=3, budget_gb=235.0, max_rows=65536):
     nsta = rng.integers(20, 401, size=nseg)
     nstep = rng.integers(6, 49, size=nseg)
 
-    def depth(w):  # operand depth of a plan: windows rounded up to 16
-        return (w + 15) // 16 * 16
+    def depth(w):  # operand depth of a plan (the compiled depths of the contraction)
+        return next(d for d in (8, 16, 24, 32, 40, 48, 64, 96, 128) if w <= d)
 
     def gbytes(k):
-        """Peak device bytes (GB) of grouping 0 and of grouping 1."""
+        """Peak device bytes (GB) of grouping 0 and of grouping 1: spectra,
+        outputs, the split operands of the largest group and one shared
+        cross-spectrum workspace."""
         a, w = nsta[:k], nstep[:k]
         fixed = float((a * w).sum()) * nspec * 8 + float((a * (a - 1) // 2).sum()) * (2 * half + 1) * 4
         ws = max_rows * 18528 * 8.0
         padded = (a + 31) // 32 * 32
         one = float(padded.sum()) * nspec * 2 * depth(int(w.max())) * 4 + ws
         kinds = sorted({depth(int(x)) for x in w})
-        per = sum(float(padded[[depth(int(x)) == d for x in w]].sum()) * nspec * 2 * d * 4
-                  for d in kinds) + len(kinds) * ws
+        per = max(float(padded[[depth(int(x)) == d for x in w]].sum()) * nspec * 2 * d * 4
+                  for d in kinds) + ws
         return (fixed + one) / 1e9, (fixed + per) / 1e9
 
     k = nseg

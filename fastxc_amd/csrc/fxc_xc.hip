@@ -1224,11 +1224,13 @@ struct fxc_xc_plan {
   // ordered on a stream never reads freed memory)
   size_t cap_tiles = 0, cap_tile_rows = 0, cap_rows = 0, cap_row_src = 0, cap_row_rec = 0;
   std::vector<int> h_tile_rows, h_row_task, h_row_src, h_row_rec;
-  // workspaces
+  // workspaces; ws_shared: d_xspec, d_hi and d_lo belong to a batch that
+  // runs its plans one after another on one stream (not freed or grown here)
   float2 *d_xspec = nullptr;
   size_t xspec_rows = 0;
   unsigned short *d_hi = nullptr, *d_lo = nullptr;
   size_t split_elems = 0;
+  bool ws_shared = false;
   unsigned int *d_amax = nullptr;
   float *d_inv_scale = nullptr, *d_scale = nullptr;
   int scale_cap = 0;
@@ -1401,9 +1403,11 @@ extern "C" int fxc_xc_plan_destroy(fxc_xc_plan *p) {
   if (!p) return FXC_OK;
   xc_drop_events(p);
   xc_free_pairs(p);
-  dfree(p->d_xspec);
-  dfree(p->d_hi);
-  dfree(p->d_lo);
+  if (!p->ws_shared) {
+    dfree(p->d_xspec);
+    dfree(p->d_hi);
+    dfree(p->d_lo);
+  }
   dfree(p->d_amax);
   dfree(p->d_inv_scale);
   dfree(p->d_scale);
@@ -1547,6 +1551,12 @@ __global__ void band_zero_kernel(float2 *__restrict__ x, long long pitch, long l
 }
 
 static int xc_ensure_workspace(fxc_xc_plan *p, size_t rows) {
+  if (p->ws_shared) {
+    FXC_CHECK_ARG(rows <= p->xspec_rows &&
+                      (size_t)p->nsta * p->nspec * (size_t)(2 * p->kh) <= p->split_elems,
+                  "xc: shared workspace too small");
+    return FXC_OK;
+  }
   if (rows > p->xspec_rows) {
     dfree(p->d_xspec);
     p->d_xspec = nullptr;
@@ -1953,6 +1963,11 @@ struct fxc_xc_batch {
   int4 *d_tinfo = nullptr;
   float *d_scale_keep = nullptr;  // KEEP-mode row scales (fixed by the shapes)
   float *d_scale = nullptr;       // EXCLUDE-mode row scales (from the valid bytes)
+  // one cross-spectrum workspace and one pair of split operands for all the
+  // groups: their plans run one after another on one stream
+  float2 *d_ws_x = nullptr;
+  unsigned short *d_ws_hi = nullptr, *d_ws_lo = nullptr;
+  size_t ws_rows = 0, ws_elems = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -2126,6 +2141,9 @@ extern "C" int fxc_xc_batch_destroy(fxc_xc_batch *b) {
   batch_free_groups(b);
   dfree(b->d_segs);
   dfree(b->d_scale);
+  dfree(b->d_ws_x);
+  dfree(b->d_ws_hi);
+  dfree(b->d_ws_lo);
   delete b;
   return FXC_OK;
 }
@@ -2158,10 +2176,57 @@ extern "C" int fxc_xc_batch_set_rows(fxc_xc_batch *b, long long max_rows) {
   return batch_build(b);
 }
 
-// Every launch of one batch run on `st`; allocates nothing once the plans'
+// The shared workspaces, sized for the largest group (grow-only), lent to
+// every group's plan.
+static int batch_share_workspace(fxc_xc_batch *b) {
+  size_t rows = 0, elems = 0, pitch = 0;
+  for (auto &g : b->groups) {
+    fxc_xc_plan *p = g.plan;
+    pitch = std::max(pitch, (size_t)p->xpitch);
+    if (p->npairs == 0) continue;
+    long long maxrows = 0;
+    FXC_TRY(fxc_xc_pairs_info(p, nullptr, nullptr, &maxrows));
+    rows = std::max(rows, (size_t)maxrows);
+    elems = std::max(elems, (size_t)p->nsta * p->nspec * (size_t)(2 * p->kh));
+  }
+  if (rows > b->ws_rows) {
+    dfree(b->d_ws_x);
+    b->d_ws_x = nullptr;
+    b->ws_rows = 0;
+    FXC_TRY(dmalloc((void **)&b->d_ws_x, rows * pitch * sizeof(float2)));
+    b->ws_rows = rows;
+  }
+  if (elems > b->ws_elems) {
+    dfree(b->d_ws_hi);
+    dfree(b->d_ws_lo);
+    b->d_ws_hi = b->d_ws_lo = nullptr;
+    b->ws_elems = 0;
+    FXC_TRY(dmalloc((void **)&b->d_ws_hi, elems * sizeof(unsigned short)));
+    FXC_TRY(dmalloc((void **)&b->d_ws_lo, elems * sizeof(unsigned short)));
+    b->ws_elems = elems;
+  }
+  for (auto &g : b->groups) {
+    fxc_xc_plan *p = g.plan;
+    if (!p->ws_shared) {
+      dfree(p->d_xspec);
+      dfree(p->d_hi);
+      dfree(p->d_lo);
+    }
+    p->d_xspec = b->d_ws_x;
+    p->xspec_rows = b->ws_rows;
+    p->d_hi = b->d_ws_hi;
+    p->d_lo = b->d_ws_lo;
+    p->split_elems = b->ws_elems;
+    p->ws_shared = true;
+  }
+  return FXC_OK;
+}
+
+// Every launch of one batch run on `st`; allocates nothing once the shared
 // workspaces exist (the first run creates them).
 static int batch_run(fxc_xc_batch *b, const float2 *spec, const unsigned char *valid, float *cc,
                      hipStream_t st) {
+  FXC_TRY(batch_share_workspace(b));
   const float *scale = b->d_scale_keep;
   if (b->gap_mode == FXC_GAPS_EXCLUDE && valid && b->ntasks > 0) {
     hipLaunchKernelGGL(batch_task_scale_kernel, dim3((unsigned)ceil_div(b->ntasks, 256)),

@@ -304,15 +304,13 @@ class XcBatch:
             self.set_grouping(g)
             self.set_rows(r)
             return {"key": key, "grouping": g, "max_rows": r, "cached": True, "ms": None}
-        depths = {(w + 7) // 8 * 8 if w <= 48 else (64 if w <= 64 else 96 if w <= 96 else 128)
-                  for w in self.nstep}
         xpitch = (self.nspec + 15) // 16 * 16
         trials = []
         cur = torch.cuda.current_stream(spec.device)
         for g in (0, 1):
-            ngroups = 1 if g == 0 else len(depths)
             for r in rows:
-                ws = ngroups * min(r, self.ntasks) * xpitch * 8 / 1e9
+                # the groups share one cross-spectrum workspace of the largest chunk
+                ws = min(r, self.ntasks) * xpitch * 8 / 1e9
                 if budget_gb is not None and ws > budget_gb:
                     trials.append({"grouping": g, "max_rows": r, "ms": None, "skipped_gb": ws})
                     continue

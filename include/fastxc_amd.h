@@ -114,7 +114,9 @@ int fxc_xc_correlate(fxc_xc_plan *plan, const float *d_spec, float *d_cc, void *
  * (native/xc/worker/async_workers.cu:66); EXCLUDE divides each pair by the
  * windows in which both stations are valid (0 of them: a zero row).
  * Per segment the result is fxc_xc_correlate's for that segment alone.
- * max_rows bounds each plan's cross-spectrum workspace (0 = no bound).
+ * max_rows bounds the cross-spectrum workspace (0 = no bound); the plans
+ * of a batch run one after another on the caller's stream and share one
+ * workspace and one set of split operands, sized for the largest plan.
  * No reference counterpart: the reference runs one timestamp's job at a
  * time (native/xc/worker/async_workers.cu:18-86). */
 #define FXC_GAPS_KEEP 0
@@ -130,7 +132,7 @@ int fxc_xc_batch_info(const fxc_xc_batch *batch, long long *ntasks, long long *s
  * segment); 1: one plan per operand depth, the window count rounded up to
  * the next compiled depth (8, 16, 24, 32, 40, 48, 64, 96, 128). */
 int fxc_xc_batch_set_grouping(fxc_xc_batch *batch, int grouping);
-/* Row cap of each plan's cross-spectrum workspace (0 = no cap): the row
+/* Row cap of the shared cross-spectrum workspace (0 = no cap): the row
  * chunk every persistent contraction and lag launch of the batch covers. */
 int fxc_xc_batch_set_rows(fxc_xc_batch *batch, long long max_rows);
 int fxc_xc_batch_correlate(fxc_xc_batch *batch, const float *d_spec, const unsigned char *d_valid,
