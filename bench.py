@@ -90,7 +90,7 @@ def parse():
                     help="measure the batched uneven-segment form only")
     ap.add_argument("--no-planner", action="store_true",
                     help="skip the inventory path-planner measurement")
-    ap.add_argument("--stream-rates", default="128,160,192",
+    ap.add_argument("--stream-rates", default="160,192,224",
                     help="Poisson arrival rates (segments/s) of the streaming sweep")
     ap.add_argument("--stream-seconds", type=float, default=30.0,
                     help="arrival time per rate of the streaming sweep")

@@ -989,6 +989,40 @@ __device__ __forceinline__ void lag_r0_load(float2 (&a)[LagPass0<FP>::kPer][FP::
   }
 }
 
+// lag_r0_load of the one-window stack X[k] = conj(Ss[k]) Sr[k], formed from
+// the two stations' spectra as it is read.
+template <class FP>
+__device__ __forceinline__ void lag_r0_load_prod(float2 (&a)[LagPass0<FP>::kPer][FP::kRadix[0]],
+                                                 float2 (&b)[LagPass0<FP>::kPer][FP::kRadix[0]],
+                                                 const float2 *__restrict__ Ss,
+                                                 const float2 *__restrict__ Sr) {
+  using L = LagPass0<FP>;
+  auto x = [&](int k) {
+    const float2 u = Ss[k], v = Sr[k];
+    return make_float2(u.x * v.x + u.y * v.y, u.x * v.y - u.y * v.x);
+  };
+#pragma unroll
+  for (int it = 0; it < L::kPer; ++it) {
+    const int j = threadIdx.x + it * kLagThreads;
+    if (j < L::kPairs) {
+      if (j == 0) {
+#pragma unroll
+        for (int m = 0; m < L::R0; ++m) a[it][m] = x(L::S0 * m);
+        b[it][0] = x(L::M);
+      } else {
+#pragma unroll
+        for (int m = 0; m < L::R0; ++m) {
+          a[it][m] = x(j + L::S0 * m);
+          b[it][m] = x(L::M - j - L::S0 * m);
+        }
+      }
+    }
+    // one group's loads in flight at a time: both spectra of every group at
+    // once would not fit beside the rest of the kernel's registers
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 template <class FP>
 __device__ __forceinline__ void lag_r0_butterfly(float2 *lds, float2 (&u)[FP::kRadix[0]], int j,
                                                  float2 t) {
@@ -1014,13 +1048,19 @@ constexpr int lag_r0_lds_count() {
 
 // kEdge: pass P-2 forms only the outputs the kept lags read
 // (fixed_dif_pass_edge; the host checks that half_cc allows it).
-template <class FP, bool kEdge>
+// kProd (one window): row r's stack is conj(S[row_src[r]]) S[row_rec[r]],
+// formed from the spectra (rows of xpitch) as they are read, so no
+// cross-spectrum is written or read; the loads of a row are then issued at
+// its start instead of during the previous row.
+template <class FP, bool kEdge, bool kProd = false>
 __global__ void __launch_bounds__(kLagThreads)
 lag_kernel_fixed_r0(const float2 *__restrict__ xspec, long long xpitch, long long nrows,
                     const int *__restrict__ row_out, float *__restrict__ cc, int half_cc,
                     double weight, const float *__restrict__ task_scale,
                     const float2 *__restrict__ twg, const float2 *__restrict__ twl_g,
-                    const float2 *__restrict__ foldg, const float2 *__restrict__ roots) {
+                    const float2 *__restrict__ foldg, const float2 *__restrict__ roots,
+                    const int *__restrict__ row_src = nullptr,
+                    const int *__restrict__ row_rec = nullptr) {
   using L = LagPass0<FP>;
   constexpr int R0 = L::R0, S0 = L::S0;
   constexpr int RL = FP::kRadix[FP::kPasses - 1];
@@ -1046,8 +1086,12 @@ lag_kernel_fixed_r0(const float2 *__restrict__ xspec, long long xpitch, long lon
     firsts[threadIdx.x] = base * 16 + t;
   }
   long long row = blockIdx.x;
-  if (row < nrows) lag_r0_load<FP>(a, b, xspec + row * xpitch);
+  if constexpr (!kProd)
+    if (row < nrows) lag_r0_load<FP>(a, b, xspec + row * xpitch);
   for (; row < nrows; row += gridDim.x) {
+    if constexpr (kProd)
+      lag_r0_load_prod<FP>(a, b, xspec + (size_t)row_src[row] * xpitch,
+                           xspec + (size_t)row_rec[row] * xpitch);
 #pragma unroll
     for (int it = 0; it < L::kPer; ++it) {
       const int j = threadIdx.x + it * kLagThreads;
@@ -1082,7 +1126,8 @@ lag_kernel_fixed_r0(const float2 *__restrict__ xspec, long long xpitch, long lon
     }
     __syncthreads();
     const long long nxt = row + gridDim.x;
-    if (nxt < nrows) lag_r0_load<FP>(a, b, xspec + nxt * xpitch);
+    if constexpr (!kProd)
+      if (nxt < nrows) lag_r0_load<FP>(a, b, xspec + nxt * xpitch);
     if constexpr (kEdge) {
       fixed_dif_passes_lds<FP, 1, kLagThreads, FP::kPasses - 2>(lds, twl, 1.f);
       fixed_dif_pass_edge<FP, kLagThreads>(lds, twl, 1.f);
@@ -1505,7 +1550,7 @@ int xc_set_pairs_on(fxc_xc_plan *p, int nsta, const int *h_src, const int *h_rec
   FXC_TRY(up(p->d_tiles, p->h_tiles.data(), p->h_tiles.size() * sizeof(TileDesc)));
   FXC_TRY(up(p->d_tile_rows, rows.data(), rows.size() * sizeof(int)));
   FXC_TRY(up(p->d_row_task, row_task.data(), row_task.size() * sizeof(int)));
-  if (p->precision == FXC_PREC_PLAIN) {
+  if (p->precision == FXC_PREC_PLAIN || p->nstep == 1) {
     std::vector<int> &rs = p->h_row_src, &rr = p->h_row_rec;
     rs.resize(row_task.size());
     rr.resize(row_task.size());
@@ -1744,10 +1789,15 @@ static bool xc_lag_r0_on() {
 }
 
 // task_scale (per task, may be null) multiplies the output scale of each row.
+// prod_src/prod_rec (one window, the compile-time lag plan only): xs are the
+// station spectra (rows of xpitch) and row r's stack is their product.
 static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long long nrows,
                         const int *row_out, float *d_cc, hipStream_t st,
-                        const float *task_scale = nullptr) {
+                        const float *task_scale = nullptr, const int *prod_src = nullptr,
+                        const int *prod_rec = nullptr) {
   if (nrows == 0) return FXC_OK;
+  FXC_CHECK_ARG(!prod_src || (p->d_fix_tw && !p->large_lag && xc_lag_r0_on()),
+                "xc: the one-window product form needs the compile-time lag plan");
   const double weight0 = 1.0 / (double)((float)p->nfft * p->dt);
   if (p->large_lag) {
     // four-step inverse of the folded spectrum, kLagLargeChunk rows at a time
@@ -1794,12 +1844,14 @@ static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long
     constexpr int G2 = 2 * fixed_edge_group<FP>();
     const int n_neg = (p->half_cc + 1) / 2, n_pos = p->half_cc / 2;
     const bool edge = n_pos < G2 && n_neg <= G2;
-    auto fk = edge ? lag_kernel_fixed_r0<FP, true> : lag_kernel_fixed_r0<FP, false>;
+    auto fk = prod_src ? (edge ? lag_kernel_fixed_r0<FP, true, true>
+                               : lag_kernel_fixed_r0<FP, false, true>)
+                       : (edge ? lag_kernel_fixed_r0<FP, true> : lag_kernel_fixed_r0<FP, false>);
     FXC_HIP(hipFuncSetAttribute((const void *)fk, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)flds));
     hipLaunchKernelGGL(fk, dim3((unsigned)grid), dim3(kLagThreads), flds, st, xs, xpitch, nrows,
                        row_out, d_cc, p->half_cc, weight, task_scale, p->d_fix_tw, p->d_fix_twl,
-                       p->d_fix_fold, p->d_fix_roots);
+                       p->d_fix_fold, p->d_fix_roots, prod_src, prod_rec);
   } else if (p->d_fix_tw) {
     using FP = LagPlan18522;
     const size_t flds = (size_t)(FP::kM + 1 + fixed_lds_tw_count<FP>()) * sizeof(float2);
@@ -1854,11 +1906,23 @@ extern "C" int fxc_xc_lags(fxc_xc_plan *p, const float *d_xspec, long long nrows
 }
 
 // Full correlation: cc[task][0..2H] for every task set by fxc_xc_set_pairs.
+// One window at the compile-time lag length, every bin: a pair's stack is the
+// product conj(S[src]) S[rec] of its two spectra, formed in FP32 by the lag
+// kernel as it reads them (the plan's operand form is not used: there is no
+// sum over windows to contract).
+static bool xc_one_window_product(const fxc_xc_plan *p) {
+  return p->nstep == 1 && p->precision != FXC_PREC_PLAIN && p->band_c1 < 0 && p->d_fix_tw &&
+         !p->large_lag && xc_lag_r0_on() && p->d_row_src && p->d_row_rec;
+}
+
 extern "C" int fxc_xc_correlate(fxc_xc_plan *p, const float *d_spec, float *d_cc, void *stream) {
   FXC_CHECK_ARG(p && d_spec && d_cc, "fxc_xc_correlate: invalid arguments");
   FXC_CHECK_ARG(p->chunk_tile_begin.size() >= 1, "fxc_xc_correlate: no pairs set");
   if (p->npairs == 0) return FXC_OK;
   hipStream_t st = (hipStream_t)stream;
+  if (xc_one_window_product(p))
+    return xc_lags_rows(p, (const float2 *)d_spec, p->nspec, p->npairs, p->d_row_task, d_cc, st,
+                        nullptr, p->d_row_src, p->d_row_rec);
   long long maxrows = 0;
   FXC_TRY(fxc_xc_pairs_info(p, nullptr, nullptr, &maxrows));
   FXC_TRY(xc_ensure_workspace(p, (size_t)maxrows));

@@ -155,6 +155,40 @@ def test_production_length_mfma_forms(gpu, precision):
     assert err < TOL[precision]
 
 
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16", "fp16x3", "fp32"])
+def test_one_window_product_form(gpu, precision):
+    """One window at nfft 37044: the stack of a pair is conj(S[src]) S[rec],
+    which the lag kernel forms in FP32 from the two spectra as it reads them
+    (no contraction).  Every form then meets the FP32 tolerance against FP64
+    and agrees with the PLAIN form (the reference's step order through the
+    cross-spectrum); pairs in both orientations, a delayed copy lands at
+    exactly H + d."""
+    import torch
+    from fastxc_amd.ops import XcPlan
+
+    rng = np.random.default_rng(21)
+    nsta, seg, nfft, H, dt, d = 40, 36000, 37044, 1000, 0.1, 123
+    spec = _spectra(rng, 1, nsta, seg, nfft, dt)
+    base = rng.standard_normal(seg + d)
+    spec[0, :2] = (np.fft.rfft(np.stack([base[d:d + seg], base[:seg]]), n=nfft, axis=-1)
+                   * dt).astype(np.complex64)
+    src, rec = _all_pairs(nsta)
+    src = np.concatenate([src, rec[:50]]).astype(np.int32)  # 50 pairs reversed
+    rec = np.concatenate([rec, src[:50]]).astype(np.int32)
+    sd = torch.from_numpy(spec).to(gpu)
+    plan = XcPlan(nfft // 2 + 1, 1, dt, H, precision)
+    plan.set_pairs(nsta, src, rec)
+    cc = plan.correlate(sd).cpu().numpy()
+    want = xc.correlate(spec, src, rec, nfft, H, dt)
+    err = xc.rel_l2(cc, want)
+    print(f"{precision} one window rel_l2={err:.3e} maxabs={np.abs(cc - want).max():.3e}")
+    assert err < 1e-5
+    ref = XcPlan(nfft // 2 + 1, 1, dt, H, "plain")
+    ref.set_pairs(nsta, src, rec)
+    assert xc.rel_l2(cc, ref.correlate(sd).cpu().numpy()) < 1e-6
+    assert int(np.argmax(cc[0])) == H + d
+
+
 @pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
 def test_uneven_segments_with_gaps(gpu, precision):
     """Segments of different station and window counts one after another, 10 %
