@@ -662,7 +662,8 @@ __device__ __forceinline__ void stream_phi(float2 *lds, const float *__restrict_
 
 // Closes up to kCloseRows sub-stacks per workgroup (per_wg of them):
 // S[slot] += cnt phi(open[slot] / cnt), open[slot] = 0, with phi the steps
-// of stream_phi.  At n = 2001 one row keeps only 69-87 of the 256 threads
+// of stream_phi.  With rows (sub-stacks of one record): the t-th closing
+// sub-stack is row t of rows itself, cnt is 1, and no open sum is kept.  At n = 2001 one row keeps only 69-87 of the 256 threads
 // busy in the radix-23 and radix-29 passes, so the rows of a workgroup share
 // each transform call (as in analytic_kernel) and the twiddles are staged
 // once for all of them.
@@ -672,8 +673,9 @@ template <int kPer, bool kBlue>
 __global__ void __launch_bounds__(kStkThreads)
 stream_pws_close_kernel(float2 *__restrict__ S, float *__restrict__ open,
                         const int *__restrict__ slot, const int *__restrict__ cnt,
-                        long long nclose, int per_wg, StackFftDev F,
-                        const float2 *__restrict__ tw_f, const float2 *__restrict__ tw_i) {
+                        const float *__restrict__ rows, long long nclose, int per_wg,
+                        StackFftDev F, const float2 *__restrict__ tw_f,
+                        const float2 *__restrict__ tw_i) {
   extern __shared__ float2 lds[];
   const int n = F.n, ld = F.ld, tid = threadIdx.x;
   float2 *twl = lds + (size_t)per_wg * ld;
@@ -682,9 +684,11 @@ stream_pws_close_kernel(float2 *__restrict__ S, float *__restrict__ open,
   const int nt = (int)std::min<long long>(per_wg, nclose - r0);
   for (int i = tid; i < nt * ld; i += kStkThreads) {
     const int t = i / ld, j = i - t * ld;
-    lds[i] = j < n ? make_float2(open[(long long)slot[r0 + t] * n + j] * (1.0f / (float)cnt[r0 + t]),
-                                 0.f)
-                   : make_float2(0.f, 0.f);
+    float v = 0.f;
+    if (j < n)
+      v = rows ? rows[(r0 + t) * n + j]
+               : open[(long long)slot[r0 + t] * n + j] * (1.0f / (float)cnt[r0 + t]);
+    lds[i] = make_float2(v, 0.f);
   }
   __syncthreads();
   stack_fft_dir<-1, kBlue>(lds, nt, F, twl);
@@ -717,9 +721,9 @@ stream_pws_close_kernel(float2 *__restrict__ S, float *__restrict__ open,
   const float invn = 1.f / (float)n;
   for (int t = 0; t < nt; ++t) {
     const long long sl = slot[r0 + t];
-    const float fc = (float)cnt[r0 + t];
+    const float fc = rows ? 1.f : (float)cnt[r0 + t];
     float2 *Sr = S + sl * n;
-    float *o = open + sl * n;
+    float *o = rows ? nullptr : open + sl * n;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const int j = tid + k * kStkThreads;
@@ -730,7 +734,7 @@ stream_pws_close_kernel(float2 *__restrict__ S, float *__restrict__ open,
         mod = mod > 1e-7f ? mod : 1e-7f;
         const float2 v = Sr[j];
         Sr[j] = make_float2(v.x + fc * (re / mod), v.y + fc * (im / mod));
-        o[j] = 0.f;
+        if (o) o[j] = 0.f;
       }
     }
   }
@@ -1012,7 +1016,7 @@ static int stream_pws_launch_check(int n, StackFft *S, size_t *lds) {
 }
 
 int stream_pws_close(float2 *S, float *open, const int *d_slot, const int *d_cnt, long long nclose,
-                     int n, hipStream_t st) {
+                     int n, hipStream_t st, const float *rows) {
   if (nclose == 0) return FXC_OK;
   StackFft F;
   size_t lds = 0;
@@ -1020,18 +1024,18 @@ int stream_pws_close(float2 *S, float *open, const int *d_slot, const int *d_cnt
   const int per = (n + kStkThreads - 1) / kStkThreads;
   // rows per workgroup: as many of kCloseRows as the LDS holds beside the
   // twiddles (3 at n = 2001, where three workgroups share a CU)
-  const int rows = (int)std::min<long long>(
+  const int nrows_wg = (int)std::min<long long>(
       kCloseRows, ((long long)kLdsComplexCapacity - kLdsReserve - F.twc) / F.dev.ld);
-  FXC_CHECK_ARG(rows >= 1, "stream PWS: length %d does not fit the LDS", n);
-  lds = ((size_t)rows * F.dev.ld + F.twc) * sizeof(float2);
-  const unsigned grid = (unsigned)ceil_div((size_t)nclose, (size_t)rows);
+  FXC_CHECK_ARG(nrows_wg >= 1, "stream PWS: length %d does not fit the LDS", n);
+  lds = ((size_t)nrows_wg * F.dev.ld + F.twc) * sizeof(float2);
+  const unsigned grid = (unsigned)ceil_div((size_t)nclose, (size_t)nrows_wg);
 #define FXC_SPC(P)                                                                            \
   if (per <= P) {                                                                            \
     auto *k = F.dev.blue ? stream_pws_close_kernel<P, true> : stream_pws_close_kernel<P, false>; \
     FXC_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,   \
                                 (int)lds));                                                  \
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kStkThreads), lds, st, S, open, d_slot, d_cnt,     \
-                       nclose, rows, F.dev, F.tw_f, F.tw_i);                                 \
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kStkThreads), lds, st, S, open, d_slot, d_cnt, rows, \
+                       nclose, nrows_wg, F.dev, F.tw_f, F.tw_i);                             \
     FXC_HIP(hipGetLastError());                                                              \
     return FXC_OK;                                                                           \
   }

@@ -48,7 +48,7 @@ int xc_set_pairs_on(fxc_xc_plan *p, int nsta, const int *h_src, const int *h_rec
                     hipStream_t stream);
 namespace fxc {
 int stream_pws_close(float2 *S, float *open, const int *d_slot, const int *d_cnt, long long nclose,
-                     int n, hipStream_t st);
+                     int n, hipStream_t st, const float *rows = nullptr);
 int stream_pws_read(const float2 *S, const float *open, const double *acc, const int *d_slot,
                     const long long *d_total, const int *d_nopen, long long npairs, int n,
                     float power, float *out, hipStream_t st);
@@ -269,8 +269,11 @@ int lane_run(fxc_stream *s, StreamLane &L, const StreamJob &job, const std::vect
         set_error("stream: update launch failed");
         rc = FXC_ERR_HIP;
       }
-      // sub-stacks this segment completes: [slots | counts] in d_close
-      if (rc == FXC_OK && nclose > 0)
+      // sub-stacks this segment completes: [slots | counts] in d_close; with
+      // sub-stacks of one record every pair closes, from its new correlation
+      if (rc == FXC_OK && s->pws_B == 1)
+        rc = stream_pws_close(s->d_pws_S, nullptr, L.d_slot, nullptr, np, s->ncc, s->upd, L.d_cc);
+      else if (rc == FXC_OK && nclose > 0)
         rc = stream_pws_close(s->d_pws_S, s->d_pws_open, L.d_close, L.d_close + nclose, nclose,
                               s->ncc, s->upd);
       if (rc == FXC_OK && hipEventRecord(L.ev_done, s->upd) != hipSuccess) {
@@ -591,12 +594,16 @@ extern "C" int fxc_stream_set_pws(fxc_stream *s, int substack) {
     FXC_CHECK_ARG(s->submitted == 0 && s->pws_B == 0,
                   "fxc_stream_set_pws: must be called once, before the first segment");
   }
+  // sub-stacks of one record close as they arrive: no open sums are kept
   const size_t n = (size_t)s->max_pairs * s->ncc;
   FXC_TRY(dmalloc((void **)&s->d_pws_S, n * sizeof(float2)));
-  FXC_TRY(dmalloc((void **)&s->d_pws_open, n * sizeof(float)));
   FXC_HIP(hipMemset(s->d_pws_S, 0, n * sizeof(float2)));
-  FXC_HIP(hipMemset(s->d_pws_open, 0, n * sizeof(float)));
-  s->state_bytes += n * (sizeof(float2) + sizeof(float));
+  s->state_bytes += n * sizeof(float2);
+  if (substack > 1) {
+    FXC_TRY(dmalloc((void **)&s->d_pws_open, n * sizeof(float)));
+    FXC_HIP(hipMemset(s->d_pws_open, 0, n * sizeof(float)));
+    s->state_bytes += n * sizeof(float);
+  }
   s->pws_B = substack;
   return FXC_OK;
 }

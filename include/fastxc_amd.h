@@ -333,7 +333,8 @@ int fxc_stream_read(fxc_stream *stream, int kind, const int *h_src, const int *h
                     float taper_hz, float *d_out, long long *h_count, void *hip_stream);
 /* Running PWS state (call once, before the first segment): every pair then
  * keeps sum_g n_g phi_g over its closed sub-stacks of `substack` records
- * and the sum of its open sub-stack (12 B per lag per pair of max_pairs),
+ * and the sum of its open sub-stack (12 B per lag per pair of max_pairs;
+ * 8 B with substack 1, whose sub-stacks close as they arrive),
  * so fxc_stream_read kind 1 returns the PWS of ALL its records in
  * submission order, equal to fxc_pws over them with sub-stacks of
  * `substack` (the reference's PWS, native/pws/io/sourcepack_io.cu:385-478,

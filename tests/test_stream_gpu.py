@@ -32,12 +32,12 @@ def _plan(tmp_path):
 
 
 def _engine(plan, lanes, history=0, precision="fp32", max_pairs=1 << 12, max_seg_chan=NCHAN,
-            pws_substack=0):
+            pws_substack=0, H=200):
     from fastxc_amd.ops import StreamEngine
 
     resp = np.stack([r[: plan.nfft_filter // 2 + 1] for r in plan.responses])
     return StreamEngine(plan.npts, plan.seg, plan.shift, plan.nfft_filter, plan.nfft_out, 1,
-                        plan.delta, plan.freq_low, plan.idx, resp, plan.freq_lows, NCHAN, 200,
+                        plan.delta, plan.freq_low, plan.idx, resp, plan.freq_lows, NCHAN, H,
                         precision=precision, history=history, max_pairs=max_pairs,
                         max_seg_chan=max_seg_chan, lanes=lanes, pws_substack=pws_substack)
 
@@ -59,7 +59,7 @@ def _run(eng, segs):
     return [eng.wait(t) for t in tickets]
 
 
-def _oracle_ccfs(plan, segs):
+def _oracle_ccfs(plan, segs, H=200):
     """{(a, b): [ccf per segment, submission order]} in FP64."""
     out = {}
     for chan, tr in segs:
@@ -70,7 +70,7 @@ def _oracle_ccfs(plan, segs):
         for i in range(len(order)):
             for j in range(i + 1, len(order)):
                 a, b = order[i], order[j]
-                cc = oxc.correlate(spec, [a], [b], plan.nfft_out, 200, plan.delta)[0]
+                cc = oxc.correlate(spec, [a], [b], plan.nfft_out, H, plan.delta)[0]
                 out.setdefault((int(chan[a]), int(chan[b])), []).append(cc)
     return out
 
@@ -219,4 +219,46 @@ def test_running_pws_and_history_tfpws_match_fp64_oracle(gpu, tmp_path, B):
         s_, w, lin = ostack.tfpws_prepare(recs, B)
         err = rel_l2(got[i], ostack.tfpws(s_, w, lin, delta=plan.delta))
         assert err < 1e-3, (k, err)
+    eng.close()
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+def test_production_length_segments_match_oracle(gpu, tmp_path, precision):
+    """One-hour segments at dt 0.1 s (seg 36000, output FFT 37044, 2001 lags):
+    the lengths of the fused sac2spec path and of xc's one-window product
+    form.  Linear and running PWS (sub-stacks of one record) read-outs
+    against the FP64 oracle's correlations of the same segments."""
+    from oracle import stack as ostack
+
+    path = str(tmp_path / "filter.txt")
+    s2s.write_filter_file(path, 0.1, "0.1/2")
+    plan = s2s.plan_lengths(3600, 0.1, 3600, 3600, 100, s2s.read_filters(path))
+    assert plan.seg == 36000 and plan.nfft_out == 37044
+    H = 1000
+    rng = np.random.default_rng(31)
+    segs = []
+    for _ in range(4):
+        n = int(rng.integers(3, 6))
+        chan = rng.choice(NCHAN, size=n, replace=False).astype(np.int32)
+        tr = rng.standard_normal((n, 36000)).astype(np.float32)
+        tr += np.float32(0.4) * np.sin(np.arange(36000) * 0.03 + chan[:, None]).astype(np.float32)
+        segs.append((chan, tr))
+    want = _oracle_ccfs(plan, segs, H)
+    keys = sorted(want)
+    src = [k[0] for k in keys]
+    rec = [k[1] for k in keys]
+    eng = _engine(plan, lanes=2, precision=precision, max_seg_chan=6, pws_substack=1, H=H)
+    _run(eng, segs)
+    got, cnt = eng.read(src, rec, "linear")
+    got = got.cpu().numpy()
+    for i, k in enumerate(keys):
+        assert cnt[i] == len(want[k])
+        err = rel_l2(got[i], np.mean(want[k], axis=0))
+        assert err < 1e-5, (k, err)
+    got, _ = eng.read(src, rec, "pws", substack=1)
+    got = got.cpu().numpy()
+    for i, k in enumerate(keys):
+        g, w, lin = ostack.pws_prepare(np.stack(want[k]), 1)
+        err = rel_l2(got[i], ostack.pws(g, w, lin))
+        assert err < 1e-4, (k, err)
     eng.close()
