@@ -90,7 +90,7 @@ def parse():
                     help="measure the batched uneven-segment form only")
     ap.add_argument("--no-planner", action="store_true",
                     help="skip the inventory path-planner measurement")
-    ap.add_argument("--stream-rates", default="176,192,224",
+    ap.add_argument("--stream-rates", default="160,192,224",
                     help="Poisson arrival rates (segments/s) of the streaming sweep")
     ap.add_argument("--stream-seconds", type=float, default=30.0,
                     help="arrival time per rate of the streaming sweep")
@@ -811,7 +811,7 @@ def stream_all(rates, seconds):
     """The Z sweep at dt 0.1 s (the rated configuration), then a
     three-component sweep and a dt 0.05 s sweep at two rates each."""
     res = {"z_dt0.1": stream_sweep(rates, seconds)}
-    res["enz_dt0.1"] = stream_sweep([12, 16], max(10.0, seconds / 2), seed=6, ncomp=3, nsta=700,
+    res["enz_dt0.1"] = stream_sweep([8, 12, 16], max(10.0, seconds / 2), seed=6, ncomp=3, nsta=700,
                                     lanes=3, history=0)
     res["z_dt0.05"] = stream_sweep([8, 12], max(10.0, seconds / 2), seed=7, dt=0.05, nsta=1000,
                                    history=0)
