@@ -809,7 +809,7 @@ def stream_sweep(rates, seconds, seed=5, ncomp=1, dt=0.1, nsta=2000, lanes=4, hi
 
 def stream_all(rates, seconds):
     """The Z sweep at dt 0.1 s (the rated configuration), then a
-    three-component sweep and a dt 0.05 s sweep at two rates each."""
+    three-component sweep (three rates) and a dt 0.05 s sweep (two rates)."""
     res = {"z_dt0.1": stream_sweep(rates, seconds)}
     res["enz_dt0.1"] = stream_sweep([8, 12, 16], max(10.0, seconds / 2), seed=6, ncomp=3, nsta=700,
                                     lanes=3, history=0)
