@@ -740,6 +740,12 @@ static_assert(S2sFixF::valid() && S2sFixS::valid() && S2sFixO::valid(), "plans")
 constexpr int kFusedPer = 37;   // RUN-ABS outputs per thread (odd: conflict-free LDS reads)
 constexpr int kFusedPerB = 19;  // whitening bins per thread
 static_assert(kFusedPer * kS2sThreads >= 2 * S2sFixS::kM, "RUN-ABS blocks cover seg");
+static_assert(kFusedPerB * kS2sThreads >= S2sFixS::kM + 1, "whitening blocks cover the bins");
+// float2 slots after the tables: the block sums of the running means
+// (kS2sThreads floats) and, in the back kernel's whitening, the two lane
+// exchanges of whiten_mean_lanes (2 x 16 x 2 x kFusedPerB floats)
+constexpr int kFusedBsSlots =
+    kS2sThreads / 2 > 2 * 16 * kFusedPerB ? kS2sThreads / 2 : 2 * 16 * kFusedPerB;
 static_assert(kFusedPerB * kS2sThreads >= S2sFixS::kM + 1, "whitening blocks cover seg/2+1");
 
 struct FixedTables {
@@ -762,7 +768,7 @@ template <class FPA, class FPB>
 constexpr int fused_lds_count() {
   return (FPA::kM > FPB::kM ? FPA::kM : FPB::kM) + 1 + fixed_lds_tw_count<FPA>() +
          fixed_lds_tw_count<FPB>() + fused_fold_count(FPA::kM) + fused_fold_count(FPB::kM) +
-         kS2sThreads / 2;
+         kFusedBsSlots;
 }
 
 // After fixed_fft_fwd of the packed real row: X[k], k = 0..M, to st(k, X[k])
@@ -918,6 +924,73 @@ __device__ __forceinline__ void sliding_mean_blocked(int n, int win, const A &a,
       ++start;
       ++end;
       if (end >= 0 && end < n) sum.add(a(end));
+    }
+  }
+}
+
+// sliding_mean of |X[k]| over kWin bins for the kFusedPerB outputs of each
+// thread, with every |X| formed once: a thread forms its own kFusedPerB
+// values and takes those of the two threads on each side from their lanes
+// (__shfl), or, across a wave boundary, from xl (lanes 62, 63 of every wave)
+// and xr (lanes 0, 1), 16 x 2 x kFusedPerB floats of LDS each.  Values
+// outside [0, nbin) are 0, which the sums add exactly, in the same order as
+// sliding_mean, so the weights are identical to it.  Every thread must call
+// it (it holds a barrier).
+template <int kWin, class Acc, class U>
+__device__ __forceinline__ void whiten_mean_lanes(const float2 *X, int nbin, float *xl, float *xr,
+                                                  const U &use) {
+  constexpr int P = kFusedPerB;
+  constexpr int H = kWin / 2;  // window of output t: bins [t - H, t - H + kWin - 1]
+  static_assert(-H >= -2 * P && P - 1 - H + kWin < 3 * P, "neighbours within two threads");
+  constexpr int kWaves = kS2sThreads / 64;
+  const int T = threadIdx.x, lane = T & 63, wave = T >> 6;
+  const int t0 = T * P;
+  float own[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int k = t0 + m;
+    float v = 0.f;
+    if (k < nbin) {
+      const float2 z = X[k];
+      v = sqrtf(z.x * z.x + z.y * z.y);
+    }
+    own[m] = v;
+  }
+  if (lane >= 62) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) xl[(wave * 2 + lane - 62) * P + m] = own[m];
+  }
+  if (lane < 2) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) xr[(wave * 2 + lane) * P + m] = own[m];
+  }
+  __syncthreads();
+  // |X| at bin t0 + r (r a constant after unrolling)
+  auto val = [&](int r) -> float {
+    const int d = r >= 0 ? r / P : -((P - 1 - r) / P);
+    const int m = r - d * P;
+    if (d == 0) return own[m];
+    float v = __shfl(own[m], lane + d);
+    const int l2 = lane + d;
+    if (l2 < 0)
+      v = wave > 0 ? xl[((wave - 1) * 2 + l2 + 2) * P + m] : 0.f;
+    else if (l2 > 63)
+      v = wave < kWaves - 1 ? xr[((wave + 1) * 2 + l2 - 64) * P + m] : 0.f;
+    return v;
+  };
+  Acc sum;
+#pragma unroll
+  for (int r = -H; r <= kWin - 1 - H; ++r) sum.add(val(r));
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int t = t0 + m;
+    const float vs = val(m - H), va = val(m - H + kWin);
+    if (t < nbin) {
+      float w = sum.mean(kWin);
+      if (!isfinite(w) || w < kMinVal) w = kMinVal;
+      use(m, t, w);
+      sum.sub(vs);
+      sum.add(va);
     }
   }
 }
@@ -1120,7 +1193,14 @@ s2s_back_kernel(float2 *X, long long fpitch, int seg, int i1, int i2, int i3, in
     return sqrtf(z.x * z.x + z.y * z.y);
   };
   auto keep = [&](int m, int, float d) { wk[m] = d; };
-  if (comp_sums)
+  // the window of one-hour windows at dt 0.1 s (int(0.02 seg dt) = 72 bins)
+  // forms each |X| once; other windows slide on sliding_mean
+  float *xl = bs + 16 * 2 * kFusedPerB;  // xr = bs [0, 16 x 2 x kFusedPerB), xl after it
+  if (whiten_win == 72 && comp_sums)
+    whiten_mean_lanes<72, AccComp>(lds, nbin, xl, bs, keep);
+  else if (whiten_win == 72)
+    whiten_mean_lanes<72, Acc64>(lds, nbin, xl, bs, keep);
+  else if (comp_sums)
     sliding_mean<kFusedPerB, AccComp>(nbin, whiten_win, absx, keep);
   else
     sliding_mean<kFusedPerB, Acc64>(nbin, whiten_win, absx, keep);
