@@ -1780,6 +1780,35 @@ static int xc_contract_chunk(fxc_xc_plan *p, const float2 *d_spec, long long c, 
 
 // FXC_XC_LAG_R0=0 selects the production-length lag kernel without the
 // fold in pass 0 (lag_kernel_fixed).
+// large_fold_kernel of the one-window stack X[k] = conj(S[src][k]) S[rec][k]
+// (rows r0.. of the product rows), formed from the spectra (rows of pitch) as
+// they are read.
+__global__ void large_fold_prod_kernel(const float2 *__restrict__ S, long long pitch,
+                                       const int *__restrict__ src, const int *__restrict__ rec,
+                                       int M, const float2 *__restrict__ half,
+                                       float2 *__restrict__ Z) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  const long long row = blockIdx.y;
+  const float2 *ss = S + (size_t)src[row] * pitch, *sr = S + (size_t)rec[row] * pitch;
+  auto x = [&](int i) {
+    const float2 u = ss[i], v = sr[i];
+    return make_float2(u.x * v.x + u.y * v.y, u.x * v.y - u.y * v.x);
+  };
+  float2 v;
+  if (k == 0) {
+    const float a = x(0).x, b = x(M).x;
+    v = make_float2(a + b, a - b);
+  } else {
+    const float2 a = x(k);
+    const float2 b = cconj(x(M - k));
+    const float2 sv = cadd(a, b);
+    const float2 d = cmul(half[k], csub(a, b));
+    v = make_float2(sv.x - d.y, sv.y + d.x);
+  }
+  Z[row * M + k] = v;
+}
+
 static bool xc_lag_r0_on() {
   static const bool off = [] {
     const char *e = getenv("FXC_XC_LAG_R0");
@@ -1796,8 +1825,8 @@ static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long
                         const float *task_scale = nullptr, const int *prod_src = nullptr,
                         const int *prod_rec = nullptr) {
   if (nrows == 0) return FXC_OK;
-  FXC_CHECK_ARG(!prod_src || (p->d_fix_tw && !p->large_lag && xc_lag_r0_on()),
-                "xc: the one-window product form needs the compile-time lag plan");
+  FXC_CHECK_ARG(!prod_src || p->large_lag || (p->d_fix_tw && xc_lag_r0_on()),
+                "xc: the one-window product form needs the compile-time or four-step lag");
   const double weight0 = 1.0 / (double)((float)p->nfft * p->dt);
   if (p->large_lag) {
     // four-step inverse of the folded spectrum, kLagLargeChunk rows at a time
@@ -1809,8 +1838,13 @@ static int xc_lags_rows(fxc_xc_plan *p, const float2 *xs, long long xpitch, long
     FXC_TRY(xc_mark(p, kTimeLag, true, st));
     for (long long r0 = 0; r0 < nrows; r0 += kLagLargeChunk) {
       const long long nr = std::min<long long>(kLagLargeChunk, nrows - r0);
-      hipLaunchKernelGGL(large_fold_kernel, dim3((unsigned)ceil_div(M, 256), (unsigned)nr),
-                         dim3(256), 0, st, xs + r0 * xpitch, xpitch, M, lp->d_half, w0);
+      if (prod_src)
+        hipLaunchKernelGGL(large_fold_prod_kernel, dim3((unsigned)ceil_div(M, 256), (unsigned)nr),
+                           dim3(256), 0, st, xs, xpitch, prod_src + r0, prod_rec + r0, M,
+                           lp->d_half, w0);
+      else
+        hipLaunchKernelGGL(large_fold_kernel, dim3((unsigned)ceil_div(M, 256), (unsigned)nr),
+                           dim3(256), 0, st, xs + r0 * xpitch, xpitch, M, lp->d_half, w0);
       FXC_HIP(hipGetLastError());
       FXC_TRY(large_c2c(lp, w0, M, w0, M, nr, w1, 1.f, st));
       hipLaunchKernelGGL(lag_extract_kernel,
@@ -1906,13 +1940,13 @@ extern "C" int fxc_xc_lags(fxc_xc_plan *p, const float *d_xspec, long long nrows
 }
 
 // Full correlation: cc[task][0..2H] for every task set by fxc_xc_set_pairs.
-// One window at the compile-time lag length, every bin: a pair's stack is the
-// product conj(S[src]) S[rec] of its two spectra, formed in FP32 by the lag
-// kernel as it reads them (the plan's operand form is not used: there is no
-// sum over windows to contract).
+// One window at the compile-time lag length or a four-step one, every bin: a
+// pair's stack is the product conj(S[src]) S[rec] of its two spectra, formed
+// in FP32 as the lag synthesis reads them (the plan's operand form is not
+// used: there is no sum over windows to contract).
 static bool xc_one_window_product(const fxc_xc_plan *p) {
-  return p->nstep == 1 && p->precision != FXC_PREC_PLAIN && p->band_c1 < 0 && p->d_fix_tw &&
-         !p->large_lag && xc_lag_r0_on() && p->d_row_src && p->d_row_rec;
+  return p->nstep == 1 && p->precision != FXC_PREC_PLAIN && p->band_c1 < 0 &&
+         (p->large_lag || (p->d_fix_tw && xc_lag_r0_on())) && p->d_row_src && p->d_row_rec;
 }
 
 extern "C" int fxc_xc_correlate(fxc_xc_plan *p, const float *d_spec, float *d_cc, void *stream) {

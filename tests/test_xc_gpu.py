@@ -190,6 +190,33 @@ def test_one_window_product_form(gpu, precision):
 
 
 @pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+def test_one_window_product_form_dt005(gpu, precision):
+    """One window at nfft 74088 (dt 0.05 s): the four-step lag synthesis folds
+    the product conj(S[src]) S[rec] straight from the spectra; against FP64
+    and the PLAIN form, pairs in both orientations."""
+    import torch
+    from fastxc_amd.ops import XcPlan
+
+    rng = np.random.default_rng(75)
+    nsta, seg, nfft, H, dt = 9, 72000, 74088, 2000, 0.05
+    spec = _spectra(rng, 1, nsta, seg, nfft, dt)
+    src, rec = _all_pairs(nsta)
+    src, rec = np.concatenate([src, rec[:5]]), np.concatenate([rec, src[:5]])
+    src, rec = src.astype(np.int32), rec.astype(np.int32)
+    sd = torch.from_numpy(spec).to(gpu)
+    plan = XcPlan(nfft // 2 + 1, 1, dt, H, precision)
+    plan.set_pairs(nsta, src, rec)
+    cc = plan.correlate(sd).cpu().numpy()
+    want = xc.correlate(spec, src, rec, nfft, H, dt)
+    err = xc.rel_l2(cc, want)
+    print(f"nfft 74088 one window {precision} rel_l2={err:.3e}")
+    assert cc.shape == (len(src), 4001) and err < 1e-5
+    ref = XcPlan(nfft // 2 + 1, 1, dt, H, "plain")
+    ref.set_pairs(nsta, src, rec)
+    assert xc.rel_l2(cc, ref.correlate(sd).cpu().numpy()) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
 def test_uneven_segments_with_gaps(gpu, precision):
     """Segments of different station and window counts one after another, 10 %
     of station-windows zero-filled (gaps).  As the reference does, a gap
