@@ -101,9 +101,10 @@ int fxc_xc_get_timing(fxc_xc_plan *plan, double *ms3, long long *launches3);
 int fxc_xc_set_band(fxc_xc_plan *plan, int k_lo, int k_hi);
 /* Stack + lag synthesis for every task (execute_row_batch_job,
  * native/xc/worker/async_workers.cu:18-86, minus file I/O).  With one
- * window at nfft 37044 (every bin, any form but PLAIN) there is no sum over
- * windows: the stack conj(S[src]) S[rec] is formed in FP32 by the lag
- * kernel as it reads the two spectra, whatever the plan's operand form. */
+ * window at nfft 37044 or at a four-step lag length such as 74088 (every
+ * bin, any form but PLAIN) there is no sum over windows: the stack
+ * conj(S[src]) S[rec] is formed in FP32 by the lag synthesis as it reads
+ * the two spectra, whatever the plan's operand form. */
 int fxc_xc_correlate(fxc_xc_plan *plan, const float *d_spec, float *d_cc, void *stream);
 
 /* Uneven time segments in one set of launches (split-bf16 form only).
