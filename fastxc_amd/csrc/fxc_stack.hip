@@ -836,7 +836,16 @@ linear_stack_kernel(const float *__restrict__ x, const long long *__restrict__ o
   const long long r0 = off[pair], r1 = off[pair + 1];
   double acc = 0.0;
   const float *p = x + (size_t)r0 * n + j;
-  for (long long r = r0; r < r1; ++r, p += n) acc += (double)*p;
+  long long r = r0;
+  // eight records' loads in flight at once, added in record order
+  for (; r + 8 <= r1; r += 8, p += 8 * (size_t)n) {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = p[i * (size_t)n];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc += (double)v[i];
+  }
+  for (; r < r1; ++r, p += n) acc += (double)*p;
   out[(size_t)pair * n + j] = (float)(acc / (double)(r1 - r0));
 }
 
