@@ -139,6 +139,7 @@ __device__ __forceinline__ void stack_fft_dir(float2 *buf, int nt, const StackFf
 // non-finite value transforms its rows one by one instead, so such a value
 // stays in its own row, as in a transform per row.
 constexpr int kAnalyticMaxPer = 3;
+constexpr int kLoadBatch = 8;  // loads a thread issues before it uses the first
 
 template <bool kBlue>
 __global__ void __launch_bounds__(kStkThreads)
@@ -161,18 +162,36 @@ analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, StackFf
   float m[2 * kAnalyticMaxPer];
 #pragma unroll
   for (int r = 0; r < 2 * kAnalyticMaxPer; ++r) m[r] = 0.f;
-  for (int i = tid; i < nt * n; i += blockDim.x) {
-    const int t = i / n, j = i - t * n;
-    const float a = x[(r0 + 2 * t) * n + j];
-    const float b = 2 * t + 1 < nr ? x[(r0 + 2 * t + 1) * n + j] : 0.f;
-    lds[t * ld + j] = make_float2(a, b);
-    nonfinite |= !isfinite(a) || !isfinite(b);
+  // kLoadBatch positions' loads are issued before the first is used
+  const int total = nt * n;
+  for (int i0 = tid; i0 < total; i0 += kLoadBatch * (int)blockDim.x) {
+    float av[kLoadBatch], bv[kLoadBatch];
 #pragma unroll
-    for (int r = 0; r < kAnalyticMaxPer; ++r)
-      if (r == t) {
-        m[2 * r] = fmaxf(m[2 * r], fabsf(a));
-        m[2 * r + 1] = fmaxf(m[2 * r + 1], fabsf(b));
+    for (int u = 0; u < kLoadBatch; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      av[u] = bv[u] = 0.f;
+      if (i < total) {
+        const int t = i / n, j = i - t * n;
+        av[u] = x[(r0 + 2 * t) * n + j];
+        if (2 * t + 1 < nr) bv[u] = x[(r0 + 2 * t + 1) * n + j];
       }
+    }
+#pragma unroll
+    for (int u = 0; u < kLoadBatch; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < total) {
+        const int t = i / n, j = i - t * n;
+        const float a = av[u], b = bv[u];
+        lds[t * ld + j] = make_float2(a, b);
+        nonfinite |= !isfinite(a) || !isfinite(b);
+#pragma unroll
+        for (int r = 0; r < kAnalyticMaxPer; ++r)
+          if (r == t) {
+            m[2 * r] = fmaxf(m[2 * r], fabsf(a));
+            m[2 * r + 1] = fmaxf(m[2 * r + 1], fabsf(b));
+          }
+      }
+    }
   }
 #pragma unroll
   for (int r = 0; r < 2 * kAnalyticMaxPer; ++r)
@@ -279,9 +298,20 @@ pws_kernel(const float2 *__restrict__ A, const float *__restrict__ weights,
   for (long long gb = g0; gb < g1; gb += nbatch) {
     const int nb = (int)std::min<long long>(nbatch, g1 - gb);
     __syncthreads();
-    for (int i = tid; i < nb * n; i += kThreads) {
-      const int b = i / n, c = i - b * n;
-      buf[b * ld + c] = A[(gb + b) * n + c];
+    // half a batch: W already holds 2 kPer registers here
+    constexpr int kB = kLoadBatch / 2;
+    for (int i0 = tid; i0 < nb * n; i0 += kB * kThreads) {
+      float2 v[kB];
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        const int i = i0 + u * kThreads;
+        if (i < nb * n) v[u] = A[(gb + i / n) * n + (i - (i / n) * n)];
+      }
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        const int i = i0 + u * kThreads;
+        if (i < nb * n) buf[(i / n) * ld + (i - (i / n) * n)] = v[u];
+      }
     }
     __syncthreads();
     stack_fft(buf, nb, F, twl, 1.f);
@@ -682,13 +712,22 @@ stream_pws_close_kernel(float2 *__restrict__ S, float *__restrict__ open,
   stage_stack_tw(twl, F, tw_f, tw_i);
   const long long r0 = (long long)blockIdx.x * per_wg;
   const int nt = (int)std::min<long long>(per_wg, nclose - r0);
-  for (int i = tid; i < nt * ld; i += kStkThreads) {
-    const int t = i / ld, j = i - t * ld;
-    float v = 0.f;
-    if (j < n)
-      v = rows ? rows[(r0 + t) * n + j]
-               : open[(long long)slot[r0 + t] * n + j] * (1.0f / (float)cnt[r0 + t]);
-    lds[i] = make_float2(v, 0.f);
+  for (int i0 = tid; i0 < nt * ld; i0 += kLoadBatch * kStkThreads) {
+    float v[kLoadBatch];
+#pragma unroll
+    for (int u = 0; u < kLoadBatch; ++u) {
+      const int i = i0 + u * kStkThreads;
+      const int t = i / ld, j = i - t * ld;
+      v[u] = 0.f;
+      if (i < nt * ld && j < n)
+        v[u] = rows ? rows[(r0 + t) * n + j]
+                    : open[(long long)slot[r0 + t] * n + j] * (1.0f / (float)cnt[r0 + t]);
+    }
+#pragma unroll
+    for (int u = 0; u < kLoadBatch; ++u) {
+      const int i = i0 + u * kStkThreads;
+      if (i < nt * ld) lds[i] = make_float2(v[u], 0.f);
+    }
   }
   __syncthreads();
   stack_fft_dir<-1, kBlue>(lds, nt, F, twl);
