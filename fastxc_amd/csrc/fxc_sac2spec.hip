@@ -153,20 +153,37 @@ __device__ inline void spectral_emit(const float2 *lds, const FftPlanDev &plan, 
 // pads to 2M samples.
 __device__ inline void load_row(float *ldsf, int twoM, const float *__restrict__ x, int seg,
                                 int mode, const float *__restrict__ w) {
-  for (int i = threadIdx.x; i < twoM; i += blockDim.x) {
-    float v = 0.f;
-    if (i < seg) {
-      v = x[i];
-      if (mode == kLoadRunabs) {
-        const float d = w[i];
-        v = (!isfinite(d) || d < kMinValD) ? 0.f : v / d;
-        if (!isfinite(v)) v = 0.f;
-        v = v > kMaxVal ? kMaxVal : (v < -kMaxVal ? -kMaxVal : v);
-      } else if (mode == kLoadOnebit) {
-        v = v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f);
+  constexpr int kB = 8;  // samples whose loads are issued before the first is used
+  for (int i0 = threadIdx.x; i0 < twoM; i0 += kB * blockDim.x) {
+    float xv[kB], wv[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int i = i0 + u * blockDim.x;
+      xv[u] = 0.f;
+      wv[u] = 0.f;
+      if (i < seg && i < twoM) {
+        xv[u] = x[i];
+        if (mode == kLoadRunabs) wv[u] = w[i];
       }
     }
-    ldsf[i] = v;
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int i = i0 + u * blockDim.x;
+      if (i >= twoM) continue;
+      float v = 0.f;
+      if (i < seg) {
+        v = xv[u];
+        if (mode == kLoadRunabs) {
+          const float d = wv[u];
+          v = (!isfinite(d) || d < kMinValD) ? 0.f : v / d;
+          if (!isfinite(v)) v = 0.f;
+          v = v > kMaxVal ? kMaxVal : (v < -kMaxVal ? -kMaxVal : v);
+        } else if (mode == kLoadOnebit) {
+          v = v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f);
+        }
+      }
+      ldsf[i] = v;
+    }
   }
   __syncthreads();
 }
@@ -442,25 +459,42 @@ s2s_whiten_inv_kernel(const float2 *__restrict__ X, long long fpitch, const floa
   const float *wr = ww + (size_t)frame * fpitch;
   const int nbin = Ms + 1;
   const int t4 = i4 < nbin ? i4 : nbin;
-  auto whitened = [&](int k) -> float2 {
-    float2 v = Xr[k];
-    const float d = wr[k];
+  auto whitened_from = [&](int k, float2 v, float d) -> float2 {
     v = (!isfinite(d) || d < kMinValD) ? make_float2(0.f, 0.f) : make_float2(v.x / d, v.y / d);
     v = cscale(v, spec_taper(k, i1, i2, i3, t4));
     if (!cfinite(v)) v = make_float2(0.f, 0.f);
     return v;
   };
-  // fold into digit-reversed positions for the DIT inverse
-  for (int k = threadIdx.x; k < Ms; k += blockDim.x) {
-    float2 a = whitened(k);
-    float2 b = cconj(whitened(Ms - k));
-    if (k == 0) {
-      a.y = 0.f;
-      b.y = 0.f;
+  // fold into digit-reversed positions for the DIT inverse; a batch of
+  // bins' loads is issued before the first is used
+  constexpr int kB = 4;
+  for (int k0 = threadIdx.x; k0 < Ms; k0 += kB * blockDim.x) {
+    float2 xa[kB], xb[kB];
+    float da[kB], db[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int k = k0 + u * blockDim.x;
+      if (k < Ms) {
+        xa[u] = Xr[k];
+        da[u] = wr[k];
+        xb[u] = Xr[Ms - k];
+        db[u] = wr[Ms - k];
+      }
     }
-    const float2 s = cadd(a, b);
-    const float2 d = cmul(cconj(fold_lookup(fold_s, k)), csub(a, b));
-    lds[fft_pos(ps, k)] = make_float2(s.x - d.y, s.y + d.x);
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int k = k0 + u * blockDim.x;
+      if (k >= Ms) continue;
+      float2 a = whitened_from(k, xa[u], da[u]);
+      float2 b = cconj(whitened_from(Ms - k, xb[u], db[u]));
+      if (k == 0) {
+        a.y = 0.f;
+        b.y = 0.f;
+      }
+      const float2 s = cadd(a, b);
+      const float2 d = cmul(cconj(fold_lookup(fold_s, k)), csub(a, b));
+      lds[fft_pos(ps, k)] = make_float2(s.x - d.y, s.y + d.x);
+    }
   }
   __syncthreads();
   lds_fft_dit(lds, 1, ps, tws_inv, 1.f);
