@@ -822,12 +822,23 @@ __device__ __forceinline__ void fixed_r2c_emit(const float2 *lds, const float2 *
   }
 }
 
-// spectral_pairs at the plan's digit-reversed positions: f(k, X[k], X[M-k])
-// -> (Y[k], Y[M-k]), refolded in place for fixed_fft_inv.
+// Pair trips of fixed_spectral_pairs per thread: pairs k = 0..M/2 over
+// kS2sThreads threads.
+template <class FP>
+constexpr int fixed_pair_trips() {
+  return (FP::kM / 2 + kS2sThreads) / kS2sThreads;
+}
+
+// spectral_pairs at the plan's digit-reversed positions: f(it, k, X[k],
+// X[M-k]) -> (Y[k], Y[M-k]) for the it-th pair of the thread, k = tid + it
+// kS2sThreads, refolded in place for fixed_fft_inv.
 template <class FP, class F>
 __device__ __forceinline__ void fixed_spectral_pairs(float2 *lds, const float2 *fold, const F &f) {
   constexpr int M = FP::kM;
-  for (int k = threadIdx.x; 2 * k <= M; k += blockDim.x) {
+#pragma unroll
+  for (int it = 0; it < fixed_pair_trips<FP>(); ++it) {
+    const int k = (int)threadIdx.x + it * kS2sThreads;
+    if (2 * k > M) continue;
     const int pk = fixed_pos<FP>(k);
     const int pm = fixed_pos<FP>(k == 0 ? 0 : M - k);
     const float2 a = lds[pk];
@@ -849,7 +860,7 @@ __device__ __forceinline__ void fixed_spectral_pairs(float2 *lds, const float2 *
       xm = cadd(e2, cmul(fold_lookup(fold, M - k), o2));
     }
     float2 yk, ym;
-    f(k, xk, xm, yk, ym);
+    f(it, k, xk, xm, yk, ym);
     if (k == 0) {
       lds[pk] = make_float2(yk.x + ym.x, yk.x - ym.x);
     } else {
@@ -1162,15 +1173,24 @@ s2s_front_kernel(const float *__restrict__ traces, int npts, int nfiles,
   preprocess_row<2 * FPF::kM>(ldsf, traces + (size_t)file * npts,
                               (long long)steps[step] * shift, npts, seg, taper_size);
   constexpr int Mf = FPF::kM;
+  // this thread's response values, loaded while the forward transform runs
+  float2 rsp[fixed_pair_trips<FPF>()];
+#pragma unroll
+  for (int it = 0; it < fixed_pair_trips<FPF>(); ++it) {
+    const int k = (int)threadIdx.x + it * kS2sThreads;
+    rsp[it] = make_float2(0.f, 0.f);
+    if (2 * k <= Mf) rsp[it] = make_float2(resp0[k], resp0[k == 0 ? Mf : Mf - k]);
+  }
   fixed_fft_fwd<FPF, kS2sThreads>(lds, tf.tw0, twlf);
-  fixed_spectral_pairs<FPF>(lds, foldf, [&](int k, float2 xk, float2 xm, float2 &yk, float2 &ym) {
-    xk = cscale(xk, dt);
-    xm = cscale(xm, dt);
-    if (!cfinite(xk)) xk = make_float2(0.f, 0.f);
-    if (!cfinite(xm)) xm = make_float2(0.f, 0.f);
-    yk = cscale(xk, resp0[k]);
-    ym = cscale(xm, resp0[k == 0 ? Mf : Mf - k]);
-  });
+  fixed_spectral_pairs<FPF>(lds, foldf,
+                            [&](int it, int, float2 xk, float2 xm, float2 &yk, float2 &ym) {
+                              xk = cscale(xk, dt);
+                              xm = cscale(xm, dt);
+                              if (!cfinite(xk)) xk = make_float2(0.f, 0.f);
+                              if (!cfinite(xm)) xm = make_float2(0.f, 0.f);
+                              yk = cscale(xk, rsp[it].x);
+                              ym = cscale(xm, rsp[it].y);
+                            });
   fixed_fft_inv<FPF, kS2sThreads>(lds, tf.tw0, twlf);
   for (int i = threadIdx.x; i < seg; i += blockDim.x) ldsf[i] = (float)((double)ldsf[i] * inv_weight);
   __syncthreads();
