@@ -1075,6 +1075,16 @@ __device__ __forceinline__ void normalise_row(float *x, int n, int mode, int win
 // into the float view of LDS, zero padded to kTwoM samples
 // (compute.cu:34-58, core/preprocess.cu:23-105).  Every load of the window
 // is issued before the first LDS store, so their latencies overlap.
+// Streaming row accesses of the fused kernels (each byte is touched once):
+// nontemporal, so the small tables every workgroup reads stay in L2.
+__device__ __forceinline__ float stream_load(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float2 stream_load(const float2 *p) {
+  return __builtin_bit_cast(float2, __builtin_nontemporal_load((const unsigned long long *)p));
+}
+__device__ __forceinline__ void stream_store(float2 *p, float2 v) {
+  __builtin_nontemporal_store(__builtin_bit_cast(unsigned long long, v), (unsigned long long *)p);
+}
+
 template <int kTwoM>
 __device__ __forceinline__ void preprocess_row(float *ldsf, const float *__restrict__ src, long long off,
                                       int npts, int seg, int taper_size) {
@@ -1084,7 +1094,7 @@ __device__ __forceinline__ void preprocess_row(float *ldsf, const float *__restr
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int i = threadIdx.x + j * kS2sThreads;
-    v[j] = (i < seg && off + i < npts) ? src[off + i] : 0.f;
+    v[j] = (i < seg && off + i < npts) ? stream_load(src + off + i) : 0.f;
   }
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
@@ -1147,7 +1157,7 @@ __device__ __forceinline__ void fused_emit(float2 *lds, int seg, const float2 *_
       v = (!isfinite(a) || a <= kMinVal) ? make_float2(0.f, 0.f) : cscale(v, 1.f / a);
       if (!cfinite(v)) v = make_float2(0.f, 0.f);
     }
-    dst[k] = v;
+    stream_store(dst + k, v);
   });
 }
 
@@ -1228,7 +1238,7 @@ s2s_back_kernel(float2 *X, long long fpitch, int seg, int i1, int i2, int i3, in
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
       const int k = threadIdx.x + j * kS2sThreads;
-      if (k <= Ms) v[j] = Xr[k];
+      if (k <= Ms) v[j] = stream_load(Xr + k);
     }
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
