@@ -147,6 +147,17 @@ __device__ inline void spectral_emit(const float2 *lds, const FftPlanDev &plan, 
   r2c_unfold(lds, plan, fold, st);
 }
 
+// Streaming row accesses (each byte of a row is touched once per kernel):
+// nontemporal, so the small tables every workgroup reads stay in L2.
+__device__ __forceinline__ float stream_load(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float2 stream_load(const float2 *p) {
+  return __builtin_bit_cast(float2, __builtin_nontemporal_load((const unsigned long long *)p));
+}
+__device__ __forceinline__ void stream_store(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void stream_store(float2 *p, float2 v) {
+  __builtin_nontemporal_store(__builtin_bit_cast(unsigned long long, v), (unsigned long long *)p);
+}
+
 // Loads one row of the time series into the float view of LDS applying the
 // pending time-domain normalisation (reference order: x/w with 0 for an
 // invalid weight, non-finite -> 0, clip to +-MAXVAL; or sign(x)), then zero
@@ -162,7 +173,7 @@ __device__ inline void load_row(float *ldsf, int twoM, const float *__restrict__
       xv[u] = 0.f;
       wv[u] = 0.f;
       if (i < seg && i < twoM) {
-        xv[u] = x[i];
+        xv[u] = stream_load(x + i);
         if (mode == kLoadRunabs) wv[u] = w[i];
       }
     }
@@ -206,14 +217,25 @@ s2s_prefilter_kernel(const float *__restrict__ traces, int npts, int nfiles,
   const int step = row / nfiles, file = row - step * nfiles;
   const long long off = (long long)steps[step] * shift;
   const float *src = traces + (size_t)file * npts;
-  // load + non-finite -> 0 + FP64 sum
+  // load + non-finite -> 0 + FP64 sum; a batch of samples' loads is
+  // issued before the first is used
   double s = 0.0, unused1 = 0.0, unused2 = 0.0;
-  for (int i = threadIdx.x; i < 2 * M; i += blockDim.x) {
-    float v = 0.f;
-    if (i < seg && off + i < npts) v = src[off + i];
-    if (!isfinite(v)) v = 0.f;
-    ldsf[i] = v;
-    if (i < seg) s += v;
+  constexpr int kB = 8;
+  for (int i0 = threadIdx.x; i0 < 2 * M; i0 += kB * blockDim.x) {
+    float xv[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int i = i0 + u * blockDim.x;
+      xv[u] = (i < seg && off + i < npts) ? stream_load(src + off + i) : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int i = i0 + u * blockDim.x;
+      if (i >= 2 * M) continue;
+      const float v = isfinite(xv[u]) ? xv[u] : 0.f;
+      ldsf[i] = v;
+      if (i < seg) s += v;
+    }
   }
   block_sum3(s, unused1, unused2);
   // demean: x -= sum/W (double), then sums for the linear trend
@@ -255,7 +277,8 @@ s2s_prefilter_kernel(const float *__restrict__ traces, int npts, int nfiles,
   });
   lds_fft_dit(lds, 1, plan, tw_inv, 1.f);
   float *y = xout + (size_t)row * seg;
-  for (int i = threadIdx.x; i < seg; i += blockDim.x) y[i] = (float)((double)ldsf[i] * inv_weight);
+  for (int i = threadIdx.x; i < seg; i += blockDim.x)
+    stream_store(y + i, (float)((double)ldsf[i] * inv_weight));
 }
 
 // ---------------------------------------------------------------------------
@@ -417,8 +440,8 @@ s2s_whiten_fwd_kernel(const float *__restrict__ x, int seg, int nch, int mode,
   spectral_emit(lds, plan, fold, [&](int k, float2 v) {
     v = cscale(v, dt);
     if (!cfinite(v)) v = make_float2(0.f, 0.f);
-    Xr[k] = v;
-    ar[k] = sqrtf(v.x * v.x + v.y * v.y);
+    stream_store(Xr + k, v);
+    stream_store(ar + k, sqrtf(v.x * v.x + v.y * v.y));
   });
 }
 
@@ -475,9 +498,9 @@ s2s_whiten_inv_kernel(const float2 *__restrict__ X, long long fpitch, const floa
     for (int u = 0; u < kB; ++u) {
       const int k = k0 + u * blockDim.x;
       if (k < Ms) {
-        xa[u] = Xr[k];
+        xa[u] = stream_load(Xr + k);
         da[u] = wr[k];
-        xb[u] = Xr[Ms - k];
+        xb[u] = stream_load(Xr + Ms - k);
         db[u] = wr[Ms - k];
       }
     }
@@ -501,7 +524,8 @@ s2s_whiten_inv_kernel(const float2 *__restrict__ X, long long fpitch, const floa
   float *ldsf = (float *)lds;
   if (!do_out) {
     float *y = xout + (size_t)row * seg;
-    for (int i = threadIdx.x; i < seg; i += blockDim.x) y[i] = (float)((double)ldsf[i] * inv_weight_seg);
+    for (int i = threadIdx.x; i < seg; i += blockDim.x)
+      stream_store(y + i, (float)((double)ldsf[i] * inv_weight_seg));
     return;
   }
   for (int i = threadIdx.x; i < 2 * Mo; i += blockDim.x)
@@ -518,7 +542,7 @@ s2s_whiten_inv_kernel(const float2 *__restrict__ X, long long fpitch, const floa
       v = (!isfinite(a) || a <= kMinVal) ? make_float2(0.f, 0.f) : cscale(v, 1.f / a);
       if (!cfinite(v)) v = make_float2(0.f, 0.f);
     }
-    o[k] = v;
+    stream_store(o + k, v);
   });
 }
 
@@ -548,7 +572,7 @@ s2s_output_kernel(const float *__restrict__ x, int seg, int nch, int mode,
       v = (!isfinite(a) || a <= kMinVal) ? make_float2(0.f, 0.f) : cscale(v, 1.f / a);
       if (!cfinite(v)) v = make_float2(0.f, 0.f);
     }
-    o[k] = v;
+    stream_store(o + k, v);
   });
 }
 
@@ -1075,16 +1099,6 @@ __device__ __forceinline__ void normalise_row(float *x, int n, int mode, int win
 // into the float view of LDS, zero padded to kTwoM samples
 // (compute.cu:34-58, core/preprocess.cu:23-105).  Every load of the window
 // is issued before the first LDS store, so their latencies overlap.
-// Streaming row accesses of the fused kernels (each byte is touched once):
-// nontemporal, so the small tables every workgroup reads stay in L2.
-__device__ __forceinline__ float stream_load(const float *p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ float2 stream_load(const float2 *p) {
-  return __builtin_bit_cast(float2, __builtin_nontemporal_load((const unsigned long long *)p));
-}
-__device__ __forceinline__ void stream_store(float2 *p, float2 v) {
-  __builtin_nontemporal_store(__builtin_bit_cast(unsigned long long, v), (unsigned long long *)p);
-}
-
 template <int kTwoM>
 __device__ __forceinline__ void preprocess_row(float *ldsf, const float *__restrict__ src, long long off,
                                       int npts, int seg, int taper_size) {
