@@ -141,6 +141,13 @@ __device__ __forceinline__ void stack_fft_dir(float2 *buf, int nt, const StackFf
 constexpr int kAnalyticMaxPer = 3;
 constexpr int kLoadBatch = 8;  // loads a thread issues before it uses the first
 
+// Loads of rows a kernel reads exactly once: nontemporal, so the twiddles
+// and tables every workgroup reads stay in L2.
+__device__ __forceinline__ float stream_load(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float2 stream_load(const float2 *p) {
+  return __builtin_bit_cast(float2, __builtin_nontemporal_load((const unsigned long long *)p));
+}
+
 template <bool kBlue>
 __global__ void __launch_bounds__(kStkThreads)
 analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, StackFftDev F,
@@ -172,8 +179,8 @@ analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, StackFf
       av[u] = bv[u] = 0.f;
       if (i < total) {
         const int t = i / n, j = i - t * n;
-        av[u] = x[(r0 + 2 * t) * n + j];
-        if (2 * t + 1 < nr) bv[u] = x[(r0 + 2 * t + 1) * n + j];
+        av[u] = stream_load(x + (r0 + 2 * t) * n + j);
+        if (2 * t + 1 < nr) bv[u] = stream_load(x + (r0 + 2 * t + 1) * n + j);
       }
     }
 #pragma unroll
@@ -305,7 +312,7 @@ pws_kernel(const float2 *__restrict__ A, const float *__restrict__ weights,
 #pragma unroll
       for (int u = 0; u < kB; ++u) {
         const int i = i0 + u * kThreads;
-        if (i < nb * n) v[u] = A[(gb + i / n) * n + (i - (i / n) * n)];
+        if (i < nb * n) v[u] = stream_load(A + (gb + i / n) * n + (i - (i / n) * n));
       }
 #pragma unroll
       for (int u = 0; u < kB; ++u) {
@@ -720,7 +727,7 @@ stream_pws_close_kernel(float2 *__restrict__ S, float *__restrict__ open,
       const int t = i / ld, j = i - t * ld;
       v[u] = 0.f;
       if (i < nt * ld && j < n)
-        v[u] = rows ? rows[(r0 + t) * n + j]
+        v[u] = rows ? stream_load(rows + (r0 + t) * n + j)
                     : open[(long long)slot[r0 + t] * n + j] * (1.0f / (float)cnt[r0 + t]);
     }
 #pragma unroll
