@@ -832,9 +832,13 @@ constexpr int fused_lds_count() {
 // After fixed_fft_fwd of the packed real row: X[k], k = 0..M, to st(k, X[k])
 // (the unfold of r2c_unfold at the plan's digit-reversed positions).
 template <class FP, class St>
-__device__ __forceinline__ void fixed_r2c_emit(const float2 *lds, const float2 *fold, const St &st) {
+__device__ __forceinline__ void fixed_r2c_emit(const float2 *lds, const float2 *fold, const St &st,
+                                               int sh = 0) {
   constexpr int M = FP::kM;
-  for (int k = threadIdx.x; k <= M; k += blockDim.x) {
+  // bins start at -sh: with sh the destination row's offset within a
+  // 128-byte line (in float2), 16 consecutive lanes store one whole line
+  for (int k = (int)threadIdx.x - sh; k <= M; k += blockDim.x) {
+    if (k < 0) continue;
     const float2 a = lds[fixed_pos<FP>(k == M ? 0 : k)];
     const float2 b = cconj(lds[fixed_pos<FP>(k == 0 ? 0 : M - k)]);
     const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
@@ -1172,7 +1176,7 @@ __device__ __forceinline__ void fused_emit(float2 *lds, int seg, const float2 *_
       if (!cfinite(v)) v = make_float2(0.f, 0.f);
     }
     stream_store(dst + k, v);
-  });
+  }, (int)((reinterpret_cast<uintptr_t>(dst) >> 3) & 15));
 }
 
 template <class FPF, class FPN, bool kOut>
