@@ -457,7 +457,8 @@ def _stage(name, units, unit, sec, work, bound, peak, punit, detail):
     return {"stage": name, "value": round(units / sec, 1), "unit": unit,
             "ms_per_call": round(sec * 1e3, 3), "detail": detail,
             "roofline": {"bound": bound, "achieved": round(ach, 2), "peak": peak, "unit": punit,
-                         "frac": round(ach / peak, 4), "traffic": _stage_traffic(name)}}
+                         "frac": round(ach / peak, 4), "traffic": _stage_traffic(name),
+                         "limiter": read_limiter(STAGE_TRAFFIC, name)}}
 
 
 STAGE_CPU_SECONDS = 3.0
