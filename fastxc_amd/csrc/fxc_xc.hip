@@ -481,6 +481,16 @@ struct TileOps {
   SrcFrag<NK16> sh[2], sl[kSplit ? 2 : 1];
   RecFrag<NK16> rh[2], rl[kSplit ? 2 : 1];
 };
+// The source half and the receiver half of TileOps, held apart where
+// consecutive items share their sources.
+template <int NK16, bool kSplit>
+struct SrcOps {
+  SrcFrag<NK16> sh[2], sl[kSplit ? 2 : 1];
+};
+template <int NK16, bool kSplit>
+struct RecOps {
+  RecFrag<NK16> rh[2], rl[kSplit ? 2 : 1];
+};
 
 // Per-lane addressing of one work item (tile x 16-bin chunk).
 struct ItemAddr {
@@ -542,16 +552,54 @@ __device__ __forceinline__ void load_tile_ops(const unsigned short *__restrict__
   }
 }
 
+// Operands of bin b of an item: the source or the receiver fragments only.
 template <int kMode, int NK16, bool kSplit>
-__device__ __forceinline__ void compute_tile_bin(const TileOps<NK16, kSplit> &o, const ItemAddr &ia,
-                                                 int b, int lane_r, int lane_g, float2 *stage) {
+__device__ __forceinline__ void load_src_ops(const unsigned short *__restrict__ hi,
+                                             const unsigned short *__restrict__ lo,
+                                             const ItemAddr &ia, int b, int nsta, int nspec,
+                                             int lane_g, SrcOps<NK16, kSplit> &o) {
+  const size_t gstride = (size_t)nsta * 8;
+  int k = ia.k0 + b;
+  k = k < nspec ? k : nspec - 1;  // tail bins compute a valid bin, never stored
+  const size_t kb = (size_t)k * (16 * NK16) * nsta;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    load_src(hi + kb + ia.s_off[h], gstride, lane_g, o.sh[h]);
+    if constexpr (kSplit) load_src(lo + kb + ia.s_off[h], gstride, lane_g, o.sl[h]);
+  }
+}
+template <int kMode, int NK16, bool kSplit>
+__device__ __forceinline__ void load_rec_ops(const unsigned short *__restrict__ hi,
+                                             const unsigned short *__restrict__ lo,
+                                             const ItemAddr &ia, int b, int nsta, int nspec,
+                                             int lane_g, RecOps<NK16, kSplit> &o) {
+  const size_t gstride = (size_t)nsta * 8;
+  int k = ia.k0 + b;
+  k = k < nspec ? k : nspec - 1;
+  const size_t kb = (size_t)k * (16 * NK16) * nsta;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    load_rec(hi + kb + ia.r_off[h], gstride, lane_g, o.rh[h]);
+    if constexpr (kSplit) load_rec(lo + kb + ia.r_off[h], gstride, lane_g, o.rl[h]);
+  }
+}
+
+// One bin of the tile from its source fragments (sh, sl) and receiver
+// fragments (rh, rl), staged in LDS.
+template <int kMode, int NK16, bool kSplit>
+__device__ __forceinline__ void compute_tile_bin(const SrcFrag<NK16> (&sh)[2],
+                                                 const SrcFrag<NK16> (&sl)[kSplit ? 2 : 1],
+                                                 const RecFrag<NK16> (&rh)[2],
+                                                 const RecFrag<NK16> (&rl)[kSplit ? 2 : 1],
+                                                 const ItemAddr &ia, int b, int lane_r, int lane_g,
+                                                 float2 *stage) {
   // every B2 of the bin is formed before the first MFMA: a VALU result that
   // an MFMA reads right away costs wait states
   RecB2<NK16> b2h[2], b2l[kSplit ? 2 : 1];
 #pragma unroll
   for (int sj = 0; sj < 2; ++sj) {
-    derive_b2(o.rh[sj], b2h[sj]);
-    if constexpr (kSplit) derive_b2(o.rl[sj], b2l[sj]);
+    derive_b2(rh[sj], b2h[sj]);
+    if constexpr (kSplit) derive_b2(rl[sj], b2l[sj]);
   }
 #pragma unroll
   for (int si = 0; si < 2; ++si)
@@ -561,10 +609,10 @@ __device__ __forceinline__ void compute_tile_bin(const TileOps<NK16, kSplit> &o,
       f32x4_t ci = {0.f, 0.f, 0.f, 0.f};
       f32x4_t hr = {0.f, 0.f, 0.f, 0.f};
       f32x4_t hi = {0.f, 0.f, 0.f, 0.f};
-      mfma_bin<kMode>(o.sh[si], o.rh[sj], b2h[sj], cr, ci, hr, hi);  // hi.hi
+      mfma_bin<kMode>(sh[si], rh[sj], b2h[sj], cr, ci, hr, hi);  // hi.hi
       if constexpr (kSplit) {
-        mfma_bin<kMode>(o.sl[si], o.rh[sj], b2h[sj], cr, ci, hr, hi);  // lo.hi
-        mfma_bin<kMode>(o.sh[si], o.rl[sj], b2l[sj], cr, ci, hr, hi);  // hi.lo
+        mfma_bin<kMode>(sl[si], rh[sj], b2h[sj], cr, ci, hr, hi);  // lo.hi
+        mfma_bin<kMode>(sh[si], rl[sj], b2l[sj], cr, ci, hr, hi);  // hi.lo
       }
       if constexpr (SrcFrag<NK16>::kHalf) {
         if constexpr (SrcFrag<NK16>::kFull > 0) {
@@ -584,6 +632,12 @@ __device__ __forceinline__ void compute_tile_bin(const TileOps<NK16, kSplit> &o,
         stage[stage_index(a * kTile + c, b)] = make_float2(cr[r] * sc, ci[r] * sc);
       }
     }
+}
+
+template <int kMode, int NK16, bool kSplit>
+__device__ __forceinline__ void compute_tile_bin(const TileOps<NK16, kSplit> &o, const ItemAddr &ia,
+                                                 int b, int lane_r, int lane_g, float2 *stage) {
+  compute_tile_bin<kMode, NK16, kSplit>(o.sh, o.sl, o.rh, o.rl, ia, b, lane_r, lane_g, stage);
 }
 
 // More than 128 windows: the operand depth runs in kchunks chunks of
@@ -669,52 +723,90 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
   constexpr bool kSplit = kMode == FXC_PREC_BF16X3 || kMode == FXC_PREC_FP16X3;
   constexpr bool kPipe = NK16 * (kSplit ? 2 : 1) <= 8;
   constexpr int kBins = kBinChunk / 8;  // bins per wave and item
-  const long long nitems = (long long)ntiles * ((nchunks + kChunkGroup - 1) / kChunkGroup * kChunkGroup);
-  // next valid item at or after `it` (items whose chunk is past the end are skipped)
-  auto next_item = [&](long long it, int &tile, int &chunk) -> long long {
-    for (; it < nitems; it += gridDim.x)
-      if (tile_chunk_of(it, ntiles, nchunks, tile, chunk)) return it;
-    return nitems;
+  static_assert(kBins == 2, "two bins per wave and item");
+  // Item order.  Workgroups are dispatched round-robin over the 8 XCDs, so
+  // workgroup w runs on XCD x = w mod 8 and takes chunks 8 g + x only: each
+  // XCD's L2 then serves one chunk of every station's spectra at a time.
+  // Within an XCD's stream j = 0, 1, ... (tile j mod ntiles of chunk group
+  // j / ntiles) the U = gridDim.x / 8 workgroups of the XCD take runs of
+  // kRun consecutive j, so a workgroup's consecutive items are mostly
+  // neighbouring tiles of one tile row at one chunk, which share their
+  // source stations.  (gridDim.x is a multiple of 8.)
+  constexpr int kRun = 8;
+  const int xcd = blockIdx.x & 7, wu = blockIdx.x >> 3, nu = gridDim.x >> 3;
+  const long long nj = (long long)ntiles * ((nchunks + kChunkGroup - 1) / kChunkGroup);
+  // next valid item at or after this workgroup's k-th (items whose chunk is
+  // past the end are skipped); -1 when none is left
+  auto next_item = [&](long long k, int &tile, int &chunk) -> long long {
+    for (;; ++k) {
+      const long long j = (k / kRun) * nu * kRun + (long long)wu * kRun + k % kRun;
+      if (j >= nj) return -1;
+      const long long g = j / ntiles;
+      tile = (int)(j - g * ntiles);
+      chunk = (int)(g * kChunkGroup) + xcd;
+      if (chunk < nchunks) return k;
+    }
   };
   int tile = 0, chunk = 0;
-  long long item = next_item(blockIdx.x, tile, chunk);
-  if (item >= nitems) return;
+  long long item = next_item(0, tile, chunk);
+  if (item < 0) return;
   ItemAddr ia;
   item_addr<kMode>(tiles[tile], chunk0 + chunk, nsta, lane_r, lane_g, scale, sta_scale, ia);
-  TileOps<NK16, kSplit> cur;
   // (the chunked-depth form takes the unpipelined path)
   constexpr bool pipe = kPipe && !kChunked;
-  if constexpr (pipe) load_tile_ops<kMode>(hi, lo, ia, wave, nsta, nspec, lane_g, cur);
+  // pipelined operands: the sources of the wave's two bins (sa: bin wave,
+  // sb: bin wave + 8) stay in registers while consecutive items share their
+  // source stations and chunk; the receivers of the next bin load while the
+  // current bin's MFMAs run
+  SrcOps<NK16, kSplit> sa, sb;
+  RecOps<NK16, kSplit> rc;
+  bool sb_stale = true;  // sb does not hold this item's sources yet
+  if constexpr (pipe) {
+    load_src_ops<kMode>(hi, lo, ia, wave, nsta, nspec, lane_g, sa);
+    load_rec_ops<kMode>(hi, lo, ia, wave, nsta, nspec, lane_g, rc);
+  }
   // workspace rows of the tile's 1024 pairs, two per thread, loaded one item
   // ahead so the LDS fill after the barrier does not wait on memory
   static_assert(kTilePairs == 2 * 512, "two row entries per thread");
   int rreg0 = tile_rows[(size_t)tile * kTilePairs + tid];
   int rreg1 = tile_rows[(size_t)tile * kTilePairs + tid + 512];
-  while (item < nitems) {
+  while (item >= 0) {
     __syncthreads();  // the previous item's store phase is done with rows/stage
     rows[tid] = rreg0;
     rows[tid + 512] = rreg1;
     int ntile = 0, nchunk = 0;
-    const long long nitem = next_item(item + gridDim.x, ntile, nchunk);
+    const long long nitem = next_item(item + 1, ntile, nchunk);
     ItemAddr na = ia;
-    if (nitem < nitems) {
+    bool same_src = false;
+    if (nitem >= 0) {
       item_addr<kMode>(tiles[ntile], chunk0 + nchunk, nsta, lane_r, lane_g, scale, sta_scale, na);
       rreg0 = tile_rows[(size_t)ntile * kTilePairs + tid];
       rreg1 = tile_rows[(size_t)ntile * kTilePairs + tid + 512];
+      same_src = nchunk == chunk && tiles[ntile].src_base == tiles[tile].src_base;
     }
     if constexpr (pipe) {
-#pragma unroll
-      for (int bi = 0; bi < kBins; ++bi) {
-        TileOps<NK16, kSplit> nxt;
-        if (bi + 1 < kBins)
-          load_tile_ops<kMode>(hi, lo, ia, wave + 8 * (bi + 1), nsta, nspec, lane_g, nxt);
-        else if (nitem < nitems)
-          load_tile_ops<kMode>(hi, lo, na, wave, nsta, nspec, lane_g, nxt);
+      {  // bin wave: sources sa; load bin wave + 8's receivers (and sources)
+        RecOps<NK16, kSplit> rn;
+        load_rec_ops<kMode>(hi, lo, ia, wave + 8, nsta, nspec, lane_g, rn);
+        if (sb_stale) load_src_ops<kMode>(hi, lo, ia, wave + 8, nsta, nspec, lane_g, sb);
         // keep the scheduler from sinking the loads below the MFMAs
         __builtin_amdgcn_sched_barrier(0);
-        compute_tile_bin<kMode>(cur, ia, wave + 8 * bi, lane_r, lane_g, stage);
-        cur = nxt;
+        compute_tile_bin<kMode, NK16, kSplit>(sa.sh, sa.sl, rc.rh, rc.rl, ia, wave, lane_r, lane_g,
+                                              stage);
+        rc = rn;
       }
+      {  // bin wave + 8: sources sb; load the next item's bin wave
+        RecOps<NK16, kSplit> rn;
+        if (nitem >= 0) {
+          load_rec_ops<kMode>(hi, lo, na, wave, nsta, nspec, lane_g, rn);
+          if (!same_src) load_src_ops<kMode>(hi, lo, na, wave, nsta, nspec, lane_g, sa);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        compute_tile_bin<kMode, NK16, kSplit>(sb.sh, sb.sl, rc.rh, rc.rl, ia, wave + 8, lane_r,
+                                              lane_g, stage);
+        rc = rn;
+      }
+      sb_stale = !same_src;
     } else if constexpr (kChunked) {
 #pragma unroll 1
       for (int bi = 0; bi < kBins; ++bi)
