@@ -1057,6 +1057,12 @@ struct LagPass0 {
   static_assert(S0 % 2 == 1, "pass-0 butterflies pair up as (j, S0 - j)");
 };
 
+// A cross-spectrum value, read once: nontemporal, so the per-row tables the
+// lag kernels read stay in L2.
+__device__ __forceinline__ float2 xspec_load(const float2 *p) {
+  return __builtin_bit_cast(float2, __builtin_nontemporal_load((const unsigned long long *)p));
+}
+
 template <class FP>
 __device__ __forceinline__ void lag_r0_load(float2 (&a)[LagPass0<FP>::kPer][FP::kRadix[0]],
                                             float2 (&b)[LagPass0<FP>::kPer][FP::kRadix[0]],
@@ -1068,13 +1074,13 @@ __device__ __forceinline__ void lag_r0_load(float2 (&a)[LagPass0<FP>::kPer][FP::
     if (j < L::kPairs) {
       if (j == 0) {
 #pragma unroll
-        for (int m = 0; m < L::R0; ++m) a[it][m] = X[L::S0 * m];
-        b[it][0] = X[L::M];
+        for (int m = 0; m < L::R0; ++m) a[it][m] = xspec_load(X + L::S0 * m);
+        b[it][0] = xspec_load(X + L::M);
       } else {
 #pragma unroll
         for (int m = 0; m < L::R0; ++m) {
-          a[it][m] = X[j + L::S0 * m];
-          b[it][m] = X[L::M - j - L::S0 * m];
+          a[it][m] = xspec_load(X + j + L::S0 * m);
+          b[it][m] = xspec_load(X + L::M - j - L::S0 * m);
         }
       }
     }
