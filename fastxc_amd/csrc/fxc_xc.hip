@@ -754,13 +754,17 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
   item_addr<kMode>(tiles[tile], chunk0 + chunk, nsta, lane_r, lane_g, scale, sta_scale, ia);
   // (the chunked-depth form takes the unpipelined path)
   constexpr bool pipe = kPipe && !kChunked;
+  // unpipelined depths up to 48 windows (40 in split FP16, whose scales
+  // take the registers the 48-window sources would need) hold the sources
+  // of both bins too
+  constexpr bool kHoldSrc = !pipe && !kChunked && NK16 <= (kMode == FXC_PREC_FP16X3 ? 5 : 6);
   // pipelined operands: the sources of the wave's two bins (sa: bin wave,
   // sb: bin wave + 8) stay in registers while consecutive items share their
   // source stations and chunk; the receivers of the next bin load while the
   // current bin's MFMAs run
   SrcOps<NK16, kSplit> sa, sb;
   RecOps<NK16, kSplit> rc;
-  bool sb_stale = true;  // sb does not hold this item's sources yet
+  bool sb_stale = true;  // sb (and sa, unpipelined) do not hold this item's sources yet
   if constexpr (pipe) {
     load_src_ops<kMode>(hi, lo, ia, wave, nsta, nspec, lane_g, sa);
     load_rec_ops<kMode>(hi, lo, ia, wave, nsta, nspec, lane_g, rc);
@@ -812,6 +816,26 @@ contract_mfma_kernel(const unsigned short *__restrict__ hi, const unsigned short
       for (int bi = 0; bi < kBins; ++bi)
         compute_tile_bin_chunked<kMode, NK16, kSplit>(hi, lo, ia, wave + 8 * bi, nsta, nspec,
                                                       lane_r, lane_g, kchunks, stage);
+    } else if constexpr (kHoldSrc) {
+      // unpipelined, the sources of both bins held across items sharing
+      // them; every operand load of a bin is issued before its first MFMA
+      {
+        RecOps<NK16, kSplit> r;
+        load_rec_ops<kMode>(hi, lo, ia, wave, nsta, nspec, lane_g, r);
+        if (sb_stale) load_src_ops<kMode>(hi, lo, ia, wave, nsta, nspec, lane_g, sa);
+        __builtin_amdgcn_sched_barrier(0);
+        compute_tile_bin<kMode, NK16, kSplit>(sa.sh, sa.sl, r.rh, r.rl, ia, wave, lane_r, lane_g,
+                                              stage);
+      }
+      {
+        RecOps<NK16, kSplit> r;
+        load_rec_ops<kMode>(hi, lo, ia, wave + 8, nsta, nspec, lane_g, r);
+        if (sb_stale) load_src_ops<kMode>(hi, lo, ia, wave + 8, nsta, nspec, lane_g, sb);
+        __builtin_amdgcn_sched_barrier(0);
+        compute_tile_bin<kMode, NK16, kSplit>(sb.sh, sb.sl, r.rh, r.rl, ia, wave + 8, lane_r,
+                                              lane_g, stage);
+      }
+      sb_stale = !same_src;
     } else {
 #pragma unroll 1
       for (int bi = 0; bi < kBins; ++bi) {
