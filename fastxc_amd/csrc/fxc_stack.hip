@@ -141,6 +141,24 @@ __device__ __forceinline__ void stack_fft_dir(float2 *buf, int nt, const StackFf
 constexpr int kAnalyticMaxPer = 3;
 constexpr int kLoadBatch = 8;  // loads a thread issues before it uses the first
 
+// Row t[u] and column j[u] of flat index i0 + u step in rows of n, u < kB,
+// from one division (a division per element costs a kernel that issues VALU
+// most of its time several percent).
+template <int kB>
+__device__ __forceinline__ void flat_rows(int i0, int step, int n, int (&t)[kB], int (&j)[kB]) {
+  int tt = i0 / n, jj = i0 - tt * n;
+#pragma unroll
+  for (int u = 0; u < kB; ++u) {
+    t[u] = tt;
+    j[u] = jj;
+    jj += step;
+    while (jj >= n) {
+      jj -= n;
+      ++tt;
+    }
+  }
+}
+
 // Loads of rows a kernel reads exactly once: nontemporal, so the twiddles
 // and tables every workgroup reads stay in L2.
 __device__ __forceinline__ float stream_load(const float *p) { return __builtin_nontemporal_load(p); }
@@ -173,12 +191,14 @@ analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, StackFf
   const int total = nt * n;
   for (int i0 = tid; i0 < total; i0 += kLoadBatch * (int)blockDim.x) {
     float av[kLoadBatch], bv[kLoadBatch];
+    int tu[kLoadBatch], ju[kLoadBatch];
+    flat_rows<kLoadBatch>(i0, (int)blockDim.x, n, tu, ju);
 #pragma unroll
     for (int u = 0; u < kLoadBatch; ++u) {
       const int i = i0 + u * (int)blockDim.x;
       av[u] = bv[u] = 0.f;
       if (i < total) {
-        const int t = i / n, j = i - t * n;
+        const int t = tu[u], j = ju[u];
         av[u] = stream_load(x + (r0 + 2 * t) * n + j);
         if (2 * t + 1 < nr) bv[u] = stream_load(x + (r0 + 2 * t + 1) * n + j);
       }
@@ -187,7 +207,7 @@ analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, StackFf
     for (int u = 0; u < kLoadBatch; ++u) {
       const int i = i0 + u * (int)blockDim.x;
       if (i < total) {
-        const int t = i / n, j = i - t * n;
+        const int t = tu[u], j = ju[u];
         const float a = av[u], b = bv[u];
         lds[t * ld + j] = make_float2(a, b);
         nonfinite |= !isfinite(a) || !isfinite(b);
@@ -208,14 +228,16 @@ analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, StackFf
   const int half = (n + 1) / 2;
   if (bad) {
     for (int h = 0; h < 2; ++h) {
-      for (int i = tid; i < nt * n; i += blockDim.x) {
-        const int t = i / n, j = i - t * n, r = 2 * t + h;
-        lds[t * ld + j] = make_float2(r < nr ? x[(r0 + r) * n + j] : 0.f, 0.f);
-      }
+      for (int t = 0; t < nt; ++t)
+        for (int j = tid; j < n; j += blockDim.x) {
+          const int r = 2 * t + h;
+          lds[t * ld + j] = make_float2(r < nr ? x[(r0 + r) * n + j] : 0.f, 0.f);
+        }
       __syncthreads();
       stack_fft_dir<-1, kBlue>(lds, nt, F, twl);
-      for (int i = tid; i < nt * n; i += blockDim.x) {
-        const int t = i / n, k = i - t * n, r = 2 * t + h;
+      for (int t = 0; t < nt; ++t)
+      for (int k = tid; k < n; k += blockDim.x) {
+        const int r = 2 * t + h;
         float2 v = make_float2(0.f, 0.f);
         if (k >= 1 && k < half) {
           const float2 z = lds[t * ld + stack_pos(F, k)];
@@ -237,39 +259,33 @@ analytic_kernel(const float *__restrict__ x, long long rows, int per_wg, StackFf
     if (mx > 0.f && isfinite(mx)) (void)frexpf(mx, &e);
     sc[r] = ldexpf(1.f, -e);
   }
-  for (int i = tid; i < nt * n; i += blockDim.x) {
-    const int t = i / n;
-    float s0 = 1.f, s1 = 1.f;
 #pragma unroll
-    for (int r = 0; r < kAnalyticMaxPer; ++r)
-      if (r == t) {
-        s0 = sc[2 * r];
-        s1 = sc[2 * r + 1];
-      }
-    const float2 v = lds[t * ld + (i - t * n)];
-    lds[t * ld + (i - t * n)] = make_float2(v.x * s0, v.y * s1);
+  for (int t = 0; t < kAnalyticMaxPer; ++t) {
+    if (t >= nt) break;
+    const float s0 = sc[2 * t], s1 = sc[2 * t + 1];
+    for (int j = tid; j < n; j += blockDim.x) {
+      const float2 v = lds[t * ld + j];
+      lds[t * ld + j] = make_float2(v.x * s0, v.y * s1);
+    }
   }
   __syncthreads();
   stack_fft_dir<-1, kBlue>(lds, nt, F, twl);
-  for (int i = tid; i < nt * n; i += blockDim.x) {
-    const int t = i / n, k = i - t * n;
-    float2 a0 = make_float2(0.f, 0.f), a1 = make_float2(0.f, 0.f);
-    if (k >= 1 && k < half) {
-      const float2 zk = lds[t * ld + stack_pos(F, k)];
-      const float2 zm = lds[t * ld + stack_pos(F, n - k)];
-      a0 = make_float2(zk.x + zm.x, zk.y - zm.y);      // Z[k] + conj Z[n-k]
-      const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);  // Z[k] - conj Z[n-k]
-      a1 = make_float2(d.y, -d.x);                     // -i d
-    }
-    float u0 = 1.f, u1 = 1.f;
 #pragma unroll
-    for (int r = 0; r < kAnalyticMaxPer; ++r)
-      if (r == t) {
-        u0 = 1.f / sc[2 * r];
-        u1 = 1.f / sc[2 * r + 1];
+  for (int t = 0; t < kAnalyticMaxPer; ++t) {
+    if (t >= nt) break;
+    const float u0 = 1.f / sc[2 * t], u1 = 1.f / sc[2 * t + 1];
+    for (int k = tid; k < n; k += blockDim.x) {
+      float2 a0 = make_float2(0.f, 0.f), a1 = make_float2(0.f, 0.f);
+      if (k >= 1 && k < half) {
+        const float2 zk = lds[t * ld + stack_pos(F, k)];
+        const float2 zm = lds[t * ld + stack_pos(F, n - k)];
+        a0 = make_float2(zk.x + zm.x, zk.y - zm.y);              // Z[k] + conj Z[n-k]
+        const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);  // Z[k] - conj Z[n-k]
+        a1 = make_float2(d.y, -d.x);                             // -i d
       }
-    A[(r0 + 2 * t) * n + k] = make_float2(a0.x * u0, a0.y * u0);
-    if (2 * t + 1 < nr) A[(r0 + 2 * t + 1) * n + k] = make_float2(a1.x * u1, a1.y * u1);
+      A[(r0 + 2 * t) * n + k] = make_float2(a0.x * u0, a0.y * u0);
+      if (2 * t + 1 < nr) A[(r0 + 2 * t + 1) * n + k] = make_float2(a1.x * u1, a1.y * u1);
+    }
   }
 }
 
@@ -309,15 +325,17 @@ pws_kernel(const float2 *__restrict__ A, const float *__restrict__ weights,
     constexpr int kB = kLoadBatch / 2;
     for (int i0 = tid; i0 < nb * n; i0 += kB * kThreads) {
       float2 v[kB];
+      int tu[kB], ju[kB];
+      flat_rows<kB>(i0, kThreads, n, tu, ju);
 #pragma unroll
       for (int u = 0; u < kB; ++u) {
         const int i = i0 + u * kThreads;
-        if (i < nb * n) v[u] = stream_load(A + (gb + i / n) * n + (i - (i / n) * n));
+        if (i < nb * n) v[u] = stream_load(A + (gb + tu[u]) * n + ju[u]);
       }
 #pragma unroll
       for (int u = 0; u < kB; ++u) {
         const int i = i0 + u * kThreads;
-        if (i < nb * n) buf[(i / n) * ld + (i - (i / n) * n)] = v[u];
+        if (i < nb * n) buf[tu[u] * ld + ju[u]] = v[u];
       }
     }
     __syncthreads();
@@ -413,12 +431,13 @@ tfpws_voice_kernel(const float2 *__restrict__ Ag, const float2 *__restrict__ Ali
       const int nb = bi < nbat ? (int)std::min<long long>(nbatch, g1 - gb) : 0;
       const int rows = nb + (bi == total - 1 ? 1 : 0);  // + linear stack in the last
       __syncthreads();
-      for (int i = tid; i < rows * n; i += kThreads) {
-        const int row = i / n, c = i - row * n;
-        int src = f + c;
-        src = src >= n ? src - n : src;
-        const float2 a = row == nb ? Al[src] : Ag[(gb + row) * n + src];
-        buf[row * ld + c] = cscale(a, gauss[c]);
+      for (int row = 0; row < rows; ++row) {
+        const float2 *arow = row == nb ? Al : Ag + (gb + row) * n;
+        for (int c = tid; c < n; c += kThreads) {
+          int src = f + c;
+          src = src >= n ? src - n : src;
+          buf[row * ld + c] = cscale(arow[src], gauss[c]);
+        }
       }
       __syncthreads();
       stack_fft(buf, rows, F, twl, 1.f);
@@ -509,12 +528,12 @@ __device__ __forceinline__ void tfpws_voice_weight(float (&c)[kPer], int v, cons
   for (long long gb = g0; gb < g1; gb += nrows) {
     const int nb = (int)std::min<long long>(nrows, g1 - gb);
     __syncthreads();
-    for (int i = tid; i < nb * n; i += kThreads) {
-      const int row = i / n, cc = i - row * n;
-      int src = v + cc;
-      src = src >= n ? src - n : src;
-      buf[row * ld + cc] = cscale(Ag[(gb + row) * n + src], gauss[cc]);
-    }
+    for (int row = 0; row < nb; ++row)
+      for (int cc = tid; cc < n; cc += kThreads) {
+        int src = v + cc;
+        src = src >= n ? src - n : src;
+        buf[row * ld + cc] = cscale(Ag[(gb + row) * n + src], gauss[cc]);
+      }
     __syncthreads();
     stack_fft(buf, nb, F, twl, 1.f);
 #pragma unroll
@@ -721,10 +740,12 @@ stream_pws_close_kernel(float2 *__restrict__ S, float *__restrict__ open,
   const int nt = (int)std::min<long long>(per_wg, nclose - r0);
   for (int i0 = tid; i0 < nt * ld; i0 += kLoadBatch * kStkThreads) {
     float v[kLoadBatch];
+    int tu[kLoadBatch], ju[kLoadBatch];
+    flat_rows<kLoadBatch>(i0, kStkThreads, ld, tu, ju);
 #pragma unroll
     for (int u = 0; u < kLoadBatch; ++u) {
       const int i = i0 + u * kStkThreads;
-      const int t = i / ld, j = i - t * ld;
+      const int t = tu[u], j = ju[u];
       v[u] = 0.f;
       if (i < nt * ld && j < n)
         v[u] = rows ? stream_load(rows + (r0 + t) * n + j)
