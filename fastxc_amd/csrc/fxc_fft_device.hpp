@@ -306,7 +306,9 @@ __device__ __forceinline__ void dif_pass(float2 *buf, int ntrans, const FftPlanD
       asm volatile("" : "+v"(off));
       float2 *obase = buf + off;
       const int e = j * tstep;
-      const int eR = (int)(((long long)e * R) % n);
+      // j < stride = span / R and tstep = n / span, so e < n / R: R e needs
+      // no reduction modulo n
+      const int eR = e * R;
       int eq = 0;
       dft_odd_stream<R>(v, sgn, [&](int q, float2 y) {
         if (q != 0 && j > 0) {
