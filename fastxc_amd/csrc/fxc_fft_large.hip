@@ -17,6 +17,27 @@ constexpr int kLargeThreads = 512;
 constexpr int kLargeCols = 64;  // columns (rows) per workgroup in each pass
 
 // Pass 1: transforms of the columns n2 in [c0, c0 + C) of each row,
+// Quotient q and remainder r of i = q d + r along i = i0, i0 + step, ...:
+// one division at the start, then an add and a compare per step (a runtime
+// integer division per element costs a few dozen VALU instructions).
+struct FlatWalk {
+  int q, r, dq, dr, d;
+  __device__ __forceinline__ FlatWalk(int i0, int step, int d_) : d(d_) {
+    q = i0 / d;
+    r = i0 - q * d;
+    dq = step / d;
+    dr = step - dq * d;
+  }
+  __device__ __forceinline__ void next() {
+    q += dq;
+    r += dr;
+    if (r >= d) {
+      r -= d;
+      ++q;
+    }
+  }
+};
+
 // Y[k1][n2] = w_M^(n2 k1) sum_n1 x[n1 N2 + n2] w_N1^(n1 k1).
 template <unsigned kMask>
 __global__ void __launch_bounds__(kLargeThreads)
@@ -31,18 +52,21 @@ large_cols_kernel(const float2 *__restrict__ in, long long in_dist, float2 *__re
   const int c0 = blockIdx.x * C;
   const int nc = min(C, N2 - c0);
   const float2 *x = in + row * in_dist;
-  for (int i = threadIdx.x; i < C * N1; i += blockDim.x) {
-    const int n1 = i / C, c = i - n1 * C;
+  FlatWalk wl(threadIdx.x, blockDim.x, C);
+  for (int i = threadIdx.x; i < C * N1; i += blockDim.x, wl.next()) {
+    const int n1 = wl.q, c = wl.r;
     lds[c * N1 + n1] = c < nc ? x[(long long)n1 * N2 + c0 + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   lds_fft<kMask>(lds, C, p1, twl, sgn);
   float2 *y = tmp + row * (long long)M;
-  for (int i = threadIdx.x; i < nc * N1; i += blockDim.x) {
-    const int k1 = i / nc, c = i - k1 * nc;
+  FlatWalk ws(threadIdx.x, blockDim.x, nc);
+  for (int i = threadIdx.x; i < nc * N1; i += blockDim.x, ws.next()) {
+    const int k1 = ws.q, c = ws.r;
     const int n2 = c0 + c;
     const float2 v = lds[c * N1 + fft_pos(p1, k1)];
-    y[(long long)k1 * N2 + n2] = cmul(v, twM[(int)(((long long)n2 * k1) % M)]);
+    // n2 < N2 and k1 < N1, so n2 k1 < M needs no reduction
+    y[(long long)k1 * N2 + n2] = cmul(v, twM[n2 * k1]);
   }
 }
 
@@ -61,15 +85,17 @@ large_rows_kernel(const float2 *__restrict__ tmp, float2 *__restrict__ out, long
   const int k0 = blockIdx.x * C;
   const int nc = min(C, N1 - k0);
   const float2 *y = tmp + row * (long long)M;
-  for (int i = threadIdx.x; i < C * N2; i += blockDim.x) {
-    const int c = i / N2, n2 = i - c * N2;
+  FlatWalk wl(threadIdx.x, blockDim.x, N2);
+  for (int i = threadIdx.x; i < C * N2; i += blockDim.x, wl.next()) {
+    const int c = wl.q, n2 = wl.r;
     lds[i] = c < nc ? y[(long long)(k0 + c) * N2 + n2] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   lds_fft<kMask>(lds, C, p2, twl, sgn);
   float2 *o = out + row * out_dist;
-  for (int i = threadIdx.x; i < nc * N2; i += blockDim.x) {
-    const int k2 = i / nc, c = i - k2 * nc;
+  FlatWalk ws(threadIdx.x, blockDim.x, nc);
+  for (int i = threadIdx.x; i < nc * N2; i += blockDim.x, ws.next()) {
+    const int k2 = ws.q, c = ws.r;
     o[k0 + c + (long long)N1 * k2] = cscale(lds[c * N2 + fft_pos(p2, k2)], scale);
   }
 }
