@@ -251,11 +251,12 @@ __device__ __forceinline__ void store_stream2(float2 *dst, float2 a, float2 b) {
   __builtin_nontemporal_store(v, (u64x2 *)dst);
 }
 
-// Work item -> (tile, bin chunk).  Workgroups are dispatched round-robin over
-// the 8 XCDs, so 8 consecutive items take 8 consecutive bin chunks of one
-// tile: each XCD's L2 then serves one chunk of every station's spectra, and
-// each pair row receives 8 x 128 contiguous bytes at once.  A persistent
-// grid whose size is a multiple of 8 keeps item i on XCD i mod 8 as well.
+// Work item -> (tile, bin chunk) of the FP32 contraction.  Workgroups are
+// dispatched round-robin over the 8 XCDs, so 8 consecutive items take 8
+// consecutive bin chunks of one tile: each XCD's L2 then serves one chunk of
+// every station's spectra, and each pair row receives 8 x 128 contiguous
+// bytes at once.  (The MFMA contraction keeps the same chunk-per-XCD rule
+// with its own order of tiles; see contract_mfma_kernel.)
 constexpr int kChunkGroup = 8;
 
 __device__ __forceinline__ bool tile_chunk_of(long long item, int ntiles, int nchunks, int &tile,
@@ -699,10 +700,13 @@ __device__ __forceinline__ void compute_tile_bin_chunked(const unsigned short *_
 // One work item = one 32x32 station tile x 16 bins; 8 waves, each computing
 // the whole tile (four 16x16 MFMA blocks) for two of the 16 bins, so every
 // station fragment is read once per workgroup.  Persistent: a workgroup walks
-// items blockIdx.x, +gridDim.x, ...  With small K each wave software-
-// pipelines its bins: the operands of the next bin -- of this item or of the
-// next one -- are loaded before the current bin's MFMAs, so load latency
-// overlaps the MFMAs, the LDS staging and the previous item's stores.
+// runs of neighbouring tiles at one bin chunk (the order is set out at the
+// top of the kernel), and each wave keeps the source operands of its two
+// bins in registers while consecutive tiles share their source stations.
+// With small K each wave software-pipelines its bins: the receiver operands
+// of the next bin -- of this item or of the next one -- are loaded before the
+// current bin's MFMAs, so load latency overlaps the MFMAs, the LDS staging
+// and the previous item's stores.
 // The LDS staging tile admits one workgroup (8 waves) per CU, 2 waves per
 // SIMD; waves_per_eu tells the scheduler so, and it then keeps a bin's
 // operand loads in flight together instead of trimming registers for an
