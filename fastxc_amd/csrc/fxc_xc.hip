@@ -1821,6 +1821,12 @@ static int launch_mfma_kh(dim3 grid, size_t lds, hipStream_t st, const unsigned 
   unsigned g = (unsigned)std::max(8, ncu / 8 * 8);
   if (const char *env = getenv("FXC_XC_MFMA_GRID")) g = (unsigned)std::max(8, atoi(env) / 8 * 8);
   const dim3 grid1(std::min(g, grid.x));
+  // the kernel's item order gives each XCD (blockIdx.x mod 8) its own chunks
+  // and covers every item only when the grid is a multiple of 8
+  if (grid1.x % 8 != 0) {
+    set_error("MFMA contraction: grid %u is not a multiple of 8", grid1.x);
+    return FXC_ERR_UNSUPPORTED;
+  }
   FXC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
   hipLaunchKernelGGL(kern, grid1, dim3(512), lds, st, hi, lo, nsta, nspec, sc, tiles, trows, dst,
