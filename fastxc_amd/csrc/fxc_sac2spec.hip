@@ -629,12 +629,15 @@ s2s_mf_band_kernel(const float2 *__restrict__ base, long long bpitch, RealTables
 }
 
 // total += runabs-normalised band signal (normalization.cu:371-385).
+// Workgroup b covers samples [(b mod nbr) blockDim, +blockDim) of row
+// b / nbr, so the row comes from one division per workgroup rather than
+// one per sample.
 __global__ void s2s_mf_accum_kernel(const float *__restrict__ xb, const float *__restrict__ w,
-                                    int seg, int nch, long long rows, float *__restrict__ total) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows * seg) return;
-  const long long row = i / seg;
-  const int t = (int)(i - row * seg);
+                                    int seg, int nch, int nbr, float *__restrict__ total) {
+  const int row = (int)(blockIdx.x / (unsigned)nbr);
+  const int t = (int)(blockIdx.x - (unsigned)row * nbr) * blockDim.x + threadIdx.x;
+  if (t >= seg) return;
+  const long long i = (long long)row * seg + t;
   const long long frame = row / nch;
   float v = xb[i];
   const float d = w[frame * seg + t];
@@ -1789,9 +1792,9 @@ static int s2s_process_large(fxc_s2s_plan *p, const float *d_traces, int nfiles,
                        st));
       FXC_TRY(launch_weight(false, p->d_xb, c.seg, c.seg, c.num_ch, p->mf_win[i], frames, p->d_w,
                             c.seg, st));
-      const size_t n = (size_t)rows * c.seg;
-      hipLaunchKernelGGL(s2s_mf_accum_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st,
-                         p->d_xb, p->d_w, c.seg, c.num_ch, rows, p->d_total);
+      const int nbr = (int)ceil_div(c.seg, 256);
+      hipLaunchKernelGGL(s2s_mf_accum_kernel, dim3((unsigned)(rows * nbr)), dim3(256), 0, st,
+                         p->d_xb, p->d_w, c.seg, c.num_ch, nbr, p->d_total);
       FXC_HIP(hipGetLastError());
     }
     FXC_HIP(hipMemcpyAsync(p->d_x, p->d_total, (size_t)rows * c.seg * sizeof(float),
@@ -1927,9 +1930,9 @@ extern "C" int fxc_s2s_process(fxc_s2s_plan *p, const float *d_traces, int nfile
       FXC_HIP(hipGetLastError());
       FXC_TRY(launch_weight(false, p->d_xb, c.seg, c.seg, c.num_ch, p->mf_win[i], frames, p->d_w,
                             c.seg, st));
-      const size_t n = (size_t)rows * c.seg;
-      hipLaunchKernelGGL(s2s_mf_accum_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st,
-                         p->d_xb, p->d_w, c.seg, c.num_ch, rows, p->d_total);
+      const int nbr = (int)ceil_div(c.seg, 256);
+      hipLaunchKernelGGL(s2s_mf_accum_kernel, dim3((unsigned)(rows * nbr)), dim3(256), 0, st,
+                         p->d_xb, p->d_w, c.seg, c.num_ch, nbr, p->d_total);
       FXC_HIP(hipGetLastError());
     }
     FXC_HIP(hipMemcpyAsync(p->d_x, p->d_total, (size_t)rows * c.seg * sizeof(float),
