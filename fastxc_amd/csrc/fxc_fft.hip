@@ -67,16 +67,14 @@ c2c_batch_kernel(const float2 *__restrict__ in, long long in_dist, float2 *__res
   const int n = plan.n;
   float2 *twl = lds + (size_t)per_wg * n;
   for (int i = threadIdx.x; i < kTwFine + plan.ncoarse; i += blockDim.x) twl[i] = tw[i];
-  for (int i = threadIdx.x; i < nt * n; i += blockDim.x) {
-    const int t = i / n, j = i - t * n;
-    lds[i] = in[(t0 + t) * in_dist + j];
-  }
+  FlatWalk wl(threadIdx.x, blockDim.x, n);
+  for (int i = threadIdx.x; i < nt * n; i += blockDim.x, wl.next())
+    lds[i] = in[(t0 + wl.q) * in_dist + wl.r];
   __syncthreads();
   lds_fft<kMask>(lds, nt, plan, twl, sgn);
-  for (int i = threadIdx.x; i < nt * n; i += blockDim.x) {
-    const int t = i / n, j = i - t * n;
-    out[(t0 + t) * out_dist + j] = cscale(lds[t * n + fft_pos(plan, j)], scale);
-  }
+  FlatWalk ws(threadIdx.x, blockDim.x, n);
+  for (int i = threadIdx.x; i < nt * n; i += blockDim.x, ws.next())
+    out[(t0 + ws.q) * out_dist + ws.r] = cscale(lds[ws.q * n + fft_pos(plan, ws.r)], scale);
 }
 
 __global__ void __launch_bounds__(kFftThreads)

@@ -277,6 +277,27 @@ __device__ __forceinline__ void apply_twiddles(float2 *v, const float2 *tw, int 
   }
 }
 
+// Quotient q and remainder r of i = q d + r along i = i0, i0 + step, ...:
+// one division at the start, then an add and a compare per step (a runtime
+// integer division per element costs a few dozen VALU instructions).
+struct FlatWalk {
+  int q, r, dq, dr, d;
+  __device__ __forceinline__ FlatWalk(int i0, int step, int d_) : d(d_) {
+    q = i0 / d;
+    r = i0 - q * d;
+    dq = step / d;
+    dr = step - dq * d;
+  }
+  __device__ __forceinline__ void next() {
+    q += dq;
+    r += dr;
+    if (r >= d) {
+      r -= d;
+      ++q;
+    }
+  }
+};
+
 // One in-place DIF pass (butterfly, then twiddle) over ntrans transforms.
 template <int R>
 __device__ __forceinline__ void dif_pass(float2 *buf, int ntrans, const FftPlanDev &plan, int p,

@@ -17,27 +17,6 @@ constexpr int kLargeThreads = 512;
 constexpr int kLargeCols = 64;  // columns (rows) per workgroup in each pass
 
 // Pass 1: transforms of the columns n2 in [c0, c0 + C) of each row,
-// Quotient q and remainder r of i = q d + r along i = i0, i0 + step, ...:
-// one division at the start, then an add and a compare per step (a runtime
-// integer division per element costs a few dozen VALU instructions).
-struct FlatWalk {
-  int q, r, dq, dr, d;
-  __device__ __forceinline__ FlatWalk(int i0, int step, int d_) : d(d_) {
-    q = i0 / d;
-    r = i0 - q * d;
-    dq = step / d;
-    dr = step - dq * d;
-  }
-  __device__ __forceinline__ void next() {
-    q += dq;
-    r += dr;
-    if (r >= d) {
-      r -= d;
-      ++q;
-    }
-  }
-};
-
 // Y[k1][n2] = w_M^(n2 k1) sum_n1 x[n1 N2 + n2] w_N1^(n1 k1).
 template <unsigned kMask>
 __global__ void __launch_bounds__(kLargeThreads)
